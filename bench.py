@@ -1,0 +1,261 @@
+"""Benchmark: 3-D compressible Euler, degree 7 (Nq = 8), periodic 32^3 hex brick
+over [0,10]^3 (16.8 M nodes, 83.9 M DOF), LSRK54 on one MI355X.
+
+A step is one LSRK54 time step = 5 fused stages (dq = a*dq + L(q); q += b*dt*dq),
+each one launch of the fused DG stage kernel.  DOF-updates/s counts
+Ns x nodes x stage evaluations per second (inputs resident in HBM).
+
+Prints ONE JSON line.  --gpus N with N > 1 exits with an explanation: this
+build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "raven.jl_amd"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+NQ, NCELL, L = 8, 32, 10.0
+GAMMA = 1.4
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c2_points(R):
+    """Node coordinates of the 32^3 brick (library generator, host only)."""
+    cell = R.LobattoCell(NQ, NQ, NQ)
+    gm = R.GridManager(cell, R.brick([np.linspace(0, L, NCELL + 1)] * 3, (True,) * 3))
+    desc = gm.brick_desc()
+    from ravenhip import _lib as RL
+    K = RL.lib().rv_brick_numcells(RL.ctypes.byref(desc))
+    pts = np.zeros((K, 3, NQ ** 3))
+    RL.check(RL.lib().rv_brick_arrays(RL.ctypes.byref(desc), RL.ptr(pts, RL.ctypes.c_double),
+                                      None, None, None, None, None, None, None))
+    return gm, pts
+
+
+def c2_initial_state(pts, seed=1234):
+    """Isentropic vortex (test/balancelaws/euler/isentropicvortex.jl:18-34)
+    extruded in z, centred at (5, 5), plus seeded uniform density noise 1e-3."""
+    import problems as PB
+    rng = np.random.default_rng(seed)
+    noise = 1e-3 * rng.uniform(-1.0, 1.0, pts[:, 0, :].shape)
+    return PB.isentropic_vortex_3d(pts, 0.0, gamma=GAMMA, centre=(5.0, 5.0), noise=noise)
+
+
+def lsrk54():
+    a = [0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238,
+         -3550918686646 / 2091501179385, -1275806237668 / 842570457699]
+    b = [1432997174477 / 9575080441755, 5161836677717 / 13612068292357,
+         1720146321549 / 2090206949498, 3134564353537 / 4481467310338,
+         2277821191437 / 14882151754819]
+    return a, b
+
+
+def run_gpu(R, torch, gm, q0, dtype, steps, warmup, dt):
+    law = R.EulerLaw(3, GAMMA)
+    dg = R.DGSEM.brick(gm, law, dtype=dtype)
+    tdt = dg.torch_dtype
+    q = torch.from_numpy(q0).to(tdt).cuda().reshape(dg.K, dg.ns, dg.Np).contiguous()
+    alt = torch.empty_like(q)
+    dq = torch.zeros_like(q)
+    a, b = lsrk54()
+    if dtype == "f32":
+        a = [float(np.float32(v)) for v in a]
+        bdt = [float(np.float32(np.float32(v) * np.float32(dt))) for v in b]
+    else:
+        bdt = [v * dt for v in b]
+    bufs = [q, alt]
+    cur = [0]
+
+    def step(evs=None):
+        for s in range(5):
+            if evs is not None:
+                evs[s][0].record()
+            dg.lsrk_stage(bufs[1 - cur[0]], bufs[cur[0]], dq, a[s], bdt[s])
+            if evs is not None:
+                evs[s][1].record()
+            cur[0] = 1 - cur[0]
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    # timed region: exactly `steps` steps, events around every launch
+    evs = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(5)]
+           for _ in range(steps)]
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for k in range(steps):
+        step(evs[k])
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    elapsed = start.elapsed_time(end) / 1e3
+    durs = np.array([[evs[k][s][0].elapsed_time(evs[k][s][1]) / 1e3 for s in range(5)]
+                     for k in range(steps)])
+    info = dg.info
+    qf = bufs[cur[0]]
+    ints = dg.integrals(qf)
+    res = dict(dg=dg, elapsed=elapsed, wall=wall, durs=durs, info=info,
+               q=qf.double().cpu().numpy().reshape(-1), integrals=ints,
+               integrals0=None)
+    # conservation reference: integrals of the initial state
+    q0d = torch.from_numpy(q0).to(tdt).cuda().reshape(dg.K, dg.ns, dg.Np).contiguous()
+    res["integrals0"] = dg.integrals(q0d)
+    del q0d
+    return res
+
+
+def cpu_baseline(q0, dt, budget_s):
+    """The CPU oracle (a C restatement of Raven's CPU path, OpenMP) on the
+    same 32^3 Nq=8 grid and state, timed for whole LSRK54 stages."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    import problems as PB
+    g = O.BrickGrid(NQ, [PB.julia_range(0, 10, NCELL + 1)] * 3, (True,) * 3)
+    dg = O.DG(g, "euler", gamma=GAMMA)
+    dg.set_threads(cores)
+    q = q0.copy()
+    dq = np.zeros_like(q)
+    t0 = time.perf_counter()
+    dg.rhs(dq, q, 0.0, increment=True)   # one stage's right-hand side
+    rhs_s = time.perf_counter() - t0
+    dofs = g.K * g.Np * 5
+    stages, t_total = 1, rhs_s
+    if 5 * rhs_s * 1.3 < budget_s:
+        q = q0.copy()
+        dq[:] = 0
+        t0 = time.perf_counter()
+        dg.lsrk_step(q, dq, 0.0, dt)
+        t_total = time.perf_counter() - t0
+        stages = 5
+        sample = "one LSRK54 step (5 stages) of the full 32^3 Nq=8 FP64 workload"
+    else:
+        sample = "one right-hand side of the full 32^3 Nq=8 FP64 workload"
+    return {"value": dofs * stages / t_total, "unit": "DOF-updates/s", "cores": cores,
+            "kind": "port", "sample": sample, "seconds": round(t_total, 3),
+            "note": "C restatement of Raven's CPU path (oracle/), not Raven itself"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is "
+              "not built in this repository", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ravenhip as R
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+
+    log("bench: building the 32^3 Nq=8 brick and initial state")
+    gm, pts = c2_points(R)
+    q0 = c2_initial_state(pts)
+    h = L / NCELL
+    dt = 0.25 * h / (NQ - 1) / (1.0 + np.sqrt(GAMMA))
+    K = pts.shape[0]
+    nodes = K * NQ ** 3
+    dofs = nodes * 5
+    del pts
+
+    log(f"bench: FP64 {args.warmup} warmup + {args.steps} timed LSRK54 steps")
+    r64 = run_gpu(R, torch, gm, q0, "f64", args.steps, args.warmup, dt)
+    stages = 5 * args.steps
+    value = dofs * stages / r64["elapsed"]
+    info = r64["info"]
+    # algorithmic bytes: stage 1 (a = 0) reads q, writes dq and q; the others
+    # also read dq; affine geometry is per cell
+    bytes_step = info.stage_bytes_first + 4 * info.stage_bytes
+    kavg = float(np.mean(r64["durs"]))
+    achieved = bytes_step / 5 / kavg / 1e9
+    out = {
+        "metric": "3D Euler p=7 DOF-updates/s + %HBM peak (FP64 vs FP32); served p50/p99 latency",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3-D Euler (5 states, gamma 1.4), periodic 32^3 hex brick on [0,10]^3, "
+                        "degree 7 (Nq=8), isentropic vortex + 1e-3 seeded density noise, "
+                        "LSRK54, Rusanov flux, weak form",
+            "cells": int(K), "nodes": int(nodes), "dof": int(dofs), "dt": dt,
+            "step": "one LSRK54 step = 5 fused stage launches",
+            "device": torch.cuda.get_device_name(0),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "dg_stage_kernel<3,8,Euler,double,affine,Rusanov>",
+            "avg_kernel_ms": kavg * 1e3,
+            "algorithmic_bytes_per_launch": bytes_step / 5,
+        },
+    }
+    if not args.no_fp32:
+        log("bench: FP32 run")
+        r32 = run_gpu(R, torch, gm, q0, "f32", args.steps, args.warmup, dt)
+        v32 = dofs * stages / r32["elapsed"]
+        k32 = float(np.mean(r32["durs"]))
+        i32 = r32["info"]
+        b32 = (i32.stage_bytes_first + 4 * i32.stage_bytes) / 5
+        q64, q32 = r64["q"], r32["q"]
+        err = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+        m0, m64, m32 = r64["integrals0"], r64["integrals"], r32["integrals"]
+        out["fp32"] = {
+            "value": v32, "unit": "DOF-updates/s", "speedup_vs_fp64": v32 / value,
+            "ms_per_step": r32["elapsed"] / args.steps * 1e3,
+            "avg_kernel_ms": k32 * 1e3, "achieved_GBs": b32 / k32 / 1e9,
+            "frac": b32 / k32 / 1e9 / HBM_PEAK_GBS,
+            "rel_l2_vs_fp64_after_steps": err,
+            "mass_drift_fp64": float((m64[0] - m0[0]) / m0[0]),
+            "mass_drift_fp32": float((m32[0] - m0[0]) / m0[0]),
+            "energy_drift_fp64": float((m64[4] - m0[4]) / m0[4]),
+            "energy_drift_fp32": float((m32[4] - m0[4]) / m0[4]),
+            "steps_compared": args.warmup + args.steps,
+        }
+        del r32
+    del r64
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(q0, dt, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

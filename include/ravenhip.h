@@ -1,0 +1,199 @@
+/*
+ * ravenhip.h -- C ABI of the MI355X (gfx950) DGSEM right-hand side and
+ * low-storage Runge-Kutta library.
+ *
+ * Drop-in boundary for Raven.jl's `dg(dq, q, t; increment)` call
+ * (src/balancelaws/dgsem.jl:84-126) and the LSRK stage loop around it
+ * (src/balancelaws/odesolvers.jl:9-61).  Julia reaches it through `ccall`;
+ * every argument is a plain pointer or scalar.
+ *
+ * Conventions
+ *  - Arrays use Raven's memory layout (Julia column-major):
+ *      state q, dq           (Nq^dim, Ns, K)            parent(GridArray), field axis dim+1
+ *      volume metrics        (Nq^dim, dim*dim+2, K)      (dRdX column-major, J, wJ)
+ *      surface metrics       (dim+2, 2dim*Nq^(dim-1), K) field first: (n, sJ, wsJ)
+ *      vmapM, vmapP          (2dim*Nq^(dim-1), K) int64, 1-based
+ *      boundary codes        (2dim, K) int64
+ *      D                     (Nq, Nq) column-major, D[a + Nq*b] = l_b'(x_a)
+ *  - Host arrays passed to rv_create are read during the call only.
+ *  - State pointers passed to compute calls are DEVICE pointers (HIP device
+ *    memory of the current device) holding FP64 or FP32 values as the
+ *    descriptor's dtype says.  Results are complete when the handle's stream
+ *    has been synchronised (rv_synchronize) or ordered after by the caller.
+ *  - Every function returns RV_OK (0) or a negative RV_E* code; the message of
+ *    the last failure is in rv_last_error(h) (or rv_last_error(NULL) when no
+ *    handle exists).  The library never aborts the process.
+ *  - A handle is not reentrant; distinct handles may be used from different
+ *    host threads.
+ */
+#ifndef RAVENHIP_H
+#define RAVENHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RAVENHIP_ABI_VERSION 1
+
+enum {
+    RV_OK = 0,
+    RV_E_ARG = -1,       /* invalid argument or descriptor */
+    RV_E_HIP = -2,       /* HIP runtime error */
+    RV_E_NOMEM = -3,     /* device allocation failed */
+    RV_E_UNSUPPORTED = -4, /* no kernel built for this (dim, Nq, law, dtype) */
+    RV_E_MAPS = -5,      /* face maps could not be encoded */
+    RV_E_STATE = -6      /* call not valid in the handle's state */
+};
+
+/* law ids (src/balancelaws/balancelaws/{advection,euler}.jl) */
+enum { RV_LAW_ADVECTION = 1, RV_LAW_EULER = 2 };
+/* surface numerical flux (src/balancelaws/numericalfluxes.jl:39-50) */
+enum { RV_FLUX_RUSANOV = 1, RV_FLUX_CENTRAL = 2 };
+/* state precision: FP64, or FP32 state/flux arithmetic with FP64 geometry */
+enum { RV_F64 = 1, RV_F32 = 2 };
+/* geometry handling: AUTO detects affine elements (per-element scale factors)
+ * and otherwise keeps per-node FP64 metrics; GENERAL forces the latter. */
+enum { RV_GEOM_AUTO = 0, RV_GEOM_AFFINE = 1, RV_GEOM_GENERAL = 2 };
+/* LSRK schemes (odesolvers.jl:63-71) */
+enum { RV_LSRK54 = 54, RV_LSRK144 = 144 };
+
+typedef struct rv_handle rv_handle;
+
+/* Descriptor of one DGSEM operator: the fields of Raven's `DGSEM` struct
+ * (dgsem.jl:19-30) plus the grid arrays it reads. */
+typedef struct rv_desc {
+    int32_t abi_version;   /* RAVENHIP_ABI_VERSION */
+    int32_t dim;           /* 1, 2 or 3 */
+    int32_t nq;            /* Nq (equal in every direction, dgsem.jl:92) */
+    int32_t law;           /* RV_LAW_* */
+    int32_t flux;          /* RV_FLUX_* */
+    int32_t dtype;         /* RV_F64 / RV_F32 */
+    int32_t geometry;      /* RV_GEOM_* */
+    int32_t directions;    /* bit mask of active directions (Raven's `directions`), 0 = all */
+    int64_t ncells;        /* K, local cells (no ghosts on one device) */
+    double gamma;          /* Euler gamma (EulerLaw: FT(7//5)) */
+    double velocity[3];    /* advection u (AdvectionLaw) */
+    const double *D;       /* derivatives_1d(cell)[1] */
+    const double *weights; /* weights_1d(cell)[1] */
+    const double *metrics; /* first(volumemetrics(grid)) parent */
+    const double *surface_metrics; /* first(surfacemetrics(grid)) parent */
+    const int64_t *vmapM;  /* facemaps(grid).vmapM */
+    const int64_t *vmapP;  /* facemaps(grid).vmapP */
+    const int64_t *boundary_codes; /* boundarycodes(grid), may be NULL (all 0) */
+    const int64_t *dtoc;   /* discontinuoustocontinuous(grid), may be NULL */
+    int64_t ncontinuous;   /* numcnodes(grid) when dtoc is given */
+    void *stream;          /* hipStream_t to run on; NULL: the library creates one */
+} rv_desc;
+
+/* Brick descriptor: the grid a GridManager(LobattoCell(Nq,...), brick(coords,
+ * periodic); min_level) + generate() pair builds, generated inside the library
+ * (coarsegrids.jl:233-405, gridmanager.jl:1084-1208) with uniform spacing. */
+typedef struct rv_brick_desc {
+    int32_t abi_version;
+    int32_t dim, nq, law, flux, dtype;
+    int32_t n[3];          /* cells per direction of the coarse brick */
+    int32_t periodic[3];
+    int32_t level;         /* uniform refinement level (min_level) */
+    double lo[3], hi[3];   /* domain extents */
+    double gamma;
+    double velocity[3];
+    void *stream;
+} rv_brick_desc;
+
+typedef struct rv_info {
+    int32_t dim, nq, ns, dtype;
+    int32_t geometry;           /* resolved RV_GEOM_AFFINE or RV_GEOM_GENERAL */
+    int32_t faces_self;         /* faces with q+ = q- (boundary or self-mapped) */
+    int32_t faces_explicit;     /* faces needing per-node neighbour indices */
+    int32_t reserved;
+    int64_t ncells;
+    int64_t state_bytes;        /* bytes of one state array */
+    int64_t device_bytes;       /* device memory held by the handle */
+    int64_t stage_bytes;        /* algorithmic HBM bytes of one fused LSRK stage (a != 0) */
+    int64_t stage_bytes_first;  /* ... of a stage with a = 0 */
+    int64_t rhs_bytes;          /* ... of rv_rhs with increment = 0 */
+} rv_info;
+
+/* --- lifetime --------------------------------------------------------- */
+
+/* DGSEM(; law, grid, surface_numericalflux) (dgsem.jl:46-81): uploads the
+ * operator, metrics and face maps, encodes per-face neighbour records. */
+int rv_create(const rv_desc *desc, rv_handle **out);
+
+/* GridManager + generate for a uniform brick (see rv_brick_desc), then as
+ * rv_create.  Grid arrays can be read back with rv_brick_arrays. */
+int rv_create_brick(const rv_brick_desc *desc, rv_handle **out);
+
+void rv_destroy(rv_handle *h);
+const char *rv_last_error(const rv_handle *h);
+int rv_get_info(const rv_handle *h, rv_info *info);
+int rv_synchronize(rv_handle *h);
+/* the hipStream_t every call of this handle is ordered on */
+void *rv_stream(const rv_handle *h);
+
+/* --- the hot path ------------------------------------------------------ */
+
+/* dg(dq, q, t; increment) (dgsem.jl:84-126):
+ * dq = (increment ? dq : 0) + volume term + surface term. */
+int rv_rhs(rv_handle *h, void *dq, const void *q, double t, int increment);
+
+/* One fused LSRK stage (odesolvers.jl:53-58):
+ *   dq = a*dq + L(q_in, t);  q_out = q_in + b_dt*dq
+ * dq is not read when a == 0.  q_out must not alias q_in (neighbouring cells
+ * read q_in's face values while q_out is written). */
+int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double a,
+                  double b_dt, double t);
+
+/* solve!(q, timeend, LSRK(dg, q, dt; t0)) (odesolvers.jl:9-61): dq is zeroed
+ * first; the last step is shortened to land on timeend.  q is updated in
+ * place (the library ping-pongs through an internal buffer).
+ * t_out / nsteps_out may be NULL. */
+int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                double timeend, double *t_out, int64_t *nsteps_out);
+
+/* --- diagnostics ------------------------------------------------------- */
+
+/* weightednorm(dg, q; componentwise = true) (measures.jl:6-14), p = 2, with
+ * compensated (Neumaier) FP64 sums in a fixed order: out[s], s < Ns. */
+int rv_weightednorm(rv_handle *h, const void *q, double *out);
+/* Compensated conservation integrals sum(MJ .* q_s): out[s], s < Ns. */
+int rv_integrals(rv_handle *h, const void *q, double *out);
+
+/* --- GatherScatter (gatherscatter.jl:1-103) ----------------------------- */
+
+/* gather!: dest (ncontinuous, nf) = fold(+, ctod column, ascending rows)
+ * of src (Nq^dim, nf, K).  Bit-exact with the reference order. */
+int rv_gather(rv_handle *h, void *dest, const void *src, int nf);
+/* scatter!: dest (Nq^dim, nf, K)[i] = src[dtoc[i]] */
+int rv_scatter(rv_handle *h, void *dest, const void *src, int nf);
+
+/* --- integer maps ------------------------------------------------------- */
+
+/* Rebuild vmapM, vmapP, mapM, mapP ((2dim*Nfp, K) int64, 1-based, host
+ * arrays) from the device-side face records, for bit-exact checks against
+ * facemaps(grid) (lobattocells.jl:2277-2291).  Any pointer may be NULL. */
+int rv_get_maps(rv_handle *h, int64_t *vmapM, int64_t *vmapP, int64_t *mapM, int64_t *mapP);
+
+/* Number of cells of a brick descriptor (prod(n) * 2^(dim*level)), or a
+ * negative RV_E_* code. */
+int64_t rv_brick_numcells(const rv_brick_desc *desc);
+
+/* Host arrays of the grid generate(GridManager(cell, brick(...))) builds for
+ * the brick (gridmanager.jl:1084-1208): points (Np, dim, K), volume metrics,
+ * surface metrics, vmapM, vmapP, boundary codes, dtoc (layouts as in
+ * rv_desc); any pointer may be NULL.  Host only; no device is touched. */
+int rv_brick_arrays(const rv_brick_desc *desc, double *points, double *metrics,
+                    double *surface_metrics, int64_t *vmapM, int64_t *vmapP, int64_t *bc,
+                    int64_t *dtoc, int64_t *ncontinuous);
+
+/* LobattoCell 1-D operators (lobattocells.jl:1-47): nodes, weights and D
+ * (column-major) for Nq points, rounded from 256-bit closed forms. */
+int rv_lobatto(int nq, double *points, double *weights, double *D);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1063 @@
+// C ABI of the MI355X DGSEM library (include/ravenhip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ravenhip.h"
+#include "common.hpp"
+
+namespace rv {
+StageLauncher find_d1_f64_adv(const KernelKey &);
+StageLauncher find_d1_f64_euler(const KernelKey &);
+StageLauncher find_d1_f32_adv(const KernelKey &);
+StageLauncher find_d1_f32_euler(const KernelKey &);
+StageLauncher find_d2_f64_adv(const KernelKey &);
+StageLauncher find_d2_f64_euler(const KernelKey &);
+StageLauncher find_d2_f32_adv(const KernelKey &);
+StageLauncher find_d2_f32_euler(const KernelKey &);
+StageLauncher find_d3_f64_adv(const KernelKey &);
+StageLauncher find_d3_f64_euler(const KernelKey &);
+StageLauncher find_d3_f32_adv(const KernelKey &);
+StageLauncher find_d3_f32_euler(const KernelKey &);
+
+StageLauncher find_stage_launcher(const KernelKey &k)
+{
+    using Finder = StageLauncher (*)(const KernelKey &);
+    static const Finder finders[] = {
+        find_d1_f64_adv, find_d1_f64_euler, find_d1_f32_adv, find_d1_f32_euler,
+        find_d2_f64_adv, find_d2_f64_euler, find_d2_f32_adv, find_d2_f32_euler,
+        find_d3_f64_adv, find_d3_f64_euler, find_d3_f32_adv, find_d3_f32_euler};
+    for (Finder f : finders)
+        if (StageLauncher l = f(k))
+            return l;
+    return nullptr;
+}
+}  // namespace rv
+
+using namespace rv;
+
+struct rv_handle {
+    std::string err;
+    int dim = 0, nq = 0, ns = 0, law = 0, flux = 0, dtype = 0, geo = 0, dir_mask = 0;
+    int64_t K = 0, Np = 0, nfp = 0;
+    double gamma = 1.4, u[3] = {0, 0, 0};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int wpb = 1;
+    StageLauncher launcher = nullptr;
+    // device buffers
+    FaceRec *faces = nullptr;
+    int64_t *explicit_idx = nullptr;
+    double *affine = nullptr, *vgeo = nullptr, *mji = nullptr, *sgeo = nullptr;
+    double *Dt = nullptr, *Dw = nullptr, *winv = nullptr;
+    double *MJ = nullptr;
+    double *partials = nullptr;  // reduction scratch
+    void *qtmp = nullptr;        // ping-pong state of rv_lsrk_run
+    int64_t *dtoc = nullptr;     // 0-based
+    int64_t *ctod_ptr = nullptr;
+    int32_t *ctod_rows = nullptr;  // (cell, node) pairs
+    int64_t nc = 0;
+    int64_t device_bytes = 0;
+    int faces_self = 0, faces_explicit = 0;
+    bool is_brick = false;
+    BrickSpec brick{};
+};
+
+namespace {
+
+std::string g_last_error;
+
+int fail(rv_handle *h, int code, const std::string &msg)
+{
+    if (h)
+        h->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(h, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((h), RV_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T>
+int dalloc(rv_handle *h, T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(h, RV_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                       " bytes failed: " + hipGetErrorString(e));
+    }
+    h->device_bytes += (int64_t)(count * sizeof(T));
+    return RV_OK;
+}
+
+template <typename T>
+int upload(rv_handle *h, T **p, const T *src, size_t count)
+{
+    int rc = dalloc(h, p, count);
+    if (rc)
+        return rc;
+    if (count)
+        HIP_TRY(h, hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return RV_OK;
+}
+
+int64_t ipow(int64_t b, int e)
+{
+    int64_t r = 1;
+    while (e-- > 0)
+        r *= b;
+    return r;
+}
+
+__host__ __device__ int64_t face_node_rt(int dim, int nq, int f, int64_t m)
+{
+    const int d = f >> 1;
+    const int64_t fixed = (f & 1) ? nq - 1 : 0;
+    if (dim == 1)
+        return fixed;
+    if (dim == 2)
+        return d == 0 ? fixed + nq * m : m + nq * fixed;
+    const int64_t a = m % nq, c = m / nq;
+    if (d == 0)
+        return fixed + nq * a + (int64_t)nq * nq * c;
+    if (d == 1)
+        return a + nq * fixed + (int64_t)nq * nq * c;
+    return a + nq * c + (int64_t)nq * nq * fixed;
+}
+
+__host__ __device__ int64_t orient_rt(int dim, int nq, int o, int64_t m)
+{
+    if (dim == 1)
+        return 0;
+    if (dim == 2)
+        return (o & 1) ? nq - 1 - m : m;
+    int64_t a = m % nq, c = m / nq;
+    if (o & 4) {
+        const int64_t t = a;
+        a = c;
+        c = t;
+    }
+    if (o & 1)
+        a = nq - 1 - a;
+    if (o & 2)
+        c = nq - 1 - c;
+    return a + nq * c;
+}
+
+void free_handle(rv_handle *h)
+{
+    if (!h)
+        return;
+    void *bufs[] = {h->faces, h->explicit_idx, h->affine, h->vgeo, h->mji, h->sgeo, h->Dt,
+                    h->Dw, h->winv, h->MJ, h->partials, h->qtmp, h->dtoc, h->ctod_ptr,
+                    h->ctod_rows};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    if (h->own_stream && h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+// --------------------------------------------------------------------------
+// Auxiliary kernels.
+// --------------------------------------------------------------------------
+
+// Neumaier two-sum accumulation, fixed order per thread and fixed tree.
+struct Comp {
+    double s, c;
+};
+
+__device__ inline Comp comp_add(Comp a, double x)
+{
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x))
+        a.c += (a.s - t) + x;
+    else
+        a.c += (x - t) + a.s;
+    a.s = t;
+    return a;
+}
+
+__device__ inline Comp comp_merge(Comp a, Comp b)
+{
+    Comp r = comp_add(a, b.s);
+    r.c += b.c;
+    return r;
+}
+
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+
+// partial[s * RED_BLOCKS + block] = compensated sum over the block's share of
+// MJ * f(q_s); f = square (power 2) or identity (power 1)
+template <typename R>
+__global__ void __launch_bounds__(RED_THREADS)
+    reduce_kernel(const R *q, const double *MJ, int64_t Np, int ns, int64_t K, int power,
+                  double *partial)
+{
+    __shared__ Comp sh[RED_THREADS];
+    const int64_t n = Np * K;
+    const int s = blockIdx.y;
+    Comp acc{0.0, 0.0};
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = (lo + chunk < n) ? lo + chunk : n;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) {
+        const int64_t e = i / Np, node = i - e * Np;
+        const double v = (double)q[node + Np * (s + (int64_t)ns * e)];
+        const double x = power == 2 ? MJ[i] * (v * v) : MJ[i] * v;
+        acc = comp_add(acc, x);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            sh[threadIdx.x] = comp_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * (s * RED_BLOCKS + blockIdx.x)] = sh[0].s;
+        partial[2 * (s * RED_BLOCKS + blockIdx.x) + 1] = sh[0].c;
+    }
+}
+
+// GatherScatter: dest[c + nc*s] = 0.0 + src[rows...] in ascending row order
+template <typename R>
+__global__ void gather_kernel(R *dest, const R *src, const int64_t *ptr, const int32_t *rows,
+                              int64_t nc, int64_t Np, int nf)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc)
+        return;
+    const int64_t lo = ptr[c], hi = ptr[c + 1];
+    for (int s = 0; s < nf; ++s) {
+        R v = R(0);
+        for (int64_t r = lo; r < hi; ++r) {
+            const int64_t e = rows[2 * r], node = rows[2 * r + 1];
+            v = v + src[node + Np * (s + (int64_t)nf * e)];
+        }
+        dest[c + nc * s] = v;
+    }
+}
+
+template <typename R>
+__global__ void scatter_kernel(R *dest, const R *src, const int64_t *dtoc, int64_t nc,
+                               int64_t Np, int64_t K, int nf)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np * K)
+        return;
+    const int64_t e = i / Np, node = i - e * Np;
+    const int64_t c = dtoc[i];
+    for (int s = 0; s < nf; ++s)
+        dest[node + Np * (s + (int64_t)nf * e)] = src[c + nc * s];
+}
+
+// Rebuild 1-based vmapM/vmapP/mapM/mapP from the face records.
+__global__ void maps_kernel(const FaceRec *faces, const int64_t *explicit_idx, int dim, int nq,
+                            int64_t Np, int64_t nfp, int ns, int64_t K, int64_t *vmapM,
+                            int64_t *vmapP, int64_t *mapM, int64_t *mapP)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = 2 * dim;
+    const int64_t nfpt = nf * nfp;
+    if (idx >= nfpt * K)
+        return;
+    const int64_t e = idx / nfpt, slot = idx - e * nfpt;
+    const int f = (int)(slot / nfp);
+    const int64_t m = slot - f * nfp;
+    const int64_t vm = face_node_rt(dim, nq, f, m) + Np * e + 1;
+    const FaceRec rec = faces[e * nf + f];
+    int64_t vp = vm, mp = idx + 1;
+    if (rec.x >= 0) {
+        const int fn = rec.y & 7, o = (rec.y >> 3) & 7;
+        const int64_t mm = orient_rt(dim, nq, o, m);
+        vp = face_node_rt(dim, nq, fn, mm) + Np * rec.x + 1;
+        mp = fn * nfp + mm + nfpt * rec.x + 1;
+    } else if (rec.x == FACE_EXPLICIT) {
+        const int64_t off = explicit_idx[(int64_t)rec.y + m];
+        const int64_t en = off / (ns * Np), node = off - en * ns * Np;
+        vp = node + Np * en + 1;
+        mp = 0;
+    }
+    vmapM[idx] = vm;
+    vmapP[idx] = vp;
+    mapM[idx] = idx + 1;
+    mapP[idx] = mp;
+}
+
+// ---------------------------------------------------------------------------
+
+int resolve_stream(rv_handle *h, void *stream)
+{
+    HIP_TRY(h, hipGetDevice(&h->device));
+    if (stream) {
+        h->stream = static_cast<hipStream_t>(stream);
+        h->own_stream = false;
+    } else {
+        HIP_TRY(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    return RV_OK;
+}
+
+int finish_setup(rv_handle *h, const double *D, const double *w)
+{
+    const int nq = h->nq;
+    std::vector<double> Dt(nq * nq), Dw(nq * nq), winv(nq);
+    for (int i = 0; i < nq; ++i) {
+        winv[i] = 1.0 / w[i];
+        for (int n = 0; n < nq; ++n) {
+            Dt[i * nq + n] = D[n + nq * i];
+            Dw[i * nq + n] = D[n + nq * i] * w[n] / w[i];
+        }
+    }
+    int rc;
+    if ((rc = upload(h, &h->Dt, Dt.data(), Dt.size())) ||
+        (rc = upload(h, &h->Dw, Dw.data(), Dw.size())) ||
+        (rc = upload(h, &h->winv, winv.data(), winv.size())) ||
+        (rc = dalloc(h, &h->partials, (size_t)2 * RED_BLOCKS * h->ns)))
+        return rc;
+    const KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
+    h->launcher = find_stage_launcher(key);
+    if (!h->launcher)
+        return fail(h, RV_E_UNSUPPORTED,
+                    "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +
+                        " law=" + std::to_string(h->law) + " dtype=" + std::to_string(h->dtype));
+    return RV_OK;
+}
+
+int build_gatherscatter(rv_handle *h, const int64_t *dtoc, int64_t nc)
+{
+    if (!dtoc)
+        return RV_OK;
+    const int64_t n = h->Np * h->K;
+    std::vector<int64_t> d0(n), ptr(nc + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        if (dtoc[i] < 1 || dtoc[i] > nc)
+            return fail(h, RV_E_ARG, "dtoc entry out of range at " + std::to_string(i));
+        d0[i] = dtoc[i] - 1;
+        ptr[d0[i] + 1] += 1;
+    }
+    for (int64_t c = 0; c < nc; ++c)
+        ptr[c + 1] += ptr[c];
+    // ctod = sparse(1:n, dtoc, true): rows of a column ascend
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    std::vector<int32_t> rows(2 * std::max<int64_t>(n, 1));
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t r = fill[d0[i]]++;
+        rows[2 * r] = (int32_t)(i / h->Np);
+        rows[2 * r + 1] = (int32_t)(i % h->Np);
+    }
+    h->nc = nc;
+    int rc;
+    if ((rc = upload(h, &h->dtoc, d0.data(), d0.size())) ||
+        (rc = upload(h, &h->ctod_ptr, ptr.data(), ptr.size())) ||
+        (rc = upload(h, &h->ctod_rows, rows.data(), rows.size())))
+        return rc;
+    return RV_OK;
+}
+
+bool close_rel(double a, double b, double rtol)
+{
+    return std::fabs(a - b) <= rtol * std::max(std::fabs(a), std::fabs(b));
+}
+
+int create_from_desc(const rv_desc *d, rv_handle *h)
+{
+    const int dim = d->dim, nq = d->nq;
+    const int64_t K = d->ncells;
+    h->dim = dim;
+    h->nq = nq;
+    h->law = d->law;
+    h->flux = d->flux ? d->flux : RV_FLUX_RUSANOV;
+    h->dtype = d->dtype;
+    h->K = K;
+    h->ns = d->law == RV_LAW_ADVECTION ? 1 : dim + 2;
+    h->Np = ipow(nq, dim);
+    h->nfp = ipow(nq, dim - 1);
+    h->gamma = d->gamma;
+    h->dir_mask = d->directions ? d->directions : (1 << dim) - 1;
+    for (int c = 0; c < 3; ++c)
+        h->u[c] = d->velocity[c];
+    const int64_t Np = h->Np, nfp = h->nfp, nf = 2 * dim, nfpt = nf * nfp;
+    const int L = dim * dim + 2;
+    if (K > 0 && (!d->D || !d->weights || !d->metrics || !d->surface_metrics || !d->vmapM ||
+                  !d->vmapP))
+        return fail(h, RV_E_ARG, "descriptor misses D, weights, metrics, surface metrics or maps");
+    if ((int64_t)K * h->ns * Np >= ((int64_t)1 << 31) * 8)
+        return fail(h, RV_E_ARG, "state too large");
+    int rc = resolve_stream(h, d->stream);
+    if (rc)
+        return rc;
+    const double *M = d->metrics, *S = d->surface_metrics, *w = d->weights;
+
+    // MJ = wJ, MJI = 1 ./ MJ (dgsem.jl:55-56)
+    std::vector<double> MJ(Np * K);
+    for (int64_t e = 0; e < K; ++e)
+        for (int64_t n = 0; n < Np; ++n)
+            MJ[n + Np * e] = M[n + Np * ((L - 1) + (int64_t)L * e)];
+
+    // ---- geometry: affine detection
+    bool affine = d->geometry != RV_GEOM_GENERAL;
+    std::vector<double> aff(K * dim);
+    for (int64_t e = 0; e < K && affine; ++e) {
+        const double *me = M + Np * (int64_t)L * e;
+        const double J = me[Np * (dim * dim)];
+        for (int a = 0; a < dim; ++a)
+            aff[e * dim + a] = me[Np * (a + dim * a)];
+        if (!(J > 0))
+            affine = false;
+        for (int a = 0; a < dim && affine; ++a)
+            if (!(aff[e * dim + a] > 0))
+                affine = false;
+        for (int64_t n = 0; n < Np && affine; ++n) {
+            for (int c = 0; c < dim && affine; ++c)
+                for (int a = 0; a < dim && affine; ++a) {
+                    const double g = me[n + Np * (a + dim * c)];
+                    if (a == c ? g != aff[e * dim + a] : g != 0.0)
+                        affine = false;
+                }
+            if (me[n + Np * (dim * dim)] != J)
+                affine = false;
+            int64_t rem = n;
+            double wprod = 0;
+            for (int c = 0; c < dim; ++c) {
+                wprod = c == 0 ? w[rem % nq] : wprod * w[rem % nq];
+                rem /= nq;
+            }
+            if (!close_rel(me[n + Np * (dim * dim + 1)], wprod * J, 1e-14))
+                affine = false;
+        }
+        // surface metrics consistent with the volume ones: n = +-e_d, wsJ
+        for (int f = 0; f < nf && affine; ++f) {
+            const int fg = f / 2;
+            const double sn = (f & 1) ? 1.0 : -1.0;
+            for (int64_t m = 0; m < nfp && affine; ++m) {
+                const double *s = S + (dim + 2) * (f * nfp + m + nfpt * e);
+                for (int c = 0; c < dim; ++c)
+                    if (s[c] != (c == fg ? sn : 0.0))
+                        affine = false;
+                double wf = 1.0;
+                if (dim == 2)
+                    wf = w[m];
+                else if (dim == 3)
+                    wf = w[m % nq] * w[m / nq];
+                if (!close_rel(s[dim + 1], J * aff[e * dim + fg] * wf, 1e-14))
+                    affine = false;
+            }
+        }
+    }
+    if (d->geometry == RV_GEOM_AFFINE && !affine)
+        return fail(h, RV_E_ARG, "geometry forced affine but the metrics are not affine");
+    h->geo = affine ? GEO_AFFINE : GEO_GENERAL;
+    if (affine) {
+        if ((rc = upload(h, &h->affine, aff.data(), aff.size())))
+            return rc;
+    } else {
+        std::vector<double> vg((size_t)K * dim * dim * Np), mji(K * Np),
+            sg((size_t)K * nf * (dim + 1) * nfp);
+        for (int64_t e = 0; e < K; ++e) {
+            const double *me = M + Np * (int64_t)L * e;
+            for (int64_t n = 0; n < Np; ++n) {
+                const double wJ = me[n + Np * (dim * dim + 1)];
+                mji[e * Np + n] = 1.0 / wJ;
+                for (int a = 0; a < dim; ++a)
+                    for (int c = 0; c < dim; ++c)
+                        vg[((e * dim + a) * dim + c) * Np + n] = wJ * me[n + Np * (a + dim * c)];
+            }
+            for (int f = 0; f < nf; ++f)
+                for (int64_t m = 0; m < nfp; ++m) {
+                    const double *s = S + (dim + 2) * (f * nfp + m + nfpt * e);
+                    for (int c = 0; c < dim; ++c)
+                        sg[((e * nf + f) * (dim + 1) + c) * nfp + m] = s[c];
+                    sg[((e * nf + f) * (dim + 1) + dim) * nfp + m] = s[dim + 1];
+                }
+        }
+        if ((rc = upload(h, &h->vgeo, vg.data(), vg.size())) ||
+            (rc = upload(h, &h->mji, mji.data(), mji.size())) ||
+            (rc = upload(h, &h->sgeo, sg.data(), sg.size())))
+            return rc;
+    }
+    if ((rc = upload(h, &h->MJ, MJ.data(), MJ.size())))
+        return rc;
+
+    // ---- face records from vmapM / vmapP / boundary codes
+    std::vector<FaceRec> recs(K * nf);
+    std::vector<int64_t> expl;
+    for (int64_t e = 0; e < K; ++e)
+        for (int f = 0; f < nf; ++f) {
+            const int64_t *vm = d->vmapM + f * nfp + nfpt * e;
+            const int64_t *vp = d->vmapP + f * nfp + nfpt * e;
+            for (int64_t m = 0; m < nfp; ++m)
+                if (vm[m] != face_node_rt(dim, nq, f, m) + Np * e + 1)
+                    return fail(h, RV_E_MAPS, "vmapM differs from the cell-face layout at cell " +
+                                                  std::to_string(e) + " face " + std::to_string(f));
+            const int64_t code = d->boundary_codes ? d->boundary_codes[f + nf * e] : 0;
+            bool self = true;
+            for (int64_t m = 0; m < nfp; ++m)
+                self = self && vp[m] == vm[m];
+            if (code != 0 || self) {
+                // boundarystate is the identity for the laws here (balancelaw.jl:22)
+                recs[e * nf + f] = {FACE_SELF, code != 0 ? 1 : 0};
+                h->faces_self++;
+                continue;
+            }
+            const int64_t en = (vp[0] - 1) / Np;
+            bool found = false;
+            if (en >= 0 && en < K) {
+                for (int cand = 0; cand < nf && !found; ++cand) {
+                    const int fn = (cand == 0) ? (f ^ 1) : (cand <= (f ^ 1) ? cand - 1 : cand);
+                    for (int o = 0; o < (dim == 3 ? 8 : dim == 2 ? 2 : 1) && !found; ++o) {
+                        bool ok = true;
+                        for (int64_t m = 0; m < nfp && ok; ++m)
+                            ok = vp[m] - 1 == face_node_rt(dim, nq, fn, orient_rt(dim, nq, o, m)) + Np * en;
+                        if (ok) {
+                            recs[e * nf + f] = {(int32_t)en, fn | (o << 3)};
+                            found = true;
+                        }
+                    }
+                }
+            }
+            if (!found) {
+                recs[e * nf + f] = {FACE_EXPLICIT, (int32_t)expl.size()};
+                for (int64_t m = 0; m < nfp; ++m) {
+                    const int64_t v = vp[m] - 1;
+                    if (v < 0 || v >= Np * K)
+                        return fail(h, RV_E_MAPS, "vmapP entry out of range");
+                    expl.push_back((v / Np) * h->ns * Np + v % Np);
+                }
+                h->faces_explicit++;
+            }
+        }
+    if ((rc = upload(h, &h->faces, recs.data(), recs.size())))
+        return rc;
+    if (!expl.empty() && (rc = upload(h, &h->explicit_idx, expl.data(), expl.size())))
+        return rc;
+    if ((rc = build_gatherscatter(h, d->dtoc, d->ncontinuous)))
+        return rc;
+    return finish_setup(h, d->D, d->weights);
+}
+
+StageParams stage_params(rv_handle *h)
+{
+    StageParams P{};
+    P.K = h->K;
+    P.dir_mask = h->dir_mask;
+    P.faces = h->faces;
+    P.explicit_idx = h->explicit_idx;
+    P.affine = h->affine;
+    P.vgeo = h->vgeo;
+    P.mji = h->mji;
+    P.sgeo = h->sgeo;
+    P.Dt = h->Dt;
+    P.Dw = h->Dw;
+    P.winv = h->winv;
+    P.gamma = h->gamma;
+    for (int c = 0; c < 3; ++c)
+        P.u[c] = h->u[c];
+    return P;
+}
+
+int launch(rv_handle *h, const StageParams &P)
+{
+    if (h->launcher(P, h->stream, h->wpb) != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(h, RV_E_HIP, std::string("stage launch failed: ") + hipGetErrorString(e));
+    }
+    return RV_OK;
+}
+
+int reduce(rv_handle *h, const void *q, int power, double *out)
+{
+    if (!q)
+        return fail(h, RV_E_ARG, "null state");
+    dim3 grid(RED_BLOCKS, h->ns);
+    if (h->dtype == RV_F64)
+        hipLaunchKernelGGL(reduce_kernel<double>, grid, dim3(RED_THREADS), 0, h->stream,
+                           static_cast<const double *>(q), h->MJ, h->Np, h->ns, h->K, power,
+                           h->partials);
+    else
+        hipLaunchKernelGGL(reduce_kernel<float>, grid, dim3(RED_THREADS), 0, h->stream,
+                           static_cast<const float *>(q), h->MJ, h->Np, h->ns, h->K, power,
+                           h->partials);
+    HIP_TRY(h, hipGetLastError());
+    std::vector<double> part((size_t)2 * RED_BLOCKS * h->ns);
+    HIP_TRY(h, hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int s = 0; s < h->ns; ++s) {
+        // Neumaier over the block partials in block order
+        double sum = 0, comp = 0;
+        for (int b = 0; b < RED_BLOCKS; ++b) {
+            for (int k = 0; k < 2; ++k) {
+                const double x = part[2 * (s * RED_BLOCKS + b) + k];
+                const double t = sum + x;
+                comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
+                sum = t;
+            }
+        }
+        out[s] = sum + comp;
+    }
+    return RV_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int rv_create(const rv_desc *desc, rv_handle **out)
+{
+    if (!out)
+        return fail(nullptr, RV_E_ARG, "out is null");
+    *out = nullptr;
+    if (!desc)
+        return fail(nullptr, RV_E_ARG, "descriptor is null");
+    if (desc->abi_version != RAVENHIP_ABI_VERSION)
+        return fail(nullptr, RV_E_ARG, "ABI version mismatch");
+    if (desc->dim < 1 || desc->dim > 3 || desc->nq < 2 || desc->nq > 16 || desc->ncells < 0 ||
+        (desc->law != RV_LAW_ADVECTION && desc->law != RV_LAW_EULER) ||
+        (desc->dtype != RV_F64 && desc->dtype != RV_F32) ||
+        (desc->flux != 0 && desc->flux != RV_FLUX_RUSANOV && desc->flux != RV_FLUX_CENTRAL))
+        return fail(nullptr, RV_E_ARG, "invalid descriptor field (dim, nq, ncells, law, dtype or flux)");
+    rv_handle *h = new rv_handle();
+    int rc = create_from_desc(desc, h);
+    if (rc) {
+        g_last_error = h->err;
+        free_handle(h);
+        return rc;
+    }
+    *out = h;
+    return RV_OK;
+}
+
+static int brick_spec(const rv_brick_desc *bd, BrickSpec *b)
+{
+    if (!bd || bd->abi_version != RAVENHIP_ABI_VERSION)
+        return fail(nullptr, RV_E_ARG, "null descriptor or ABI version mismatch");
+    if (bd->dim < 1 || bd->dim > 3 || bd->nq < 2 || bd->nq > 16 || bd->level < 0 ||
+        bd->level > 10 || (bd->law != RV_LAW_ADVECTION && bd->law != RV_LAW_EULER) ||
+        (bd->dtype != RV_F64 && bd->dtype != RV_F32))
+        return fail(nullptr, RV_E_ARG, "invalid brick descriptor");
+    *b = BrickSpec{};
+    b->dim = bd->dim;
+    b->nq = bd->nq;
+    b->level = bd->level;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        b->n[d] = d < bd->dim ? bd->n[d] : 1;
+        b->periodic[d] = d < bd->dim ? bd->periodic[d] : 0;
+        b->lo[d] = bd->lo[d];
+        b->hi[d] = bd->hi[d];
+        if (d < bd->dim && (b->n[d] < 1 || b->n[d] > (1 << 20) || !(b->hi[d] > b->lo[d])))
+            return fail(nullptr, RV_E_ARG, "invalid brick extents");
+        cells *= b->n[d];
+    }
+    if ((cells << (bd->dim * bd->level)) > ((int64_t)1 << 31))
+        return fail(nullptr, RV_E_ARG, "brick has too many cells");
+    return RV_OK;
+}
+
+int64_t rv_brick_numcells(const rv_brick_desc *bd)
+{
+    BrickSpec b;
+    const int rc = brick_spec(bd, &b);
+    return rc ? rc : brick_numcells(b);
+}
+
+int rv_create_brick(const rv_brick_desc *bd, rv_handle **out)
+{
+    if (!out)
+        return fail(nullptr, RV_E_ARG, "out is null");
+    *out = nullptr;
+    BrickSpec b;
+    if (int rc0 = brick_spec(bd, &b))
+        return rc0;
+    rv_handle *h = new rv_handle();
+    h->is_brick = true;
+    h->brick = b;
+    const int dim = b.dim, nq = b.nq;
+    h->dim = dim;
+    h->nq = nq;
+    h->law = bd->law;
+    h->flux = bd->flux ? bd->flux : RV_FLUX_RUSANOV;
+    h->dtype = bd->dtype;
+    h->K = brick_numcells(b);
+    h->ns = bd->law == RV_LAW_ADVECTION ? 1 : dim + 2;
+    h->Np = ipow(nq, dim);
+    h->nfp = ipow(nq, dim - 1);
+    h->gamma = bd->gamma;
+    h->dir_mask = (1 << dim) - 1;
+    for (int c = 0; c < 3; ++c)
+        h->u[c] = bd->velocity[c];
+    h->geo = GEO_AFFINE;
+    int rc = resolve_stream(h, bd->stream);
+    const int64_t K = h->K, Np = h->Np, nf = 2 * dim;
+    std::vector<double> x(nq), w(nq), D(nq * nq), drdx(K * dim), J(K), MJ(K * Np);
+    std::vector<int64_t> nbr(K * nf);
+    std::vector<FaceRec> recs(K * nf);
+    if (!rc) {
+        lobatto_operators(nq, x.data(), w.data(), D.data());
+        brick_affine(b, drdx.data(), J.data());
+        brick_neighbours(b, nbr.data());
+        for (int64_t e = 0; e < K; ++e) {
+            for (int64_t n = 0; n < Np; ++n) {
+                int64_t rem = n;
+                double wp = 0;
+                for (int c = 0; c < dim; ++c) {
+                    wp = c == 0 ? w[rem % nq] : wp * w[rem % nq];
+                    rem /= nq;
+                }
+                MJ[e * Np + n] = wp * J[e];
+            }
+            for (int f = 0; f < nf; ++f) {
+                const int64_t en = nbr[e * nf + f];
+                recs[e * nf + f] = en < 0 ? FaceRec{FACE_SELF, 1} : FaceRec{(int32_t)en, f ^ 1};
+                if (en < 0)
+                    h->faces_self++;
+            }
+        }
+        if (!(rc = upload(h, &h->affine, drdx.data(), drdx.size())) &&
+            !(rc = upload(h, &h->MJ, MJ.data(), MJ.size())) &&
+            !(rc = upload(h, &h->faces, recs.data(), recs.size()))) {
+            std::vector<int64_t> dtoc(K * Np);
+            const int64_t nc = brick_dtoc(b, dtoc.data());
+            rc = build_gatherscatter(h, dtoc.data(), nc);
+            if (!rc)
+                rc = finish_setup(h, D.data(), w.data());
+        }
+    }
+    if (rc) {
+        g_last_error = h->err;
+        free_handle(h);
+        return rc;
+    }
+    *out = h;
+    return RV_OK;
+}
+
+void rv_destroy(rv_handle *h)
+{
+    if (h && h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    free_handle(h);
+}
+
+const char *rv_last_error(const rv_handle *h)
+{
+    return h ? h->err.c_str() : g_last_error.c_str();
+}
+
+int rv_get_info(const rv_handle *h, rv_info *info)
+{
+    if (!h || !info)
+        return fail(nullptr, RV_E_ARG, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->dim = h->dim;
+    info->nq = h->nq;
+    info->ns = h->ns;
+    info->dtype = h->dtype;
+    info->geometry = h->geo == GEO_AFFINE ? RV_GEOM_AFFINE : RV_GEOM_GENERAL;
+    info->faces_self = h->faces_self;
+    info->faces_explicit = h->faces_explicit;
+    info->ncells = h->K;
+    const int64_t b = h->dtype == RV_F64 ? 8 : 4;
+    const int64_t nodes = h->Np * h->K;
+    info->state_bytes = nodes * h->ns * b;
+    info->device_bytes = h->device_bytes;
+    // compulsory traffic per node: read q, [read dq], write dq, write q; plus
+    // per-node FP64 geometry (wJ*dRdX, 1/wJ) and per face-node (n, wsJ) when
+    // the cells are not affine; affine geometry is per cell (~0 per node)
+    int64_t geo = 0;
+    if (h->geo == GEO_GENERAL)
+        geo = nodes * (h->dim * h->dim + 1) * 8 + h->K * 2 * h->dim * h->nfp * (h->dim + 1) * 8;
+    info->stage_bytes = 4 * info->state_bytes + geo;
+    info->stage_bytes_first = 3 * info->state_bytes + geo;
+    info->rhs_bytes = 2 * info->state_bytes + geo;
+    return RV_OK;
+}
+
+int rv_synchronize(rv_handle *h)
+{
+    if (!h)
+        return fail(nullptr, RV_E_ARG, "null handle");
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return RV_OK;
+}
+
+void *rv_stream(const rv_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+int rv_rhs(rv_handle *h, void *dq, const void *q, double t, int increment)
+{
+    (void)t;  // no time-dependent sources for the laws here (balancelaw.jl:23-26)
+    if (!h || !dq || !q)
+        return fail(h, RV_E_ARG, "null argument");
+    StageParams P = stage_params(h);
+    P.q_in = q;
+    P.q_out = nullptr;
+    P.dq = dq;
+    P.a = 1.0;
+    P.b_dt = 0.0;
+    P.read_dq = increment ? 1 : 0;
+    P.write_q = 0;
+    return launch(h, P);
+}
+
+int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double a, double b_dt,
+                  double t)
+{
+    (void)t;
+    if (!h || !q_out || !q_in || !dq)
+        return fail(h, RV_E_ARG, "null argument");
+    if (q_out == q_in)
+        return fail(h, RV_E_ARG, "q_out must not alias q_in");
+    StageParams P = stage_params(h);
+    P.q_in = q_in;
+    P.q_out = q_out;
+    P.dq = dq;
+    P.a = a;
+    P.b_dt = b_dt;
+    P.read_dq = a != 0.0 ? 1 : 0;
+    P.write_q = 1;
+    return launch(h, P);
+}
+
+int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                double timeend, double *t_out, int64_t *nsteps_out)
+{
+    if (!h || !q || !dq)
+        return fail(h, RV_E_ARG, "null argument");
+    // coefficients (odesolvers.jl:163-244) rounded to the state precision
+    static const double a54[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                                  -2404267990393.0 / 2016746695238.0,
+                                  -3550918686646.0 / 2091501179385.0,
+                                  -1275806237668.0 / 842570457699.0};
+    static const double b54[5] = {1432997174477.0 / 9575080441755.0,
+                                  5161836677717.0 / 13612068292357.0,
+                                  1720146321549.0 / 2090206949498.0,
+                                  3134564353537.0 / 4481467310338.0,
+                                  2277821191437.0 / 14882151754819.0};
+    static const double c54[5] = {0.0, 1432997174477.0 / 9575080441755.0,
+                                  2526269341429.0 / 6820363962896.0,
+                                  2006345519317.0 / 3224310063776.0,
+                                  2802321613138.0 / 2924317926251.0};
+    static const double a144[14] = {0, -0.7188012108672410, -0.7785331173421570,
+                                    -0.0053282796654044, -0.8552979934029281,
+                                    -3.9564138245774565, -1.5780575380587385,
+                                    -2.0837094552574054, -0.7483334182761610,
+                                    -0.7032861106563359, 0.0013917096117681,
+                                    -0.0932075369637460, -0.9514200470875948,
+                                    -7.1151571693922548};
+    static const double b144[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
+                                    0.0030097086818182, 0.3326293790646110, 0.2440251405350864,
+                                    0.3718879239592277, 0.6204126221582444, 0.1524043173028741,
+                                    0.0760894927419266, 0.0077604214040978, 0.0024647284755382,
+                                    0.0780348340049386, 5.5059777270269628};
+    static const double c144[14] = {0, 0.0367762454319673, 0.1249685262725025,
+                                    0.2446177702277698, 0.2476149531070420,
+                                    0.2969311120382472, 0.3978149645802642,
+                                    0.5270854589440328, 0.6981269994175695,
+                                    0.8190890835352128, 0.8527059887098624,
+                                    0.8604711817462826, 0.8627060376969976,
+                                    0.8734213127600976};
+    const double *A, *B, *C;
+    int nst;
+    if (scheme == RV_LSRK54) {
+        A = a54, B = b54, C = c54, nst = 5;
+    } else if (scheme == RV_LSRK144) {
+        A = a144, B = b144, C = c144, nst = 14;
+    } else {
+        return fail(h, RV_E_ARG, "unknown LSRK scheme");
+    }
+    const bool f32 = h->dtype == RV_F32;
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (f32 ? 4 : 8);
+    if (!h->qtmp) {
+        hipError_t e = hipMalloc(&h->qtmp, bytes ? bytes : 1);
+        if (e != hipSuccess) {
+            h->qtmp = nullptr;
+            return fail(h, RV_E_NOMEM, "hipMalloc of the ping-pong state failed");
+        }
+        h->device_bytes += (int64_t)bytes;
+    }
+    HIP_TRY(h, hipMemsetAsync(dq, 0, bytes, h->stream));
+    // LSRK(rhs!, ...) converts dt, t0 and the coefficients to the state type
+    // (odesolvers.jl:41-48); in FP32 the host arithmetic is FP32 too
+    auto rnd = [f32](double v) { return f32 ? (double)(float)v : v; };
+    double time = rnd(t0), step_dt = rnd(dt), tend = rnd(timeend);
+    void *cur = q, *alt = h->qtmp;
+    int64_t steps = 0;
+    StageParams P = stage_params(h);
+    for (;;) {
+        ++steps;
+        bool final_step = false;
+        const double tdt = f32 ? (double)((float)time + (float)step_dt) : time + step_dt;
+        if (tdt >= tend) {
+            step_dt = f32 ? (double)((float)tend - (float)time) : tend - time;
+            final_step = true;
+        }
+        for (int s = 0; s < nst; ++s) {
+            P.q_in = cur;
+            P.q_out = alt;
+            P.dq = dq;
+            P.a = rnd(A[s]);
+            P.b_dt = f32 ? (double)((float)rnd(B[s]) * (float)step_dt) : B[s] * step_dt;
+            P.read_dq = P.a != 0.0 ? 1 : 0;
+            P.write_q = 1;
+            int rc = launch(h, P);
+            if (rc)
+                return rc;
+            std::swap(cur, alt);
+        }
+        (void)C;
+        time = f32 ? (double)((float)time + (float)step_dt) : time + step_dt;
+        if (final_step)
+            break;
+        if (steps > (int64_t)1 << 40)
+            return fail(h, RV_E_ARG, "time step too small");
+    }
+    if (cur != q)
+        HIP_TRY(h, hipMemcpyAsync(q, cur, bytes, hipMemcpyDeviceToDevice, h->stream));
+    if (t_out)
+        *t_out = time;
+    if (nsteps_out)
+        *nsteps_out = steps;
+    return RV_OK;
+}
+
+int rv_weightednorm(rv_handle *h, const void *q, double *out)
+{
+    if (!h || !out)
+        return fail(h, RV_E_ARG, "null argument");
+    int rc = reduce(h, q, 2, out);
+    if (rc)
+        return rc;
+    for (int s = 0; s < h->ns; ++s)
+        out[s] = std::sqrt(out[s]);
+    return RV_OK;
+}
+
+int rv_integrals(rv_handle *h, const void *q, double *out)
+{
+    if (!h || !out)
+        return fail(h, RV_E_ARG, "null argument");
+    return reduce(h, q, 1, out);
+}
+
+int rv_gather(rv_handle *h, void *dest, const void *src, int nf)
+{
+    if (!h || !dest || !src || nf < 1)
+        return fail(h, RV_E_ARG, "invalid argument");
+    if (!h->ctod_ptr)
+        return fail(h, RV_E_STATE, "no dtoc was given to rv_create");
+    const int64_t blocks = (h->nc + 255) / 256;
+    if (h->dtype == RV_F64)
+        hipLaunchKernelGGL(gather_kernel<double>, dim3((unsigned)std::max<int64_t>(blocks, 1)),
+                           dim3(256), 0, h->stream, static_cast<double *>(dest),
+                           static_cast<const double *>(src), h->ctod_ptr, h->ctod_rows, h->nc,
+                           h->Np, nf);
+    else
+        hipLaunchKernelGGL(gather_kernel<float>, dim3((unsigned)std::max<int64_t>(blocks, 1)),
+                           dim3(256), 0, h->stream, static_cast<float *>(dest),
+                           static_cast<const float *>(src), h->ctod_ptr, h->ctod_rows, h->nc,
+                           h->Np, nf);
+    HIP_TRY(h, hipGetLastError());
+    return RV_OK;
+}
+
+int rv_scatter(rv_handle *h, void *dest, const void *src, int nf)
+{
+    if (!h || !dest || !src || nf < 1)
+        return fail(h, RV_E_ARG, "invalid argument");
+    if (!h->dtoc)
+        return fail(h, RV_E_STATE, "no dtoc was given to rv_create");
+    const int64_t n = h->Np * h->K;
+    const int64_t blocks = std::max<int64_t>((n + 255) / 256, 1);
+    if (h->dtype == RV_F64)
+        hipLaunchKernelGGL(scatter_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                           static_cast<double *>(dest), static_cast<const double *>(src), h->dtoc,
+                           h->nc, h->Np, h->K, nf);
+    else
+        hipLaunchKernelGGL(scatter_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                           static_cast<float *>(dest), static_cast<const float *>(src), h->dtoc,
+                           h->nc, h->Np, h->K, nf);
+    HIP_TRY(h, hipGetLastError());
+    return RV_OK;
+}
+
+int rv_get_maps(rv_handle *h, int64_t *vmapM, int64_t *vmapP, int64_t *mapM, int64_t *mapP)
+{
+    if (!h)
+        return fail(nullptr, RV_E_ARG, "null handle");
+    const int64_t n = 2 * h->dim * h->nfp * h->K;
+    int64_t *buf = nullptr;
+    HIP_TRY(h, hipMalloc(&buf, std::max<int64_t>(4 * n, 1) * sizeof(int64_t)));
+    const int64_t blocks = std::max<int64_t>((n + 255) / 256, 1);
+    hipLaunchKernelGGL(maps_kernel, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->faces,
+                       h->explicit_idx, h->dim, h->nq, h->Np, h->nfp, h->ns, h->K, buf, buf + n,
+                       buf + 2 * n, buf + 3 * n);
+    hipError_t e = hipGetLastError();
+    std::vector<int64_t> host(4 * n);
+    if (e == hipSuccess && n)
+        e = hipMemcpyAsync(host.data(), buf, 4 * n * sizeof(int64_t), hipMemcpyDeviceToHost,
+                           h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess)
+        return fail(h, RV_E_HIP, std::string("rv_get_maps: ") + hipGetErrorString(e));
+    int64_t *dst[4] = {vmapM, vmapP, mapM, mapP};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k] && n)
+            std::memcpy(dst[k], host.data() + k * n, n * sizeof(int64_t));
+    return RV_OK;
+}
+
+int rv_brick_arrays(const rv_brick_desc *bd, double *points, double *metrics,
+                    double *surface_metrics, int64_t *vmapM, int64_t *vmapP, int64_t *bc,
+                    int64_t *dtoc, int64_t *ncontinuous)
+{
+    BrickSpec b;
+    if (int rc = brick_spec(bd, &b))
+        return rc;
+    brick_arrays(b, points, metrics, surface_metrics, vmapM, vmapP, bc);
+    if (dtoc || ncontinuous) {
+        std::vector<int64_t> tmp;
+        int64_t *dst = dtoc;
+        if (!dst) {
+            tmp.resize(ipow(b.nq, b.dim) * brick_numcells(b));
+            dst = tmp.data();
+        }
+        const int64_t nc = brick_dtoc(b, dst);
+        if (ncontinuous)
+            *ncontinuous = nc;
+    }
+    return RV_OK;
+}
+
+int rv_lobatto(int nq, double *points, double *weights, double *D)
+{
+    if (nq < 2 || nq > 32 || !points || !weights || !D)
+        return fail(nullptr, RV_E_ARG, "invalid argument");
+    lobatto_operators(nq, points, weights, D);
+    return RV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Internal definitions shared by the host API and the kernel instantiation units.
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+namespace rv {
+
+enum LawId : int { LAW_ADVECTION = 1, LAW_EULER = 2 };
+enum GeoId : int { GEO_AFFINE = 1, GEO_GENERAL = 2 };
+enum FluxId : int { FLUX_RUSANOV = 1, FLUX_CENTRAL = 2 };
+
+// Per-face neighbour record, one per (face, cell):
+//   x >= 0 : neighbour cell; y = neighbour face | orientation << 3
+//   x == -1: q+ = q- (boundary code != 0 with the identity boundary state,
+//            or a self-mapped face: Raven's non-conforming / ghost faces)
+//   x == -2: explicit per-node offsets; y = index of the face's first entry
+//            in the explicit table (cell offsets e*Ns*Np + node, int64)
+constexpr int FACE_SELF = -1;
+constexpr int FACE_EXPLICIT = -2;
+
+struct FaceRec {
+    int32_t x;
+    int32_t y;
+};
+
+// Arguments of one fused DG stage launch.  Pointers are device pointers.
+struct StageParams {
+    int64_t K;            // cells
+    const void *q_in;     // state read (R)
+    void *q_out;          // state written (R), nullptr in RHS mode
+    void *dq;             // register (R)
+    double a;             // dq = a*dq + L
+    double b_dt;          // q_out = q_in + b_dt*dq
+    int read_dq;          // load dq (a != 0 or increment)
+    int write_q;          // stage mode
+    int dir_mask;         // active directions (bit d)
+    int reserved;
+    const FaceRec *faces;        // (2dim, K)
+    const int64_t *explicit_idx; // explicit face-node offsets
+    // affine geometry: per cell, per direction: dRdX[d,d] (FP64)
+    const double *affine;        // (K, dim)
+    // general geometry (FP64): wJ*dRdX rows, MJI, face normals and wsJ
+    const double *vgeo;          // (K, dim, dim, Np): [e][d][c][node] = wJ*dRdX[d,c]
+    const double *mji;           // (K, Np)
+    const double *sgeo;          // (K, 2dim, dim+1, Nfp): n_c ..., wsJ
+    // operators (FP64, device)
+    const double *Dt;            // [i*Nq + n] = D[n,i]            (general)
+    const double *Dw;            // [i*Nq + n] = D[n,i] w_n / w_i  (affine)
+    const double *winv;          // [i] = 1/w_i
+    double gamma;
+    double u[3];
+};
+
+// Launch the fused stage for one kernel configuration; returns false when no
+// kernel is instantiated for it.
+using StageLauncher = int (*)(const StageParams &, void *stream, int waves_per_block);
+
+struct KernelKey {
+    int dim, nq, law, real_bytes, geo, flux;
+};
+
+// Host-side grid generation (host_grid.cpp): Raven's brick GridManager +
+// generate restated for uniform bricks, and the LobattoCell 1-D operators.
+struct BrickSpec {
+    int dim, nq, level;
+    int n[3], periodic[3];
+    double lo[3], hi[3];
+};
+int64_t brick_numcells(const BrickSpec &b);
+// tree and quadrant coordinates (K x dim), p4est brick order
+void brick_quadrants(const BrickSpec &b, int32_t *tree, int32_t *quad);
+// vertex coordinates of the coarse brick along d (n[d]+1 values, correctly
+// rounded lattice lo + (hi-lo)*i/n like Julia's range(lo, stop=hi, length=n+1))
+void brick_vertices(const BrickSpec &b, int d, double *v);
+void lobatto_operators(int nq, double *x, double *w, double *D);
+// full Raven-layout arrays (any pointer may be null)
+void brick_arrays(const BrickSpec &b, double *points, double *metrics, double *surf,
+                  int64_t *vmapM, int64_t *vmapP, int64_t *bc);
+int64_t brick_dtoc(const BrickSpec &b, int64_t *dtoc);
+// per-cell diagonal of dRdX (K x dim) and neighbour cell per face (K x 2dim,
+// -1 on non-periodic boundaries) without per-node arrays
+void brick_affine(const BrickSpec &b, double *drdx, double *J);
+void brick_neighbours(const BrickSpec &b, int64_t *nbr);
+
+}  // namespace rv

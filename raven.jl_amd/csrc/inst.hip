@@ -1,0 +1,52 @@
+// Explicit instantiations of the fused stage kernel for one (dim, precision,
+// law) triple; the Makefile compiles this file once per triple with
+//   -DINST_DIM=<1|2|3> -DINST_REAL=<double|float> -DINST_LAW=<1|2> -DINST_FN=<name>
+// and api.hip asks every unit for a launcher.
+#include "dg_kernels.hpp"
+
+#ifndef INST_DIM
+#error "INST_DIM must be defined"
+#endif
+
+namespace rv {
+
+namespace {
+
+template <int NQ>
+StageLauncher pick(const KernelKey &k)
+{
+    constexpr int DIM = INST_DIM, LAW = INST_LAW;
+    using R = INST_REAL;
+    if constexpr (IPow<NQ, DIM - 1>::v > 64) {
+        return nullptr;
+    } else {
+        if (k.geo == GEO_AFFINE) {
+            if (k.flux == FLUX_RUSANOV)
+                return &launch_stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
+            return &launch_stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_CENTRAL>;
+        }
+        if (k.flux == FLUX_RUSANOV)
+            return &launch_stage<DIM, NQ, LAW, R, GEO_GENERAL, FLUX_RUSANOV>;
+        return &launch_stage<DIM, NQ, LAW, R, GEO_GENERAL, FLUX_CENTRAL>;
+    }
+}
+
+}  // namespace
+
+StageLauncher INST_FN(const KernelKey &k)
+{
+    if (k.dim != INST_DIM || k.law != INST_LAW || k.real_bytes != (int)sizeof(INST_REAL))
+        return nullptr;
+    switch (k.nq) {
+    case 2: return pick<2>(k);
+    case 3: return pick<3>(k);
+    case 4: return pick<4>(k);
+    case 5: return pick<5>(k);
+    case 6: return pick<6>(k);
+    case 7: return pick<7>(k);
+    case 8: return pick<8>(k);
+    default: return nullptr;
+    }
+}
+
+}  // namespace rv

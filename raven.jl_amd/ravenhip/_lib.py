@@ -1,0 +1,124 @@
+"""ctypes binding of libravenhip.so (include/ravenhip.h).
+
+The shared library must have been built (``__graft_entry__.build()`` or
+``make -C raven.jl_amd/csrc``); importing the package without it raises, so a
+missing HIP extension can never fall back to anything else.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libravenhip.so")
+
+ABI_VERSION = 1
+RV_OK = 0
+RV_LAW_ADVECTION, RV_LAW_EULER = 1, 2
+RV_FLUX_RUSANOV, RV_FLUX_CENTRAL = 1, 2
+RV_F64, RV_F32 = 1, 2
+RV_GEOM_AUTO, RV_GEOM_AFFINE, RV_GEOM_GENERAL = 0, 1, 2
+RV_LSRK54, RV_LSRK144 = 54, 144
+
+ERRORS = {-1: "RV_E_ARG", -2: "RV_E_HIP", -3: "RV_E_NOMEM", -4: "RV_E_UNSUPPORTED",
+          -5: "RV_E_MAPS", -6: "RV_E_STATE"}
+
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class RvDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("dim", ctypes.c_int32), ("nq", ctypes.c_int32),
+        ("law", ctypes.c_int32), ("flux", ctypes.c_int32), ("dtype", ctypes.c_int32),
+        ("geometry", ctypes.c_int32), ("directions", ctypes.c_int32),
+        ("ncells", ctypes.c_int64), ("gamma", ctypes.c_double),
+        ("velocity", ctypes.c_double * 3),
+        ("D", c_f64p), ("weights", c_f64p), ("metrics", c_f64p),
+        ("surface_metrics", c_f64p), ("vmapM", c_i64p), ("vmapP", c_i64p),
+        ("boundary_codes", c_i64p), ("dtoc", c_i64p), ("ncontinuous", ctypes.c_int64),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+class RvBrickDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("dim", ctypes.c_int32), ("nq", ctypes.c_int32),
+        ("law", ctypes.c_int32), ("flux", ctypes.c_int32), ("dtype", ctypes.c_int32),
+        ("n", ctypes.c_int32 * 3), ("periodic", ctypes.c_int32 * 3), ("level", ctypes.c_int32),
+        ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3), ("gamma", ctypes.c_double),
+        ("velocity", ctypes.c_double * 3), ("stream", ctypes.c_void_p),
+    ]
+
+
+class RvInfo(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("nq", ctypes.c_int32), ("ns", ctypes.c_int32),
+        ("dtype", ctypes.c_int32), ("geometry", ctypes.c_int32), ("faces_self", ctypes.c_int32),
+        ("faces_explicit", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("ncells", ctypes.c_int64), ("state_bytes", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64), ("stage_bytes", ctypes.c_int64),
+        ("stage_bytes_first", ctypes.c_int64), ("rhs_bytes", ctypes.c_int64),
+    ]
+
+
+class RavenHIPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_LIB = None
+H = ctypes.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: build it first (__graft_entry__.build())")
+    L = ctypes.CDLL(LIBPATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "rv_create": (ctypes.c_int, [ctypes.POINTER(RvDesc), ctypes.POINTER(H)]),
+        "rv_create_brick": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), ctypes.POINTER(H)]),
+        "rv_destroy": (None, [H]),
+        "rv_last_error": (ctypes.c_char_p, [H]),
+        "rv_get_info": (ctypes.c_int, [H, ctypes.POINTER(RvInfo)]),
+        "rv_synchronize": (ctypes.c_int, [H]),
+        "rv_stream": (vp, [H]),
+        "rv_rhs": (ctypes.c_int, [H, vp, vp, ctypes.c_double, ctypes.c_int]),
+        "rv_lsrk_stage": (ctypes.c_int, [H, vp, vp, vp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double]),
+        "rv_lsrk_run": (ctypes.c_int, [H, vp, vp, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, c_f64p, c_i64p]),
+        "rv_weightednorm": (ctypes.c_int, [H, vp, c_f64p]),
+        "rv_integrals": (ctypes.c_int, [H, vp, c_f64p]),
+        "rv_gather": (ctypes.c_int, [H, vp, vp, ctypes.c_int]),
+        "rv_scatter": (ctypes.c_int, [H, vp, vp, ctypes.c_int]),
+        "rv_get_maps": (ctypes.c_int, [H, c_i64p, c_i64p, c_i64p, c_i64p]),
+        "rv_brick_numcells": (ctypes.c_int64, [ctypes.POINTER(RvBrickDesc)]),
+        "rv_brick_arrays": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), c_f64p, c_f64p, c_f64p,
+                                           c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
+        "rv_lobatto": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc, h=None):
+    if rc != RV_OK:
+        msg = lib().rv_last_error(h)
+        raise RavenHIPError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def ptr(a, ctype):
+    """ctypes pointer to a numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,208 @@
+"""DGSEM operator over the HIP library, mirroring Raven.jl's balance-law API.
+
+``DGSEM(law=..., grid=..., surface_numericalflux=...)`` (dgsem.jl:46-81) and
+its call ``dg(dq, q, t, increment=True)`` (dgsem.jl:84-126).  State arrays are
+torch tensors on the HIP device shaped ``(K, Ns, Np)`` (C-order), which is the
+memory of Raven's GridArray parent ``(Np, Ns, K)``.  torch only provides device
+memory and the stream; all arithmetic happens in libravenhip's kernels.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class AdvectionLaw:
+    """AdvectionLaw{FT,D}(u⃗) (balancelaws/advection.jl:8-16); u⃗ defaults to ones."""
+
+    law_id = L.RV_LAW_ADVECTION
+
+    def __init__(self, dim, u=None):
+        self.dim = dim
+        self.u = tuple(float(v) for v in (u if u is not None else (1.0,) * dim))
+        if len(self.u) != dim:
+            raise ValueError("velocity needs one component per dimension")
+        self.gamma = 1.4
+        self.ns = 1
+
+
+class EulerLaw:
+    """EulerLaw{FT,D}(; γ = 7//5) (balancelaws/euler.jl:9-19)."""
+
+    law_id = L.RV_LAW_EULER
+
+    def __init__(self, dim, gamma=7 / 5):
+        self.dim = dim
+        self.gamma = float(gamma)
+        self.u = (0.0,) * dim
+        self.ns = dim + 2
+
+
+class RusanovFlux:
+    flux_id = L.RV_FLUX_RUSANOV
+
+
+class CentralFlux:
+    flux_id = L.RV_FLUX_CENTRAL
+
+
+_DTYPES = {"f64": L.RV_F64, "f32": L.RV_F32}
+_GEOMS = {"auto": L.RV_GEOM_AUTO, "affine": L.RV_GEOM_AFFINE, "general": L.RV_GEOM_GENERAL}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _current_stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DGSEM:
+    """Fused DGSEM right-hand side on one MI355X.
+
+    grid      -- a Grid from generate() (explicit arrays, the drop-in path)
+    law       -- AdvectionLaw / EulerLaw
+    dtype     -- "f64" or "f32" (FP32 state/flux arithmetic, FP64 geometry)
+    geometry  -- "auto" (affine cells detected), "affine" or "general"
+    """
+
+    def __init__(self, *, law, grid, surface_numericalflux=None, dtype="f64", geometry="auto",
+                 directions=None, stream=None, _handle=None):
+        self.law = law
+        self.grid = grid
+        self.dtype = dtype
+        flux = surface_numericalflux or RusanovFlux()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            if grid.dim != law.dim:
+                raise ValueError("law and grid dimensions differ")
+            d = L.RvDesc()
+            d.abi_version = L.ABI_VERSION
+            d.dim, d.nq = grid.dim, grid.nq
+            d.law, d.flux = law.law_id, flux.flux_id
+            d.dtype = _DTYPES[dtype]
+            d.geometry = _GEOMS[geometry]
+            d.directions = 0 if directions is None else sum(1 << (k - 1) for k in directions)
+            d.ncells = grid.K
+            d.gamma = law.gamma
+            for k in range(3):
+                d.velocity[k] = law.u[k] if k < len(law.u) else 0.0
+            cell = grid.referencecell
+            keep = [np.ascontiguousarray(a) for a in (
+                cell.D_colmajor, cell.weights_1d, grid.metrics, grid.surface)]
+            ikeep = [np.ascontiguousarray(a, dtype=np.int64) for a in (
+                grid.vmapM, grid.vmapP, grid.boundarycodes, grid.dtoc)]
+            d.D, d.weights, d.metrics, d.surface_metrics = (L.ptr(a, ctypes.c_double) for a in keep)
+            d.vmapM, d.vmapP, d.boundary_codes, d.dtoc = (L.ptr(a, ctypes.c_int64) for a in ikeep)
+            d.ncontinuous = grid.ncontinuous
+            d.stream = stream if stream is not None else _current_stream()
+            h = L.H()
+            L.check(L.lib().rv_create(ctypes.byref(d), ctypes.byref(h)))
+            self._h = h
+        info = L.RvInfo()
+        L.check(L.lib().rv_get_info(self._h, ctypes.byref(info)), self._h)
+        self.info = info
+        self.ns = info.ns
+        self.K = info.ncells
+        self.Np = info.nq ** info.dim
+
+    @classmethod
+    def brick(cls, gm, law, surface_numericalflux=None, dtype="f64", stream=None):
+        """DGSEM on a library-generated uniform brick (no per-node host arrays)."""
+        flux = surface_numericalflux or RusanovFlux()
+        desc = gm.brick_desc(law=law.law_id, dtype=_DTYPES[dtype], flux=flux.flux_id,
+                             gamma=law.gamma, velocity=law.u,
+                             stream=stream if stream is not None else _current_stream())
+        h = L.H()
+        L.check(L.lib().rv_create_brick(ctypes.byref(desc), ctypes.byref(h)))
+        return cls(law=law, grid=None, surface_numericalflux=flux, dtype=dtype, _handle=h)
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().rv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return L.lib().rv_stream(self._h)
+
+    def synchronize(self):
+        L.check(L.lib().rv_synchronize(self._h), self._h)
+
+    # -- state helpers ------------------------------------------------------
+    @property
+    def torch_dtype(self):
+        torch = _torch()
+        return torch.float64 if self.dtype == "f64" else torch.float32
+
+    def GridArray(self, fill=0.0, device="cuda"):
+        torch = _torch()
+        return torch.full((self.K, self.ns, self.Np), fill, dtype=self.torch_dtype, device=device)
+
+    def _check_state(self, t):
+        if t.dtype != self.torch_dtype or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("state must be a contiguous device tensor of the operator's dtype")
+        if t.numel() != self.K * self.ns * self.Np:
+            raise ValueError("state has the wrong number of entries")
+        return ctypes.c_void_p(t.data_ptr())
+
+    # -- the hot path ------------------------------------------------------
+    def __call__(self, dq, q, time=0.0, increment=True):
+        """dg(dq, q, t; increment) (dgsem.jl:84-126)."""
+        L.check(L.lib().rv_rhs(self._h, self._check_state(dq), self._check_state(q),
+                               float(time), 1 if increment else 0), self._h)
+
+    def lsrk_stage(self, q_out, q_in, dq, a, b_dt, time=0.0):
+        L.check(L.lib().rv_lsrk_stage(self._h, self._check_state(q_out), self._check_state(q_in),
+                                      self._check_state(dq), float(a), float(b_dt), float(time)),
+                self._h)
+
+    # -- diagnostics ------------------------------------------------------
+    def weightednorm_components(self, q):
+        out = np.zeros(self.ns)
+        L.check(L.lib().rv_weightednorm(self._h, self._check_state(q),
+                                        L.ptr(out, ctypes.c_double)), self._h)
+        return out
+
+    def integrals(self, q):
+        out = np.zeros(self.ns)
+        L.check(L.lib().rv_integrals(self._h, self._check_state(q), L.ptr(out, ctypes.c_double)),
+                self._h)
+        return out
+
+    # -- GatherScatter -------------------------------------------------------
+    def gather(self, dest, src, nf):
+        L.check(L.lib().rv_gather(self._h, ctypes.c_void_p(dest.data_ptr()),
+                                  ctypes.c_void_p(src.data_ptr()), int(nf)), self._h)
+
+    def scatter(self, dest, src, nf):
+        L.check(L.lib().rv_scatter(self._h, ctypes.c_void_p(dest.data_ptr()),
+                                   ctypes.c_void_p(src.data_ptr()), int(nf)), self._h)
+
+    def maps(self):
+        n = 2 * self.info.dim * self.info.nq ** (self.info.dim - 1) * self.K
+        out = [np.zeros(n, dtype=np.int64) for _ in range(4)]
+        L.check(L.lib().rv_get_maps(self._h, *(L.ptr(a, ctypes.c_int64) for a in out)), self._h)
+        return out
+
+
+def weightednorm(dg, q, componentwise=False):
+    """weightednorm(dg, q) (measures.jl:6-14), p = 2, compensated FP64 sums."""
+    comp = dg.weightednorm_components(q)
+    return comp if componentwise else float(np.sqrt(np.sum(comp * comp)))

@@ -1,0 +1,157 @@
+"""Reference cell and brick grids, mirroring Raven.jl's host API.
+
+``LobattoCell`` (src/lobattocells.jl:170-240), ``brick`` (src/coarsegrids.jl:
+233-405), ``GridManager`` (src/gridmanager.jl:30-91) and ``generate``
+(src/gridmanager.jl:1084-1208) with the same names and argument meaning.  The
+arrays come from the native generator in libravenhip (rv_brick_arrays) and use
+Raven's memory layout; numpy views are shaped C-order, i.e. Julia's
+``(Np, L, K)`` parent becomes ``(K, L, Np)`` here.
+"""
+import numpy as np
+
+from . import _lib as L
+
+
+class LobattoCell:
+    """LobattoCell{T}(Nq, ..., Nq): tensor-product Gauss-Lobatto cell."""
+
+    def __init__(self, *sizes):
+        if len(sizes) < 1 or len(sizes) > 3 or len(set(sizes)) != 1:
+            raise ValueError("LobattoCell needs 1-3 equal sizes (DGSEM asserts equal Nq)")
+        self.size = tuple(int(s) for s in sizes)
+        self.nq = self.size[0]
+        self.dim = len(sizes)
+        nq = self.nq
+        self.points_1d = np.zeros(nq)
+        self.weights_1d = np.zeros(nq)
+        self._D = np.zeros(nq * nq)
+        L.check(L.lib().rv_lobatto(nq, L.ptr(self.points_1d, L.ctypes.c_double),
+                                   L.ptr(self.weights_1d, L.ctypes.c_double),
+                                   L.ptr(self._D, L.ctypes.c_double)))
+
+    def derivatives_1d(self):
+        """D[a, b] = l_b'(x_a) as an (Nq, Nq) array."""
+        return self._D.reshape(self.nq, self.nq, order="F")
+
+    @property
+    def D_colmajor(self):
+        return self._D
+
+    def __len__(self):
+        return self.nq ** self.dim
+
+
+class BrickGrid:
+    """brick(coordinates, periodic): uniform coarse brick."""
+
+    def __init__(self, coordinates, periodic):
+        self.coordinates = [np.asarray(c, dtype=np.float64) for c in coordinates]
+        self.dim = len(self.coordinates)
+        self.periodic = tuple(bool(p) for p in periodic)
+        if len(self.periodic) != self.dim:
+            raise ValueError("periodic must have one flag per dimension")
+        self.n = [len(c) - 1 for c in self.coordinates]
+        self.lo = [float(c[0]) for c in self.coordinates]
+        self.hi = [float(c[-1]) for c in self.coordinates]
+
+
+def brick(coordinates, periodic=None):
+    coordinates = tuple(coordinates)
+    if periodic is None:
+        periodic = (False,) * len(coordinates)
+    return BrickGrid(coordinates, periodic)
+
+
+class GridManager:
+    def __init__(self, cell, coarsegrid, min_level=0):
+        if cell.dim != coarsegrid.dim:
+            raise ValueError("cell and coarse grid dimensions differ")
+        self.cell = cell
+        self.coarsegrid = coarsegrid
+        self.min_level = int(min_level)
+
+    def brick_desc(self, law=L.RV_LAW_ADVECTION, dtype=L.RV_F64, flux=L.RV_FLUX_RUSANOV,
+                   gamma=1.4, velocity=(0.0, 0.0, 0.0), stream=None):
+        cg = self.coarsegrid
+        d = L.RvBrickDesc()
+        d.abi_version = L.ABI_VERSION
+        d.dim, d.nq, d.law, d.flux, d.dtype = cg.dim, self.cell.nq, law, flux, dtype
+        for k in range(3):
+            d.n[k] = cg.n[k] if k < cg.dim else 1
+            d.periodic[k] = int(cg.periodic[k]) if k < cg.dim else 0
+            d.lo[k] = cg.lo[k] if k < cg.dim else 0.0
+            d.hi[k] = cg.hi[k] if k < cg.dim else 1.0
+            d.velocity[k] = velocity[k] if k < len(velocity) else 0.0
+        d.level = self.min_level
+        d.gamma = gamma
+        d.stream = stream
+        return d
+
+
+class Grid:
+    """Arrays of a generated grid in Raven's layout (C-order views)."""
+
+    def __init__(self, gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc):
+        self.gm = gm
+        self.referencecell = gm.cell
+        self.dim = gm.cell.dim
+        self.nq = gm.cell.nq
+        self.K = points.shape[0]
+        self.Np = self.nq ** self.dim
+        self.nfp = self.nq ** (self.dim - 1)
+        self.points = points          # (K, dim, Np)
+        self.metrics = metrics        # (K, dim*dim+2, Np): dRdX (column-major), J, wJ
+        self.surface = surface        # (K, 2dim*Nfp, dim+2): n, sJ, wsJ
+        self.vmapM = vmapM            # (K, 2dim*Nfp) int64, 1-based
+        self.vmapP = vmapP
+        self.boundarycodes = bc       # (K, 2dim)
+        self.dtoc = dtoc              # (K, Np) int64, 1-based
+        self.ncontinuous = nc
+
+    def __len__(self):
+        return self.K
+
+    @property
+    def MJ(self):
+        return self.metrics[:, -1, :]
+
+
+def _check_uniform(cg, desc):
+    for d in range(cg.dim):
+        n = cg.n[d]
+        got = cg.coordinates[d]
+        # the generator's lattice lo + (hi-lo)*i/n, correctly rounded
+        from fractions import Fraction
+        lo, hi = Fraction(cg.lo[d]), Fraction(cg.hi[d])
+        want = np.array([float(lo + (hi - lo) * i / n) for i in range(n + 1)])
+        if not np.array_equal(got, want):
+            raise NotImplementedError("only uniformly spaced brick coordinates are supported")
+
+
+def generate(gm, warp=None):
+    """generate(gm) / generate(warp, gm) (gridmanager.jl:1084-1208)."""
+    if warp is not None:
+        raise NotImplementedError("warped grids are not supported yet")
+    desc = gm.brick_desc()
+    _check_uniform(gm.coarsegrid, desc)
+    lib = L.lib()
+    K = lib.rv_brick_numcells(L.ctypes.byref(desc))
+    if K < 0:
+        L.check(int(K))
+    dim, nq = gm.cell.dim, gm.cell.nq
+    Np, nfp = nq ** dim, nq ** (dim - 1)
+    nfpt = 2 * dim * nfp
+    points = np.zeros((K, dim, Np))
+    metrics = np.zeros((K, dim * dim + 2, Np))
+    surface = np.zeros((K, nfpt, dim + 2))
+    vmapM = np.zeros((K, nfpt), dtype=np.int64)
+    vmapP = np.zeros((K, nfpt), dtype=np.int64)
+    bc = np.zeros((K, 2 * dim), dtype=np.int64)
+    dtoc = np.zeros((K, Np), dtype=np.int64)
+    nc = L.ctypes.c_int64(0)
+    c = L.ctypes
+    L.check(lib.rv_brick_arrays(c.byref(desc), L.ptr(points, c.c_double), L.ptr(metrics, c.c_double),
+                                L.ptr(surface, c.c_double), L.ptr(vmapM, c.c_int64),
+                                L.ptr(vmapP, c.c_int64), L.ptr(bc, c.c_int64),
+                                L.ptr(dtoc, c.c_int64), c.byref(nc)))
+    return Grid(gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc.value)
