@@ -85,7 +85,7 @@ typedef struct rv_desc {
     const int64_t *boundary_codes; /* boundarycodes(grid), may be NULL (all 0) */
     const int64_t *dtoc;   /* discontinuoustocontinuous(grid), may be NULL */
     int64_t ncontinuous;   /* numcnodes(grid) when dtoc is given */
-    void *stream;          /* hipStream_t to run on; NULL: the library creates one */
+    void *stream;          /* hipStream_t every call runs on; NULL: the default stream */
 } rv_desc;
 
 /* Brick descriptor: the grid a GridManager(LobattoCell(Nq,...), brick(coords,

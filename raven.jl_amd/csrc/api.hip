@@ -306,13 +306,10 @@ __global__ void maps_kernel(const FaceRec *faces, const int64_t *explicit_idx, i
 int resolve_stream(rv_handle *h, void *stream)
 {
     HIP_TRY(h, hipGetDevice(&h->device));
-    if (stream) {
-        h->stream = static_cast<hipStream_t>(stream);
-        h->own_stream = false;
-    } else {
-        HIP_TRY(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        h->own_stream = true;
-    }
+    // NULL selects the default (null) stream, which orders with the caller's
+    // other default-stream work (torch's default stream is that stream)
+    h->stream = static_cast<hipStream_t>(stream);
+    h->own_stream = false;
     return RV_OK;
 }
 

@@ -71,13 +71,32 @@ _LIB = None
 H = ctypes.c_void_p
 
 
+def _hip_runtimes():
+    try:
+        with open("/proc/self/maps") as fh:
+            return sorted({ln.split()[-1] for ln in fh if "libamdhip64" in ln})
+    except OSError:
+        return []
+
+
 def lib():
     global _LIB
     if _LIB is not None:
         return _LIB
     if not os.path.exists(LIBPATH):
         raise RuntimeError(f"{LIBPATH} is missing: build it first (__graft_entry__.build())")
+    # One process must hold one HIP runtime.  torch (which provides device
+    # memory and streams here) ships its own libamdhip64 with the same soname;
+    # loading it first makes libravenhip's dependency resolve to that copy.
+    try:
+        import torch  # noqa: F401
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIBPATH)
+    runtimes = _hip_runtimes()
+    if len(runtimes) > 1:
+        raise RuntimeError("two HIP runtimes are loaded in this process: " + ", ".join(runtimes))
     vp = ctypes.c_void_p
     sig = {
         "rv_create": (ctypes.c_int, [ctypes.POINTER(RvDesc), ctypes.POINTER(H)]),
