@@ -332,6 +332,9 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
         return rc;
     const KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
     h->launcher = find_stage_launcher(key);
+    // waves per workgroup measured on MI355X for the 3-D column kernel at
+    // Nq = 8: two for FP64 (256 VGPRs, 2 waves/SIMD), four for FP32
+    h->wpb = h->dim == 3 ? (h->dtype == RV_F64 ? 2 : 4) : 1;
     if (!h->launcher)
         return fail(h, RV_E_UNSUPPORTED,
                     "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +

@@ -1,0 +1,374 @@
+// Fused 3-D DGSEM stage kernel for gfx950: "column" decomposition.
+//
+// Same operator as dg_kernels.hpp (Raven's weak-form volume term, Rusanov /
+// central surface term, LSRK update: dgsem.jl:257-348, 733-784,
+// odesolvers.jl:55-57), organised so that every state value crosses HBM once:
+//
+//  * a wave owns EPW = 64 / Nq^2 cells; lane (i, j) of a cell holds the column
+//    q(i, j, 0..Nq-1) of all states in registers (loaded with lane-contiguous
+//    addresses) together with its accumulators;
+//  * the k loop visits the cell's planes: the lane evaluates the flux at (i,j,k),
+//    scatters the z-derivative contribution into its own accumulators
+//    (acc[kk] += Dz[k][kk] f3), and publishes f1, f2 in a double-buffered LDS
+//    plane from which every lane gathers its x- and y-derivatives;
+//  * faces: the z faces sit in each lane's own column; the four x / y faces
+//    (4 Nq^2 face nodes) are spread over the lanes, one node of each face per
+//    lane, with q- and the lift exchanged through LDS;
+//  * the accumulators start from a*dq_old (affine cells) so dq_old and q load
+//    together at the top; dq and q_out are stored at the end.
+#pragma once
+
+#include "dg_kernels.hpp"
+
+namespace rv {
+
+template <int NQ, int LAW, typename R, int GEO, int FLUX>
+struct Plane3 {
+    static constexpr int DIM = 3;
+    static constexpr int NS = LawInfo<3, LAW>::NS;
+    static constexpr int NQ2 = NQ * NQ;
+    static constexpr int NP = NQ2 * NQ;
+    static constexpr int EPW = 64 / NQ2;
+    // plane rows are read 16 bytes at a time when they can be: VEC values per
+    // read, rows padded to a multiple of 16 bytes whose 32-bit bank offsets
+    // differ between the Nq rows a wave reads at once
+    static constexpr int VEC = (16 / (int)sizeof(R));
+    static constexpr bool VECREAD = (NQ % VEC) == 0;
+    static constexpr bool rows_disjoint(int pad)
+    {
+        // 16-byte blocks of the NQ rows (32 banks of 4 bytes) must not overlap
+        const int w = (int)sizeof(R) / 4;
+        for (int a = 0; a < NQ; ++a)
+            for (int b = a + 1; b < NQ; ++b) {
+                const int d = ((pad * w * (b - a)) % 32 + 32) % 32;
+                if (d < 4 || d > 28)
+                    return false;
+            }
+        return true;
+    }
+    static constexpr int pick_pad()
+    {
+        if (!VECREAD)
+            return NQ + 1;
+        for (int p = NQ; p < NQ + 64; p += VEC)
+            if (rows_disjoint(p))
+                return p;
+        return NQ + VEC;
+    }
+    static constexpr int PAD = pick_pad();
+    static constexpr int PLANE = NS * NQ * PAD;      // one flux component of one plane
+    static constexpr int PBUF = 2 * PLANE;           // f1 and f2 of one plane
+    static constexpr int FACEBUF = 4 * NS * NQ2;     // q- / lift of the four x,y faces
+    static constexpr int SLAB = (2 * PBUF > FACEBUF) ? 2 * PBUF : FACEBUF;
+    static_assert(EPW >= 1, "Nq^2 must not exceed the wave size");
+    using S = Stage<3, NQ, LAW, R, GEO, FLUX>;
+};
+
+template <int NQ, int LAW, typename R, int GEO, int FLUX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+dg_plane3_kernel(const StageParams P)
+{
+    using T = Plane3<NQ, LAW, R, GEO, FLUX>;
+    using S = typename T::S;
+    constexpr int NS = T::NS, NQ2 = T::NQ2, NP = T::NP, PAD = T::PAD;
+    constexpr int PLANE = T::PLANE, PBUF = T::PBUF;
+    extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
+    R *lds = reinterpret_cast<R *>(dg_lds);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int slot = lane / NQ2;
+    const int l = lane - slot * NQ2;  // column index = i + NQ*j
+    const int i = l % NQ, j = l / NQ;
+    const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t e = (blk * wpb + wave) * T::EPW + slot;
+    const bool active = (slot < T::EPW) && (e < P.K);
+    R *slab = lds + (wave * T::EPW + (slot < T::EPW ? slot : 0)) * T::SLAB;
+    const Law<3, LAW, R> law(P);
+    const R *qbase = static_cast<const R *>(P.q_in);
+    const bool dx_on = P.dir_mask & 1, dy_on = (P.dir_mask >> 1) & 1,
+               dz_on = (P.dir_mask >> 2) & 1;
+
+    R q[NS][NQ], acc[NS][NQ];
+    R qpz[2][NS];
+    R cx[NQ], cy[NQ];  // this lane's x / y derivative weights
+    R sz = R(1);
+    const R a = R(P.a), bdt = R(P.b_dt);
+
+    if (active) {
+        const R *qe = qbase + e * (NS * NP) + l;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                q[s][k] = qe[s * NP + k * NQ2];
+        const R *dqe = static_cast<const R *>(P.dq) + e * (NS * NP) + l;
+        if (GEO == GEO_AFFINE && P.read_dq) {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[s][k] = a * dqe[s * NP + k * NQ2];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[s][k] = R(0);
+        }
+        const double *M = (GEO == GEO_AFFINE) ? P.Dw : P.Dt;
+        R sx = R(1), sy = R(1);
+        if constexpr (GEO == GEO_AFFINE) {
+            sx = R(P.affine[e * 3 + 0]);
+            sy = R(P.affine[e * 3 + 1]);
+            sz = R(P.affine[e * 3 + 2]);
+        }
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            cx[n] = dx_on ? sx * R(M[i * NQ + n]) : R(0);
+            cy[n] = dy_on ? sy * R(M[j * NQ + n]) : R(0);
+        }
+    }
+
+    // ---- plane loop: volume term, software-pipelined: the flux of plane
+    // k+1 is evaluated and published while plane k's gather reads LDS
+    struct alignas(16) Vec {
+        R v[T::VEC];
+    };
+    auto plane_flux = [&](const int k) {
+        R *buf = slab + (k & 1) * PBUF;
+        R qn[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            qn[s] = q[s][k];
+        R f1[NS], f2[NS], f3[NS];
+        R Fa[3][NS];
+        law.flux_all(qn, Fa);
+        if constexpr (GEO == GEO_AFFINE) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                f1[s] = Fa[0][s];
+                f2[s] = Fa[1][s];
+                f3[s] = sz * Fa[2][s];
+            }
+        } else {
+            const int node = l + k * NQ2;
+            R g[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    g[d][c] = R(P.vgeo[((e * 3 + d) * 3 + c) * NP + node]);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                f1[s] = g[0][0] * Fa[0][s] + g[0][1] * Fa[1][s] + g[0][2] * Fa[2][s];
+                f2[s] = g[1][0] * Fa[0][s] + g[1][1] * Fa[1][s] + g[1][2] * Fa[2][s];
+                f3[s] = g[2][0] * Fa[0][s] + g[2][1] * Fa[1][s] + g[2][2] * Fa[2][s];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            buf[s * NQ * PAD + j * PAD + i] = f1[s];          // row j, entry i
+            buf[PLANE + s * NQ * PAD + i * PAD + j] = f2[s];  // row i, entry j
+        }
+        if (dz_on) {
+            const double *M = (GEO == GEO_AFFINE) ? P.Dw : P.Dt;
+#pragma unroll
+            for (int kk = 0; kk < NQ; ++kk) {
+                const R w = R(M[kk * NQ + k]);  // contribution of plane k to output kk
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[s][kk] = acc[s][kk] + w * f3[s];
+            }
+        }
+    };
+    auto gather = [&](const int k) {
+        const R *buf = slab + (k & 1) * PBUF;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const R *r1 = buf + s * NQ * PAD + j * PAD;
+            const R *r2 = buf + PLANE + s * NQ * PAD + i * PAD;
+            R v = acc[s][k];
+            if constexpr (T::VECREAD) {
+#pragma unroll
+                for (int t = 0; t < NQ / T::VEC; ++t) {
+                    const Vec x1 = reinterpret_cast<const Vec *>(r1)[t];
+                    const Vec x2 = reinterpret_cast<const Vec *>(r2)[t];
+#pragma unroll
+                    for (int c = 0; c < T::VEC; ++c)
+                        v = v + cx[t * T::VEC + c] * x1.v[c];
+#pragma unroll
+                    for (int c = 0; c < T::VEC; ++c)
+                        v = v + cy[t * T::VEC + c] * x2.v[c];
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < NQ; ++n)
+                    v = v + cx[n] * r1[n];
+#pragma unroll
+                for (int n = 0; n < NQ; ++n)
+                    v = v + cy[n] * r2[n];
+            }
+            acc[s][k] = v;
+        }
+    };
+    if (active)
+        plane_flux(0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        if (active) {
+            if (k + 1 < NQ)
+                plane_flux(k + 1);
+            gather(k);
+        }
+        __syncthreads();
+    }
+
+    // neighbour states of the two z faces (faces 4, 5; face node = column),
+    // issued here so their latency overlaps the x / y face exchange
+    if (active && dz_on) {
+        R qm0[NS], qm1[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            qm0[s] = q[s][0];
+            qm1[s] = q[s][NQ - 1];
+        }
+        S::neighbour_state(P, qbase, e, 4, l, qm0, qpz[0]);
+        S::neighbour_state(P, qbase, e, 5, l, qm1, qpz[1]);
+    }
+
+    // ---- x and y faces: owners publish q-, every lane evaluates one node of
+    // each of the four faces, owners collect the lift
+    R *fb = slab;  // [face 0..3][s][face node]
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (i == 0)
+                    fb[(0 * NS + s) * NQ2 + j + NQ * k] = q[s][k];
+                if (i == NQ - 1)
+                    fb[(1 * NS + s) * NQ2 + j + NQ * k] = q[s][k];
+                if (j == 0)
+                    fb[(2 * NS + s) * NQ2 + i + NQ * k] = q[s][k];
+                if (j == NQ - 1)
+                    fb[(3 * NS + s) * NQ2 + i + NQ * k] = q[s][k];
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int d = f >> 1;
+            if (!((P.dir_mask >> d) & 1))
+                continue;
+            R qm[NS], qp[NS], nf[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                qm[s] = fb[(f * NS + s) * NQ2 + l];
+            S::neighbour_state(P, qbase, e, f, l, qm, qp);
+            R lift;
+            if constexpr (GEO == GEO_AFFINE) {
+                R fm[NS];
+                law.flux_row(d, qm, fm);
+                S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, qm, qp, nf);
+                lift = R(P.affine[e * 3 + d]) * R(P.winv[(f & 1) ? NQ - 1 : 0]);
+            } else {
+                const double *sg = P.sgeo + ((e * 6 + f) * 4) * NQ2 + l;
+                R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
+                lift = R(sg[3 * NQ2]);
+                S::nflux_general(law, nv, qm, qp, nf);
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                fb[(f * NS + s) * NQ2 + l] = lift * nf[s];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                R v = acc[s][k];
+                if (dx_on && i == 0)
+                    v = v - fb[(0 * NS + s) * NQ2 + j + NQ * k];
+                if (dx_on && i == NQ - 1)
+                    v = v - fb[(1 * NS + s) * NQ2 + j + NQ * k];
+                if (dy_on && j == 0)
+                    v = v - fb[(2 * NS + s) * NQ2 + i + NQ * k];
+                if (dy_on && j == NQ - 1)
+                    v = v - fb[(3 * NS + s) * NQ2 + i + NQ * k];
+                acc[s][k] = v;
+            }
+        }
+        // z faces: in the lane's own column
+        if (dz_on) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int kend = side ? NQ - 1 : 0;
+                R qm[NS], nf[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    qm[s] = q[s][kend];
+                R lift;
+                if constexpr (GEO == GEO_AFFINE) {
+                    R fm[NS];
+                    law.flux_row(2, qm, fm);
+                    S::nflux_axis(law, 2, side ? R(1) : R(-1), fm, qm, qpz[side], nf);
+                    lift = sz * R(P.winv[kend]);
+                } else {
+                    const double *sg = P.sgeo + ((e * 6 + 4 + side) * 4) * NQ2 + l;
+                    R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
+                    lift = R(sg[3 * NQ2]);
+                    S::nflux_general(law, nv, qm, qpz[side], nf);
+                }
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[s][kend] = acc[s][kend] - lift * nf[s];
+            }
+        }
+        // ---- update and store
+        R *dqo = static_cast<R *>(P.dq) + e * (NS * NP) + l;
+        R *qo = static_cast<R *>(P.q_out) + e * (NS * NP) + l;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            R mji = R(1);
+            if constexpr (GEO == GEO_GENERAL)
+                mji = R(P.mji[e * NP + l + k * NQ2]);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                R v = acc[s][k];
+                if constexpr (GEO == GEO_GENERAL) {
+                    v = mji * v;
+                    if (P.read_dq)
+                        v = a * dqo[s * NP + k * NQ2] + v;
+                }
+                dqo[s * NP + k * NQ2] = v;
+                if (P.write_q)
+                    qo[s * NP + k * NQ2] = q[s][k] + bdt * v;
+            }
+        }
+    }
+}
+
+template <int NQ, int LAW, typename R, int GEO, int FLUX>
+int launch_plane3(const StageParams &P, void *stream, int wpb)
+{
+    using T = Plane3<NQ, LAW, R, GEO, FLUX>;
+    if (P.K <= 0)
+        return 0;
+    if (wpb != 1 && wpb != 2 && wpb != 4)
+        wpb = 1;
+    const int64_t waves = (P.K + T::EPW - 1) / T::EPW;
+    const int64_t blocks = (waves + wpb - 1) / wpb;
+    const size_t lds = (size_t)wpb * T::EPW * T::SLAB * sizeof(R);
+    hipLaunchKernelGGL((dg_plane3_kernel<NQ, LAW, R, GEO, FLUX>), dim3((unsigned)blocks),
+                       dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace rv

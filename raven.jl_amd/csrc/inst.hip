@@ -1,8 +1,9 @@
-// Explicit instantiations of the fused stage kernel for one (dim, precision,
+// Explicit instantiations of the fused stage kernels for one (dim, precision,
 // law) triple; the Makefile compiles this file once per triple with
 //   -DINST_DIM=<1|2|3> -DINST_REAL=<double|float> -DINST_LAW=<1|2> -DINST_FN=<name>
-// and api.hip asks every unit for a launcher.
-#include "dg_kernels.hpp"
+// and api.hip asks every unit for a launcher.  3-D uses the column kernel
+// (dg_plane3.hpp), 1-D and 2-D the pencil kernel (dg_kernels.hpp).
+#include "dg_plane3.hpp"
 
 #ifndef INST_DIM
 #error "INST_DIM must be defined"
@@ -12,22 +13,28 @@ namespace rv {
 
 namespace {
 
-template <int NQ>
-StageLauncher pick(const KernelKey &k)
+template <int NQ, int GEO, int FLUX>
+StageLauncher pick_one()
 {
     constexpr int DIM = INST_DIM, LAW = INST_LAW;
     using R = INST_REAL;
-    if constexpr (IPow<NQ, DIM - 1>::v > 64) {
+    if constexpr (DIM == 3)
+        return &launch_plane3<NQ, LAW, R, GEO, FLUX>;
+    else
+        return &launch_stage<DIM, NQ, LAW, R, GEO, FLUX>;
+}
+
+template <int NQ>
+StageLauncher pick(const KernelKey &k)
+{
+    if constexpr (IPow<NQ, INST_DIM - 1>::v > 64) {
         return nullptr;
     } else {
-        if (k.geo == GEO_AFFINE) {
-            if (k.flux == FLUX_RUSANOV)
-                return &launch_stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
-            return &launch_stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_CENTRAL>;
-        }
-        if (k.flux == FLUX_RUSANOV)
-            return &launch_stage<DIM, NQ, LAW, R, GEO_GENERAL, FLUX_RUSANOV>;
-        return &launch_stage<DIM, NQ, LAW, R, GEO_GENERAL, FLUX_CENTRAL>;
+        if (k.geo == GEO_AFFINE)
+            return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_AFFINE, FLUX_RUSANOV>()
+                                          : pick_one<NQ, GEO_AFFINE, FLUX_CENTRAL>();
+        return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_GENERAL, FLUX_RUSANOV>()
+                                      : pick_one<NQ, GEO_GENERAL, FLUX_CENTRAL>();
     }
 }
 

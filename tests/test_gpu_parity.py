@@ -139,6 +139,30 @@ def test_lsrk_stage_and_step_match_oracle(torch, R):
     assert rel_l2(qd2.cpu().numpy().reshape(-1), qo) <= RTOL_F64
 
 
+@pytest.mark.parametrize("dtype_geom", ["auto", "general"])
+def test_three_lsrk54_steps_nq8_match_oracle(torch, R, dtype_geom):
+    """Degree 7 (the benchmark's cell) over three whole LSRK54 steps, with the
+    benchmark's density noise and time-step rule."""
+    coords = [PB.julia_range(0, 1.25, 5)] * 3
+    grid, dg, og, odg = make_pair(R, 3, 8, coords, (True,) * 3, "euler", geometry=dtype_geom)
+    P = og.point_fields()
+    rng = np.random.default_rng(5)
+    q = PB.isentropic_vortex_3d(P, 0.0, centre=(0.6, 0.6),
+                                noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    dt = 0.125 * (10 / 32) / 7 / (1 + np.sqrt(1.4))
+    qo, dqo = q.copy(), np.zeros(q.size)
+    qd = to_dev(torch, dg, q)
+    tmp = torch.empty_like(qd)
+    solver = R.LSRK54(dg, qd, dt)
+    for st in range(3):
+        odg.lsrk_step(qo, dqo, st * dt, dt)
+        R.dostep_(qd, solver, tmp)
+    dg.synchronize()
+    err = rel_l2(qd.cpu().numpy().reshape(-1), qo)
+    say(f"   3 steps Nq=8 {dtype_geom}: rel L2 = {err:.3e}")
+    assert err <= RTOL_F64
+
+
 def test_stage_rejects_aliasing(torch, R):
     _, dg, _, _ = make_pair(R, 2, 4, [PB.julia_range(0, 1, 3)] * 2, (True, True), "advection",
                             u=[1.0, 1.0])
