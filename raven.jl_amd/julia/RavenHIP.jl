@@ -1,0 +1,125 @@
+# RavenHIP.jl -- Julia binding of libravenhip (include/ravenhip.h) for Raven.jl.
+#
+# Drop-in for Raven's DGSEM right-hand side (src/balancelaws/dgsem.jl:84-126)
+# and LSRK time stepping (src/balancelaws/odesolvers.jl:9-61) on one MI355X.
+# Grids, LobattoCell operators, GridArray state and balance laws stay Raven's;
+# this module only hands their arrays to the library through ccall.
+#
+# Status: written against the C ABI; Julia is not available where this
+# library is built and tested, so this file is not exercised by the test
+# suite (the Python mirror in raven.jl_amd/ravenhip is).
+module RavenHIP
+
+using Raven
+using Raven.BalanceLaws
+using Raven.BalanceLaws.Euler: EulerLaw
+using Raven.BalanceLaws.Advection: AdvectionLaw
+
+const libravenhip = joinpath(@__DIR__, "..", "ravenhip", "libravenhip.so")
+
+const RV_LAW_ADVECTION = Int32(1)
+const RV_LAW_EULER = Int32(2)
+const RV_FLUX_RUSANOV = Int32(1)
+const RV_FLUX_CENTRAL = Int32(2)
+const RV_F64 = Int32(1)
+const RV_F32 = Int32(2)
+
+# Field order and types mirror `rv_desc` in include/ravenhip.h.
+struct RvDesc
+    abi_version::Int32
+    dim::Int32
+    nq::Int32
+    law::Int32
+    flux::Int32
+    dtype::Int32
+    geometry::Int32
+    directions::Int32
+    ncells::Int64
+    gamma::Float64
+    velocity::NTuple{3,Float64}
+    D::Ptr{Float64}
+    weights::Ptr{Float64}
+    metrics::Ptr{Float64}
+    surface_metrics::Ptr{Float64}
+    vmapM::Ptr{Int64}
+    vmapP::Ptr{Int64}
+    boundary_codes::Ptr{Int64}
+    dtoc::Ptr{Int64}
+    ncontinuous::Int64
+    stream::Ptr{Cvoid}
+end
+
+mutable struct HIPDGSEM{L}
+    law::L
+    handle::Ptr{Cvoid}
+    function HIPDGSEM(law, handle)
+        dg = new{typeof(law)}(law, handle)
+        finalizer(d -> ccall((:rv_destroy, libravenhip), Cvoid, (Ptr{Cvoid},), d.handle), dg)
+        return dg
+    end
+end
+
+check(rc, h) = rc == 0 || error("libravenhip: ", unsafe_string(
+    ccall((:rv_last_error, libravenhip), Cstring, (Ptr{Cvoid},), h)))
+
+lawid(::EulerLaw) = RV_LAW_EULER
+lawid(::AdvectionLaw) = RV_LAW_ADVECTION
+fluxid(::RusanovFlux) = RV_FLUX_RUSANOV
+fluxid(::CentralFlux) = RV_FLUX_CENTRAL
+
+"""
+    HIPDGSEM(; law, grid, surface_numericalflux, dtype = Float64, stream = C_NULL)
+
+Same keywords as `DGSEM` (dgsem.jl:46-81).  Host copies of the grid arrays
+are read once (`Array(...)` of device arrays is fine); the library keeps
+device copies and per-face neighbour records.
+"""
+function HIPDGSEM(; law, grid, surface_numericalflux, dtype = Float64, stream = C_NULL)
+    cell = referencecell(grid)
+    dim = ndims(cell)
+    Nq = size(cell, 1)
+    D = Array{Float64}(derivatives_1d(cell)[1])
+    w = Array{Float64}(vec(weights_1d(cell)[1]))
+    metrics = Array{Float64}(parent(first(volumemetrics(grid))))
+    surf = Array{Float64}(parent(first(surfacemetrics(grid))))
+    fm = facemaps(grid)
+    vmapM = Array{Int64}(fm.vmapM)
+    vmapP = Array{Int64}(fm.vmapP)
+    bc = Array{Int64}(boundarycodes(grid))
+    dtoc = Array{Int64}(discontinuoustocontinuous(grid))
+    u = law isa AdvectionLaw ? Tuple(Float64.(constants(law).u⃗)) : ()
+    vel = ntuple(i -> i <= length(u) ? u[i] : 0.0, 3)
+    γ = law isa EulerLaw ? Float64(constants(law).γ) : 1.4
+    handle = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve D w metrics surf vmapM vmapP bc dtoc begin
+        desc = RvDesc(1, dim, Nq, lawid(law), fluxid(surface_numericalflux),
+                      dtype == Float64 ? RV_F64 : RV_F32, 0, 0, length(grid), γ, vel,
+                      pointer(D), pointer(w), pointer(metrics), pointer(surf),
+                      pointer(vmapM), pointer(vmapP), pointer(bc), pointer(dtoc),
+                      numcnodes(grid), stream)
+        check(ccall((:rv_create, libravenhip), Cint, (Ref{RvDesc}, Ref{Ptr{Cvoid}}),
+                    desc, handle), C_NULL)
+    end
+    return HIPDGSEM(law, handle[])
+end
+
+# dg(dq, q, t; increment) (dgsem.jl:84): q and dq are device GridArrays whose
+# parent memory is HIP device memory in Raven's layout.
+function (dg::HIPDGSEM)(dq, q, time; increment = true)
+    check(ccall((:rv_rhs, libravenhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint),
+                dg.handle, pointer(parent(dq)), pointer(parent(q)), time, increment), dg.handle)
+end
+
+# solve!(q, timeend, LSRK54(dg, q, dt)) (odesolvers.jl:9-30) in one call.
+function lsrk_solve!(dg::HIPDGSEM, q, dq, dt, timeend; t0 = 0.0, scheme = 54)
+    tout = Ref{Cdouble}(0)
+    nsteps = Ref{Int64}(0)
+    check(ccall((:rv_lsrk_run, libravenhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble,
+                 Ref{Cdouble}, Ref{Int64}),
+                dg.handle, pointer(parent(q)), pointer(parent(dq)), scheme, t0, dt, timeend,
+                tout, nsteps), dg.handle)
+    return tout[], nsteps[]
+end
+
+end # module
