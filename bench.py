@@ -11,6 +11,7 @@ build runs on one GPU.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -24,10 +25,32 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 NQ, NCELL, L = 8, 32, 10.0
 GAMMA = 1.4
+KERNEL_NAME = {"f64": "rv::dg_plane3_kernel<8, Euler, double, affine, Rusanov>",
+               "f32": "rv::dg_plane3_kernel<8, Euler, float, affine, Rusanov>"}
+# rocprofv3 counter summary of this workload's stage kernel (HBM bytes per
+# dispatch); read for roofline.traffic, which stays null without it
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "dg_plane3_kernel_euler3d_32cube_nq8_traffic.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(dtype):
+    """HBM bytes per stage-kernel launch from the committed counter summary:
+    FETCH_SIZE and WRITE_SIZE per dispatch (KiB), each scaled by the factor the
+    same counter shows for a plain device copy of known size, averaged over
+    the five stages of an LSRK54 step.  None when no summary is present."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    if not m:
+        return None
+    entry = json.loads(m.group(1)).get(dtype)
+    return None if entry is None else float(entry["traffic_bytes_per_launch"])
 
 
 def c2_points(R):
@@ -243,8 +266,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "dg_stage_kernel<3,8,Euler,double,affine,Rusanov>",
+            "traffic": measured_traffic("f64"),
+            "kernel": KERNEL_NAME["f64"],
             "avg_kernel_ms": kavg * 1e3,
             "single_rhs_ms": r64["rhs_ms"],
             "algorithmic_bytes_per_launch": bytes_step / 5,
@@ -266,6 +289,7 @@ def main():
             "single_rhs_ms": r32["rhs_ms"],
             "avg_kernel_ms": k32 * 1e3, "achieved_GBs": b32 / k32 / 1e9,
             "frac": b32 / k32 / 1e9 / HBM_PEAK_GBS,
+            "kernel": KERNEL_NAME["f32"], "traffic": measured_traffic("f32"),
             "rel_l2_vs_fp64_after_steps": err,
             "mass_drift_fp64": float((m64[0] - m0[0]) / m0[0]),
             "mass_drift_fp32": float((m32[0] - m0[0]) / m0[0]),

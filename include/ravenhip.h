@@ -154,6 +154,12 @@ int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double 
 int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
                 double timeend, double *t_out, int64_t *nsteps_out);
 
+/* The stage coefficients rka, rkb, rkc of a scheme (coefficients_lsrk54 /
+ * coefficients_lsrk144, odesolvers.jl:163-244) in double precision, for
+ * callers that step stage by stage through rv_lsrk_stage.  Any array pointer
+ * may be NULL; *nstages receives 5 or 14.  No handle, no device work. */
+int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstages);
+
 /* --- diagnostics ------------------------------------------------------- */
 
 /* weightednorm(dg, q; componentwise = true) (measures.jl:6-14), p = 2, with

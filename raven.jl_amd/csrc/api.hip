@@ -840,53 +840,87 @@ int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double 
     return launch(h, P);
 }
 
+// LSRK54 (Carpenter & Kennedy) and LSRK144 (Niegemann, Diehl & Busch)
+// coefficients, odesolvers.jl:163-244; the rationals are rounded to double
+// like Float64(::Rational)
+namespace {
+static const double a54[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                              -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0,
+                              -1275806237668.0 / 842570457699.0};
+static const double b54[5] = {1432997174477.0 / 9575080441755.0,
+                              5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0,
+                              3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+static const double c54[5] = {0.0, 1432997174477.0 / 9575080441755.0,
+                              2526269341429.0 / 6820363962896.0,
+                              2006345519317.0 / 3224310063776.0,
+                              2802321613138.0 / 2924317926251.0};
+static const double a144[14] = {0, -0.7188012108672410, -0.7785331173421570,
+                                -0.0053282796654044, -0.8552979934029281,
+                                -3.9564138245774565, -1.5780575380587385,
+                                -2.0837094552574054, -0.7483334182761610,
+                                -0.7032861106563359, 0.0013917096117681,
+                                -0.0932075369637460, -0.9514200470875948,
+                                -7.1151571693922548};
+static const double b144[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
+                                0.0030097086818182, 0.3326293790646110, 0.2440251405350864,
+                                0.3718879239592277, 0.6204126221582444, 0.1524043173028741,
+                                0.0760894927419266, 0.0077604214040978, 0.0024647284755382,
+                                0.0780348340049386, 5.5059777270269628};
+static const double c144[14] = {0, 0.0367762454319673, 0.1249685262725025,
+                                0.2446177702277698, 0.2476149531070420,
+                                0.2969311120382472, 0.3978149645802642,
+                                0.5270854589440328, 0.6981269994175695,
+                                0.8190890835352128, 0.8527059887098624,
+                                0.8604711817462826, 0.8627060376969976,
+                                0.8734213127600976};
+
+// number of stages (0 for an unknown scheme) and the three tables
+int lsrk_tables(int scheme, const double **A, const double **B, const double **C)
+{
+    if (scheme == RV_LSRK54) {
+        *A = a54, *B = b54, *C = c54;
+        return 5;
+    }
+    if (scheme == RV_LSRK144) {
+        *A = a144, *B = b144, *C = c144;
+        return 14;
+    }
+    return 0;
+}
+}  // namespace
+
+int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstages)
+{
+    const double *A, *B, *C;
+    const int n = lsrk_tables(scheme, &A, &B, &C);
+    if (n == 0)
+        return RV_E_ARG;
+    if (nstages)
+        *nstages = n;
+    for (int s = 0; s < n; ++s) {
+        if (a)
+            a[s] = A[s];
+        if (b)
+            b[s] = B[s];
+        if (c)
+            c[s] = C[s];
+    }
+    return RV_OK;
+}
+
 int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
                 double timeend, double *t_out, int64_t *nsteps_out)
 {
     if (!h || !q || !dq)
         return fail(h, RV_E_ARG, "null argument");
-    // coefficients (odesolvers.jl:163-244) rounded to the state precision
-    static const double a54[5] = {0.0, -567301805773.0 / 1357537059087.0,
-                                  -2404267990393.0 / 2016746695238.0,
-                                  -3550918686646.0 / 2091501179385.0,
-                                  -1275806237668.0 / 842570457699.0};
-    static const double b54[5] = {1432997174477.0 / 9575080441755.0,
-                                  5161836677717.0 / 13612068292357.0,
-                                  1720146321549.0 / 2090206949498.0,
-                                  3134564353537.0 / 4481467310338.0,
-                                  2277821191437.0 / 14882151754819.0};
-    static const double c54[5] = {0.0, 1432997174477.0 / 9575080441755.0,
-                                  2526269341429.0 / 6820363962896.0,
-                                  2006345519317.0 / 3224310063776.0,
-                                  2802321613138.0 / 2924317926251.0};
-    static const double a144[14] = {0, -0.7188012108672410, -0.7785331173421570,
-                                    -0.0053282796654044, -0.8552979934029281,
-                                    -3.9564138245774565, -1.5780575380587385,
-                                    -2.0837094552574054, -0.7483334182761610,
-                                    -0.7032861106563359, 0.0013917096117681,
-                                    -0.0932075369637460, -0.9514200470875948,
-                                    -7.1151571693922548};
-    static const double b144[14] = {0.0367762454319673, 0.3136296607553959, 0.1531848691869027,
-                                    0.0030097086818182, 0.3326293790646110, 0.2440251405350864,
-                                    0.3718879239592277, 0.6204126221582444, 0.1524043173028741,
-                                    0.0760894927419266, 0.0077604214040978, 0.0024647284755382,
-                                    0.0780348340049386, 5.5059777270269628};
-    static const double c144[14] = {0, 0.0367762454319673, 0.1249685262725025,
-                                    0.2446177702277698, 0.2476149531070420,
-                                    0.2969311120382472, 0.3978149645802642,
-                                    0.5270854589440328, 0.6981269994175695,
-                                    0.8190890835352128, 0.8527059887098624,
-                                    0.8604711817462826, 0.8627060376969976,
-                                    0.8734213127600976};
     const double *A, *B, *C;
-    int nst;
-    if (scheme == RV_LSRK54) {
-        A = a54, B = b54, C = c54, nst = 5;
-    } else if (scheme == RV_LSRK144) {
-        A = a144, B = b144, C = c144, nst = 14;
-    } else {
+    const int nst = lsrk_tables(scheme, &A, &B, &C);
+    if (nst == 0)
         return fail(h, RV_E_ARG, "unknown LSRK scheme");
-    }
+    (void)C;  // the laws here do not depend on the stage time
     const bool f32 = h->dtype == RV_F32;
     const size_t bytes = (size_t)h->Np * h->ns * h->K * (f32 ? 4 : 8);
     if (!h->qtmp) {
@@ -926,7 +960,6 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
                 return rc;
             std::swap(cur, alt);
         }
-        (void)C;
         time = f32 ? (double)((float)time + (float)step_dt) : time + step_dt;
         if (final_step)
             break;

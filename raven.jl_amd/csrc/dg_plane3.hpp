@@ -64,6 +64,46 @@ struct Plane3 {
     using S = Stage<3, NQ, LAW, R, GEO, FLUX>;
 };
 
+// Each wave works in its own LDS slab and a wavefront's LDS operations
+// complete in issue order, so ordering the compiler's memory operations is all
+// the synchronisation the slab needs (no s_barrier tying the block's waves).
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A value every lane of the wave holds, moved to scalar registers.
+__device__ inline int64_t wave_uniform(int64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// q+ at node `pen` of a face from its neighbour record (conforming neighbour or
+// explicit offset); self-mapped faces (FACE_SELF) are filled from q- later.
+template <int NQ, int NS, typename R>
+__device__ inline void load_neighbour(const StageParams &P, const R *qbase, const FaceRec rec,
+                                      int pen, R *qp)
+{
+    constexpr int NP = NQ * NQ * NQ;
+    if (rec.x >= 0) {
+        const int fn = rec.y & 7, o = (rec.y >> 3) & 7;
+        const int node = face_node<3, NQ>(fn, orient_node<3, NQ>(o, pen));
+        const R *qn = qbase + (int64_t)rec.x * (NS * NP) + node;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            qp[s] = qn[s * NP];
+    } else if (rec.x == FACE_EXPLICIT) {
+        const int64_t off = P.explicit_idx[(int64_t)rec.y + pen];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            qp[s] = qbase[off + s * NP];
+    }
+}
+
 template <int NQ, int LAW, typename R, int GEO, int FLUX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 dg_plane3_kernel(const StageParams P)
@@ -82,7 +122,9 @@ dg_plane3_kernel(const StageParams P)
     const int l = lane - slot * NQ2;  // column index = i + NQ*j
     const int i = l % NQ, j = l / NQ;
     const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t e = (blk * wpb + wave) * T::EPW + slot;
+    int64_t e = (blk * wpb + wave) * T::EPW + slot;
+    if constexpr (T::EPW == 1)
+        e = wave_uniform(e);  // one cell per wave: per-cell data comes in scalar loads
     const bool active = (slot < T::EPW) && (e < P.K);
     R *slab = lds + (wave * T::EPW + (slot < T::EPW ? slot : 0)) * T::SLAB;
     const Law<3, LAW, R> law(P);
@@ -91,12 +133,33 @@ dg_plane3_kernel(const StageParams P)
                dz_on = (P.dir_mask >> 2) & 1;
 
     R q[NS][NQ], acc[NS][NQ];
-    R qpz[2][NS];
     R cx[NQ], cy[NQ];  // this lane's x / y derivative weights
     R sz = R(1);
+    FaceRec rec[6];
     const R a = R(P.a), bdt = R(P.b_dt);
+    // a*dq_old is formed at the first z scatter so that the dq loads need not
+    // complete before the first plane's flux
+    const bool late_scale = GEO == GEO_AFFINE && P.read_dq && dz_on;
 
     if (active) {
+        // every load of the cell is issued before the first wait: derivative
+        // weights, geometry factors, face records, then q and dq
+        const double *M = (GEO == GEO_AFFINE) ? P.Dw : P.Dt;
+        double mi[NQ], mj[NQ];
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            mi[n] = M[i * NQ + n];
+            mj[n] = M[j * NQ + n];
+        }
+        R sx = R(1), sy = R(1);
+        if constexpr (GEO == GEO_AFFINE) {
+            sx = R(P.affine[e * 3 + 0]);
+            sy = R(P.affine[e * 3 + 1]);
+            sz = R(P.affine[e * 3 + 2]);
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            rec[f] = P.faces[e * 6 + f];
         const R *qe = qbase + e * (NS * NP) + l;
 #pragma unroll
         for (int k = 0; k < NQ; ++k)
@@ -109,7 +172,14 @@ dg_plane3_kernel(const StageParams P)
             for (int k = 0; k < NQ; ++k)
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    acc[s][k] = a * dqe[s * NP + k * NQ2];
+                    acc[s][k] = dqe[s * NP + k * NQ2];
+            if (!late_scale) {
+#pragma unroll
+                for (int k = 0; k < NQ; ++k)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        acc[s][k] = a * acc[s][k];
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < NQ; ++k)
@@ -117,17 +187,11 @@ dg_plane3_kernel(const StageParams P)
                 for (int s = 0; s < NS; ++s)
                     acc[s][k] = R(0);
         }
-        const double *M = (GEO == GEO_AFFINE) ? P.Dw : P.Dt;
-        R sx = R(1), sy = R(1);
-        if constexpr (GEO == GEO_AFFINE) {
-            sx = R(P.affine[e * 3 + 0]);
-            sy = R(P.affine[e * 3 + 1]);
-            sz = R(P.affine[e * 3 + 2]);
-        }
+        const R fx = dx_on ? sx : R(0), fy = dy_on ? sy : R(0);
 #pragma unroll
         for (int n = 0; n < NQ; ++n) {
-            cx[n] = dx_on ? sx * R(M[i * NQ + n]) : R(0);
-            cy[n] = dy_on ? sy * R(M[j * NQ + n]) : R(0);
+            cx[n] = fx * R(mi[n]);
+            cy[n] = fy * R(mj[n]);
         }
     }
 
@@ -178,8 +242,12 @@ dg_plane3_kernel(const StageParams P)
             for (int kk = 0; kk < NQ; ++kk) {
                 const R w = R(M[kk * NQ + k]);  // contribution of plane k to output kk
 #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    acc[s][kk] = acc[s][kk] + w * f3[s];
+                for (int s = 0; s < NS; ++s) {
+                    R base = acc[s][kk];
+                    if (k == 0 && late_scale)
+                        base = a * base;
+                    acc[s][kk] = base + w * f3[s];
+                }
             }
         }
     };
@@ -215,7 +283,7 @@ dg_plane3_kernel(const StageParams P)
     };
     if (active)
         plane_flux(0);
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         if (active) {
@@ -223,20 +291,18 @@ dg_plane3_kernel(const StageParams P)
                 plane_flux(k + 1);
             gather(k);
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 
-    // neighbour states of the two z faces (faces 4, 5; face node = column),
-    // issued here so their latency overlaps the x / y face exchange
-    if (active && dz_on) {
-        R qm0[NS], qm1[NS];
+    // ---- neighbour states of all six faces, issued together: node l of each
+    // x / y face (one node per lane) and the ends of the lane's own column for
+    // the two z faces
+    R qp[6][NS];
+    if (active) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            qm0[s] = q[s][0];
-            qm1[s] = q[s][NQ - 1];
-        }
-        S::neighbour_state(P, qbase, e, 4, l, qm0, qpz[0]);
-        S::neighbour_state(P, qbase, e, 5, l, qm1, qpz[1]);
+        for (int f = 0; f < 6; ++f)
+            if ((P.dir_mask >> (f >> 1)) & 1)
+                load_neighbour<NQ, NS, R>(P, qbase, rec[f], l, qp[f]);
     }
 
     // ---- x and y faces: owners publish q-, every lane evaluates one node of
@@ -257,37 +323,72 @@ dg_plane3_kernel(const StageParams P)
                     fb[(3 * NS + s) * NQ2 + i + NQ * k] = q[s][k];
             }
         }
+        // z faces, in the lane's own column
+        if (dz_on) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int kend = side ? NQ - 1 : 0;
+                R qm[NS], nf[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    qm[s] = q[s][kend];
+                if (rec[4 + side].x == FACE_SELF) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        qp[4 + side][s] = qm[s];
+                }
+                R lift;
+                if constexpr (GEO == GEO_AFFINE) {
+                    R fm[NS];
+                    law.flux_row(2, qm, fm);
+                    S::nflux_axis(law, 2, side ? R(1) : R(-1), fm, qm, qp[4 + side], nf);
+                    lift = sz * R(P.winv[kend]);
+                } else {
+                    const double *sg = P.sgeo + ((e * 6 + 4 + side) * 4) * NQ2 + l;
+                    R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
+                    lift = R(sg[3 * NQ2]);
+                    S::nflux_general(law, nv, qm, qp[4 + side], nf);
+                }
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[s][kend] = acc[s][kend] - lift * nf[s];
+            }
+        }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (active) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int d = f >> 1;
             if (!((P.dir_mask >> d) & 1))
                 continue;
-            R qm[NS], qp[NS], nf[NS];
+            R qm[NS], nf[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s)
                 qm[s] = fb[(f * NS + s) * NQ2 + l];
-            S::neighbour_state(P, qbase, e, f, l, qm, qp);
+            if (rec[f].x == FACE_SELF) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    qp[f][s] = qm[s];
+            }
             R lift;
             if constexpr (GEO == GEO_AFFINE) {
                 R fm[NS];
                 law.flux_row(d, qm, fm);
-                S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, qm, qp, nf);
+                S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, qm, qp[f], nf);
                 lift = R(P.affine[e * 3 + d]) * R(P.winv[(f & 1) ? NQ - 1 : 0]);
             } else {
                 const double *sg = P.sgeo + ((e * 6 + f) * 4) * NQ2 + l;
                 R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
                 lift = R(sg[3 * NQ2]);
-                S::nflux_general(law, nv, qm, qp, nf);
+                S::nflux_general(law, nv, qm, qp[f], nf);
             }
 #pragma unroll
             for (int s = 0; s < NS; ++s)
                 fb[(f * NS + s) * NQ2 + l] = lift * nf[s];
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (active) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
@@ -303,32 +404,6 @@ dg_plane3_kernel(const StageParams P)
                 if (dy_on && j == NQ - 1)
                     v = v - fb[(3 * NS + s) * NQ2 + i + NQ * k];
                 acc[s][k] = v;
-            }
-        }
-        // z faces: in the lane's own column
-        if (dz_on) {
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const int kend = side ? NQ - 1 : 0;
-                R qm[NS], nf[NS];
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    qm[s] = q[s][kend];
-                R lift;
-                if constexpr (GEO == GEO_AFFINE) {
-                    R fm[NS];
-                    law.flux_row(2, qm, fm);
-                    S::nflux_axis(law, 2, side ? R(1) : R(-1), fm, qm, qpz[side], nf);
-                    lift = sz * R(P.winv[kend]);
-                } else {
-                    const double *sg = P.sgeo + ((e * 6 + 4 + side) * 4) * NQ2 + l;
-                    R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
-                    lift = R(sg[3 * NQ2]);
-                    S::nflux_general(law, nv, qm, qpz[side], nf);
-                }
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    acc[s][kend] = acc[s][kend] - lift * nf[s];
             }
         }
         // ---- update and store

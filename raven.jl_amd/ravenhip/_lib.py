@@ -120,6 +120,8 @@ def lib():
         "rv_brick_arrays": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), c_f64p, c_f64p, c_f64p,
                                            c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
         "rv_lobatto": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p]),
+        "rv_lsrk_coefficients": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p,
+                                                ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -9,20 +9,6 @@ import ctypes
 
 from . import _lib as L
 
-# odesolvers.jl:163-189
-_LSRK54 = (
-    (0, (-567301805773, 1357537059087), (-2404267990393, 2016746695238),
-     (-3550918686646, 2091501179385), (-1275806237668, 842570457699)),
-    ((1432997174477, 9575080441755), (5161836677717, 13612068292357),
-     (1720146321549, 2090206949498), (3134564353537, 4481467310338),
-     (2277821191437, 14882151754819)),
-)
-
-
-def _frac(v):
-    return float(v) if not isinstance(v, tuple) else v[0] / v[1]
-
-
 class LSRK:
     def __init__(self, dg, q, dt, t0, scheme):
         self.dg = dg
@@ -32,11 +18,15 @@ class LSRK:
         self.dq = dg.GridArray(0.0)
 
     def coefficients(self):
-        if self.scheme == L.RV_LSRK54:
-            a = [_frac(v) for v in _LSRK54[0]]
-            b = [_frac(v) for v in _LSRK54[1]]
-            return a, b
-        raise NotImplementedError("per-stage stepping is provided for LSRK54")
+        """(rka, rkb, rkc) of the scheme as Python floats, read from the
+        library (coefficients_lsrk54/144, odesolvers.jl:163-244)."""
+        import numpy as np
+        a, b, c = (np.zeros(14) for _ in range(3))
+        n = ctypes.c_int(0)
+        L.check(L.lib().rv_lsrk_coefficients(self.scheme, L.ptr(a, ctypes.c_double), L.ptr(b, ctypes.c_double),
+                                             L.ptr(c, ctypes.c_double), ctypes.byref(n)))
+        k = n.value
+        return [float(v) for v in a[:k]], [float(v) for v in b[:k]], [float(v) for v in c[:k]]
 
 
 def LSRK54(dg, q, dt, t0=0.0):
@@ -60,12 +50,14 @@ def solve_(q, timeend, solver):
 
 
 def dostep_(q, solver, qtmp):
-    """One LSRK54 step through the per-stage entry point; q updated in place
-    (qtmp is the ping-pong buffer; an odd stage count ends with a copy)."""
-    a, b = solver.coefficients()
+    """dostep!(q, lsrk) (odesolvers.jl:50-61) through the per-stage entry
+    point, stage times time + rkc*dt; q updated in place (qtmp is the
+    ping-pong buffer; an odd stage count ends with a copy)."""
+    a, b, c = solver.coefficients()
     cur, alt = q, qtmp
     for s in range(len(a)):
-        solver.dg.lsrk_stage(alt, cur, solver.dq, a[s], b[s] * solver.dt, solver.time)
+        solver.dg.lsrk_stage(alt, cur, solver.dq, a[s], b[s] * solver.dt,
+                             solver.time + c[s] * solver.dt)
         cur, alt = alt, cur
     if cur is not q:
         q.copy_(cur)

@@ -316,3 +316,55 @@ def test_full_size_c2_properties(torch, R):
     absints = dg.integrals(torch.abs(dq).contiguous())
     say(f"   sum MJ dq = {ints}, sum MJ |dq| = {absints}")
     assert np.all(np.abs(ints) <= 1e-10 * absints)
+
+
+def _gaussian_pulse_2d(P):
+    """exp(-((x-0.5)^2 + (y-0.5)^2)/0.01) at every node."""
+    x, y = P[:, 0, :], P[:, 1, :]
+    return np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01).reshape(-1).copy()
+
+
+def test_small_advection_100_steps_per_stage_rhs(torch, R):
+    """2-D linear advection, periodic 16x16 quads on [0,1]^2, Nq = 5,
+    u = (1, 0.5), upwind (Rusanov) flux, Gaussian pulse, LSRK54 at CFL 1/2
+    (dt = cfl*step(v1d)/N/norm(u), wave_2d.jl:37-38) for 100 steps.  Every
+    stage's right-hand side and the final state are compared with the oracle
+    run in lockstep; both sides advance their own state."""
+    u = [1.0, 0.5]
+    coords = [PB.julia_range(0, 1, 17)] * 2
+    grid, dg, og, odg = make_pair(R, 2, 5, coords, (True, True), "advection", u=u)
+    q = _gaussian_pulse_2d(og.point_fields())
+    dt = 0.5 * PB.julia_step(0, 1, 17) / 4 / float(np.sqrt(1.25))
+    a, b, c = R.odesolvers.LSRK54(dg, to_dev(torch, dg, q), dt).coefficients()
+    qo, dqo = q.copy(), np.zeros(q.size)
+    bufs = [to_dev(torch, dg, q), dg.GridArray(0.0)]
+    dqd, Ld = dg.GridArray(0.0), dg.GridArray(0.0)
+    cur, worst = 0, 0.0
+    for step in range(100):
+        t = step * dt
+        for s in range(5):
+            ts = t + c[s] * dt
+            Lo = np.zeros(q.size)
+            odg.rhs(Lo, qo, ts, increment=False)
+            dg(Ld, bufs[cur], ts, increment=False)
+            dg.synchronize()
+            e = rel_l2(Ld.cpu().numpy().reshape(-1), Lo)
+            worst = max(worst, e)
+            assert e <= RTOL_F64, f"step {step} stage {s}: RHS rel L2 {e:.3e}"
+            # Raven's stage: dq .*= a; dg(dq, q, t; increment = true); q .+= b*dt*dq
+            dqo *= a[s]
+            odg.rhs(dqo, qo, ts, increment=True)
+            qo += (b[s] * dt) * dqo
+            dg.lsrk_stage(bufs[1 - cur], bufs[cur], dqd, a[s], b[s] * dt, ts)
+            cur = 1 - cur
+    dg.synchronize()
+    err = rel_l2(bufs[cur].cpu().numpy().reshape(-1), qo)
+    say(f"   worst per-stage RHS rel L2 = {worst:.3e}; final state rel L2 = {err:.3e}")
+    assert err <= RTOL_F64
+    # the whole-run driver lands on the same state
+    qd = to_dev(torch, dg, q)
+    solver = R.LSRK54(dg, qd, dt)
+    n = R.solve_(qd, 100 * dt, solver)
+    dg.synchronize()
+    assert n == 100
+    assert rel_l2(qd.cpu().numpy().reshape(-1), qo) <= RTOL_F64

@@ -87,3 +87,30 @@ def test_nonuniform_brick_is_rejected():
     cell = R.LobattoCell(4, 4)
     with pytest.raises(NotImplementedError):
         R.generate(R.GridManager(cell, R.brick([[0.0, 0.3, 1.0], [0.0, 1.0]], (True, True))))
+
+
+@pytest.mark.parametrize("scheme,nst,order", [(54, 5, 4), (144, 14, 4)])
+def test_lsrk_coefficients_of_the_library(scheme, nst, order):
+    """The library's tables equal the oracle's bit for bit, and a 2N-storage
+    run of y' = y with them converges at the scheme's order."""
+    from ravenhip.odesolvers import LSRK
+
+    class _Dummy:
+        def GridArray(self, fill):
+            return None
+
+    a, b, c = LSRK(_Dummy(), None, 0.1, 0.0, scheme).coefficients()
+    oa, ob, oc = O.lsrk_coefficients(scheme)
+    assert len(a) == nst
+    assert np.array_equal(a, oa) and np.array_equal(b, ob) and np.array_equal(c, oc)
+
+    def run(h):
+        y, dy = 1.0, 0.0
+        for _ in range(round(1 / h)):
+            for s in range(nst):
+                dy = a[s] * dy + y
+                y += b[s] * h * dy
+        return abs(y - np.e)
+
+    e1, e2 = run(1 / 8), run(1 / 16)
+    assert np.log2(e1 / e2) > order - 0.3
