@@ -134,6 +134,14 @@ int rv_synchronize(rv_handle *h);
 /* the hipStream_t every call of this handle is ordered on */
 void *rv_stream(const rv_handle *h);
 
+/* Launch shape of the stage kernels: waves per workgroup (1, 2 or 4; each wave
+ * keeps its own cells and LDS slab, so results do not depend on it).  The
+ * library picks a default per configuration; an autotuner that times the
+ * choices on the caller's grid may override it.  No counterpart in Raven
+ * (KernelAbstractions picks its workgroup size at launch). */
+int rv_set_waves_per_block(rv_handle *h, int waves_per_block);
+int rv_get_waves_per_block(const rv_handle *h);
+
 /* --- the hot path ------------------------------------------------------ */
 
 /* dg(dq, q, t; increment) (dgsem.jl:84-126):
@@ -159,6 +167,33 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
  * callers that step stage by stage through rv_lsrk_stage.  Any array pointer
  * may be NULL; *nstages receives 5 or 14.  No handle, no device work. */
 int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstages);
+
+/* --- batches of independent solves -------------------------------------- */
+
+/* Many operators advanced together: ONE persistent launch per LSRK stage walks
+ * a work list of (member, first cell) items, grouped by (Nq, law, precision),
+ * so solves of different degree, size and precision share each launch.  Members
+ * are handles from rv_create / rv_create_brick with the same dim and stream,
+ * affine geometry, the Rusanov flux and all directions; 2-D members may have
+ * Nq 2..8, 3-D members Nq 4..8.  The handles must outlive the batch.  Each
+ * member's result is the one its own rv_lsrk_stage gives.  No counterpart in
+ * Raven, which advances one solver per `solve!` call (odesolvers.jl:9-30). */
+typedef struct rv_batch rv_batch;
+int rv_batch_create(rv_handle *const *members, int32_t nmembers, rv_batch **out);
+void rv_batch_destroy(rv_batch *b);
+const char *rv_batch_last_error(const rv_batch *b);
+/* work-list length, persistent waves launched, LDS bytes per wave */
+int rv_batch_info(const rv_batch *b, int64_t *nitems, int64_t *nwaves, int32_t *slab_bytes);
+/* One fused stage for every member m (arrays of nmembers entries):
+ *   dq[m] = a[m]*dq[m] + L_m(q_in[m]);  q_out[m] = q_in[m] + b_dt[m]*dq[m]. */
+int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, void *const *dq,
+                   const double *a, const double *b_dt);
+/* nsteps whole LSRK steps of every member with its own dt[m]: dq[m] is zeroed,
+ * each step replays a captured hipGraph of the scheme's stages, and q[m] holds
+ * the result (the library ping-pongs through buffers of its own).  The graphs
+ * are kept while q, dq, scheme and dt stay the same. */
+int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, const double *dt,
+                      int64_t nsteps);
 
 /* --- diagnostics ------------------------------------------------------- */
 

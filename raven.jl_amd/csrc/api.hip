@@ -334,7 +334,7 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
     h->launcher = find_stage_launcher(key);
     // waves per workgroup measured on MI355X for the 3-D column kernel at
     // Nq = 8: two for FP64 (256 VGPRs, 2 waves/SIMD), four for FP32
-    h->wpb = h->dim == 3 ? (h->dtype == RV_F64 ? 2 : 4) : 1;
+    h->wpb = h->dim == 3 && h->dtype == RV_F32 ? 2 : 1;
     if (!h->launcher)
         return fail(h, RV_E_UNSUPPORTED,
                     "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +
@@ -805,6 +805,18 @@ int rv_synchronize(rv_handle *h)
 
 void *rv_stream(const rv_handle *h) { return h ? (void *)h->stream : nullptr; }
 
+int rv_set_waves_per_block(rv_handle *h, int waves_per_block)
+{
+    if (!h)
+        return fail(nullptr, RV_E_ARG, "null handle");
+    if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+        return fail(h, RV_E_ARG, "waves per block must be 1, 2 or 4");
+    h->wpb = waves_per_block;
+    return RV_OK;
+}
+
+int rv_get_waves_per_block(const rv_handle *h) { return h ? h->wpb : 0; }
+
 int rv_rhs(rv_handle *h, void *dq, const void *q, double t, int increment)
 {
     (void)t;  // no time-dependent sources for the laws here (balancelaw.jl:23-26)
@@ -1090,6 +1102,331 @@ int rv_lobatto(int nq, double *points, double *weights, double *D)
     if (nq < 2 || nq > 32 || !points || !weights || !D)
         return fail(nullptr, RV_E_ARG, "invalid argument");
     lobatto_operators(nq, points, weights, D);
+    return RV_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches: many operators, one persistent launch per stage (batch.hip).
+// ===========================================================================
+struct rv_batch {
+    static constexpr int RING = 4;  // parameter tables in flight for rv_batch_stage
+    std::string err;
+    int dim = 0, wpb = 2, slab_bytes = 0;
+    int64_t nitems = 0, nwaves = 0;
+    hipStream_t stream = nullptr;          // the first member's stream
+    hipStream_t capture = nullptr;         // private stream graphs are captured on
+    std::vector<rv_handle *> members;
+    BatchItem *items = nullptr;            // device work list
+    StageParams *host_ring = nullptr;      // pinned, RING * n
+    StageParams *dev_ring = nullptr;       // device, RING * n
+    hipEvent_t ring_ev[RING] = {};
+    int ring = 0;
+    // rv_batch_lsrk_run: ping-pong states, stage tables and step graphs
+    std::vector<void *> qtmp;
+    StageParams *dev_tables = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<const void *> graph_key;   // pointers, scheme and dt the graphs were built for
+    int64_t device_bytes = 0;
+};
+
+namespace {
+
+int bfail(rv_batch *b, int code, const std::string &msg)
+{
+    if (b)
+        b->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+void drop_graphs(rv_batch *b)
+{
+    for (auto &e : b->exec)
+        if (e) {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+        }
+    b->graph_key.clear();
+}
+
+void free_batch(rv_batch *b)
+{
+    if (!b)
+        return;
+    if (b->stream)
+        (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (void *p : b->qtmp)
+        if (p)
+            (void)hipFree(p);
+    void *bufs[] = {b->items, b->dev_ring, b->dev_tables};
+    for (void *p : bufs)
+        if (p)
+            (void)hipFree(p);
+    if (b->host_ring)
+        (void)hipHostFree(b->host_ring);
+    for (auto &ev : b->ring_ev)
+        if (ev)
+            (void)hipEventDestroy(ev);
+    if (b->capture)
+        (void)hipStreamDestroy(b->capture);
+    delete b;
+}
+
+#define BHIP_TRY(b, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return bfail(b, RV_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
+    } while (0)
+
+int batch_launch(rv_batch *b, const StageParams *table, hipStream_t st)
+{
+    if (launch_batch(b->dim, table, b->items, b->nitems, b->nwaves, b->slab_bytes, b->wpb, st)) {
+        hipError_t e = hipGetLastError();
+        return bfail(b, RV_E_HIP, std::string("batch launch failed: ") + hipGetErrorString(e));
+    }
+    return RV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
+{
+    if (!out)
+        return bfail(nullptr, RV_E_ARG, "null output");
+    *out = nullptr;
+    if (!members || n <= 0)
+        return bfail(nullptr, RV_E_ARG, "a batch needs at least one member");
+    rv_batch *b = new rv_batch();
+    b->members.assign(members, members + n);
+    std::vector<int> kind(n), epw(n);
+    for (int m = 0; m < n; ++m) {
+        rv_handle *h = members[m];
+        if (!h) {
+            free_batch(b);
+            return bfail(nullptr, RV_E_ARG, "null member");
+        }
+        if (m == 0) {
+            b->dim = h->dim;
+            b->stream = h->stream;
+        }
+        if (h->dim != b->dim || h->stream != b->stream) {
+            free_batch(b);
+            return bfail(nullptr, RV_E_ARG, "batch members must share dim and stream");
+        }
+        if (h->geo != GEO_AFFINE || h->flux != RV_FLUX_RUSANOV || h->dir_mask != (1 << h->dim) - 1) {
+            free_batch(b);
+            return bfail(nullptr, RV_E_UNSUPPORTED,
+                         "batch members need affine geometry, the Rusanov flux and all directions");
+        }
+        int sb = 0;
+        if (!batch_kind(h->dim, h->nq, h->law, h->dtype == RV_F32 ? 4 : 8, &kind[m], &epw[m], &sb)) {
+            free_batch(b);
+            return bfail(nullptr, RV_E_UNSUPPORTED, "no batch kernel for this (dim, Nq)");
+        }
+        b->slab_bytes = std::max(b->slab_bytes, sb);
+    }
+    // work list: members grouped by kind, each member's cells in order
+    std::vector<int> order(n);
+    for (int m = 0; m < n; ++m)
+        order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kind[x] < kind[y]; });
+    std::vector<BatchItem> items;
+    for (int m : order)
+        for (int64_t c = 0; c < b->members[m]->K; c += epw[m])
+            items.push_back(BatchItem{m, kind[m], c});
+    b->nitems = (int64_t)items.size();
+    b->slab_bytes = (b->slab_bytes + 15) / 16 * 16;
+    auto bail = [&](const char *what, hipError_t e) {
+        free_batch(b);
+        return bfail(nullptr, RV_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&b->items, std::max<size_t>(1, items.size()) * sizeof(BatchItem))))
+        return bail("hipMalloc", e);
+    if ((e = hipMemcpy(b->items, items.data(), items.size() * sizeof(BatchItem),
+                       hipMemcpyHostToDevice)))
+        return bail("hipMemcpy", e);
+    if ((e = hipHostMalloc((void **)&b->host_ring, sizeof(StageParams) * n * rv_batch::RING)))
+        return bail("hipHostMalloc", e);
+    if ((e = hipMalloc(&b->dev_ring, sizeof(StageParams) * n * rv_batch::RING)))
+        return bail("hipMalloc", e);
+    for (auto &ev : b->ring_ev)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)))
+            return bail("hipEventCreate", e);
+    if ((e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking)))
+        return bail("hipStreamCreate", e);
+    b->device_bytes = (int64_t)(items.size() * sizeof(BatchItem) +
+                                sizeof(StageParams) * n * rv_batch::RING);
+    int64_t resident = 0;
+    if (batch_resident_waves(b->dim, b->wpb, b->slab_bytes, &resident) != 0 || resident <= 0) {
+        free_batch(b);
+        return bfail(nullptr, RV_E_HIP, "occupancy query of the batch kernel failed");
+    }
+    b->nwaves = std::min<int64_t>(resident, std::max<int64_t>(b->nitems, 1));
+    *out = b;
+    return RV_OK;
+}
+
+void rv_batch_destroy(rv_batch *b) { free_batch(b); }
+
+const char *rv_batch_last_error(const rv_batch *b)
+{
+    return b ? b->err.c_str() : g_last_error.c_str();
+}
+
+int rv_batch_info(const rv_batch *b, int64_t *nitems, int64_t *nwaves, int32_t *slab_bytes)
+{
+    if (!b)
+        return bfail(nullptr, RV_E_ARG, "null batch");
+    if (nitems)
+        *nitems = b->nitems;
+    if (nwaves)
+        *nwaves = b->nwaves;
+    if (slab_bytes)
+        *slab_bytes = b->slab_bytes;
+    return RV_OK;
+}
+
+int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, void *const *dq,
+                   const double *a, const double *b_dt)
+{
+    if (!b || !q_out || !q_in || !dq || !a || !b_dt)
+        return bfail(b, RV_E_ARG, "null argument");
+    const int n = (int)b->members.size();
+    const int slot = b->ring;
+    StageParams *host = b->host_ring + (size_t)slot * n;
+    StageParams *dev = b->dev_ring + (size_t)slot * n;
+    // the pinned slot is rewritten only after its previous upload completed
+    BHIP_TRY(b, hipEventSynchronize(b->ring_ev[slot]));
+    for (int m = 0; m < n; ++m) {
+        if (!q_out[m] || !q_in[m] || !dq[m])
+            return bfail(b, RV_E_ARG, "null state pointer");
+        if (q_out[m] == q_in[m])
+            return bfail(b, RV_E_ARG, "q_out must not alias q_in");
+        StageParams P = stage_params(b->members[m]);
+        P.q_in = q_in[m];
+        P.q_out = q_out[m];
+        P.dq = dq[m];
+        P.a = a[m];
+        P.b_dt = b_dt[m];
+        P.read_dq = a[m] != 0.0 ? 1 : 0;
+        P.write_q = 1;
+        host[m] = P;
+    }
+    BHIP_TRY(b, hipMemcpyAsync(dev, host, sizeof(StageParams) * n, hipMemcpyHostToDevice,
+                               b->stream));
+    BHIP_TRY(b, hipEventRecord(b->ring_ev[slot], b->stream));
+    b->ring = (slot + 1) % rv_batch::RING;
+    return batch_launch(b, dev, b->stream);
+}
+
+int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, const double *dt,
+                      int64_t nsteps)
+{
+    if (!b || !q || !dq || !dt || nsteps < 0)
+        return bfail(b, RV_E_ARG, "invalid argument");
+    const double *A, *B, *C;
+    const int nst = lsrk_tables(scheme, &A, &B, &C);
+    if (nst == 0)
+        return bfail(b, RV_E_ARG, "unknown LSRK scheme");
+    (void)C;
+    const int n = (int)b->members.size();
+    for (int m = 0; m < n; ++m)
+        if (!q[m] || !dq[m])
+            return bfail(b, RV_E_ARG, "null state pointer");
+    if (b->qtmp.empty()) {
+        b->qtmp.assign(n, nullptr);
+        for (int m = 0; m < n; ++m) {
+            rv_handle *h = b->members[m];
+            const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+            BHIP_TRY(b, hipMalloc(&b->qtmp[m], bytes ? bytes : 1));
+            b->device_bytes += (int64_t)bytes;
+        }
+    }
+    // the graphs bake pointers and coefficients: rebuild when any changed
+    std::vector<const void *> key;
+    key.push_back(reinterpret_cast<const void *>((intptr_t)scheme));
+    for (int m = 0; m < n; ++m) {
+        key.push_back(q[m]);
+        key.push_back(dq[m]);
+        int64_t bits;
+        std::memcpy(&bits, &dt[m], sizeof bits);
+        key.push_back(reinterpret_cast<const void *>((intptr_t)bits));
+    }
+    const int parities = (nst % 2) ? 2 : 1;
+    if (key != b->graph_key) {
+        drop_graphs(b);
+        if (b->dev_tables) {
+            (void)hipFree(b->dev_tables);
+            b->dev_tables = nullptr;
+        }
+        std::vector<StageParams> tab((size_t)parities * nst * n);
+        for (int p = 0; p < parities; ++p)
+            for (int s = 0; s < nst; ++s)
+                for (int m = 0; m < n; ++m) {
+                    rv_handle *h = b->members[m];
+                    const bool f32 = h->dtype == RV_F32;
+                    void *bufs[2] = {q[m], b->qtmp[m]};
+                    const int cur = (p + s) % 2;
+                    StageParams P = stage_params(h);
+                    P.q_in = bufs[cur];
+                    P.q_out = bufs[1 - cur];
+                    P.dq = dq[m];
+                    // coefficients rounded to the state type as in rv_lsrk_run
+                    P.a = f32 ? (double)(float)A[s] : A[s];
+                    P.b_dt = f32 ? (double)((float)B[s] * (float)dt[m]) : B[s] * dt[m];
+                    P.read_dq = P.a != 0.0 ? 1 : 0;
+                    P.write_q = 1;
+                    tab[((size_t)p * nst + s) * n + m] = P;
+                }
+        BHIP_TRY(b, hipMalloc(&b->dev_tables, tab.size() * sizeof(StageParams)));
+        BHIP_TRY(b, hipMemcpy(b->dev_tables, tab.data(), tab.size() * sizeof(StageParams),
+                              hipMemcpyHostToDevice));
+        for (int p = 0; p < parities; ++p) {
+            hipGraph_t g = nullptr;
+            BHIP_TRY(b, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+            int rc = RV_OK;
+            for (int s = 0; s < nst && rc == RV_OK; ++s)
+                rc = batch_launch(b, b->dev_tables + ((size_t)p * nst + s) * n, b->capture);
+            hipError_t ec = hipStreamEndCapture(b->capture, &g);
+            if (rc != RV_OK) {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess)
+                return bfail(b, RV_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+            hipError_t ei = hipGraphInstantiate(&b->exec[p], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess)
+                return bfail(b, RV_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        b->graph_key = key;
+    }
+    // solve! starts from dq = 0 (odesolvers.jl:43-44); the clear is outside the graphs
+    for (int m = 0; m < n; ++m) {
+        rv_handle *h = b->members[m];
+        const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+        BHIP_TRY(b, hipMemsetAsync(dq[m], 0, bytes, b->stream));
+    }
+    int cur = 0;  // 0: state in q, 1: in qtmp
+    for (int64_t step = 0; step < nsteps; ++step) {
+        BHIP_TRY(b, hipGraphLaunch(b->exec[parities == 2 ? cur : 0], b->stream));
+        cur = (cur + nst) % 2;
+    }
+    if (cur == 1)
+        for (int m = 0; m < n; ++m) {
+            rv_handle *h = b->members[m];
+            const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+            BHIP_TRY(b, hipMemcpyAsync(q[m], b->qtmp[m], bytes, hipMemcpyDeviceToDevice,
+                                       b->stream));
+        }
     return RV_OK;
 }
 

@@ -1,0 +1,174 @@
+// Batched fused LSRK stages: many independent operators (different Nq, law,
+// precision and cell count) advanced by ONE persistent launch per stage.
+//
+// The work list holds one item per wave's worth of cells of one member
+// (member, kind, first cell); items are grouped by kind so that the waves
+// running at the same time mostly execute the same code.  Each persistent wave
+// walks the list with a stride of the launched wave count and hands every item
+// to the same per-wave routine the single-operator kernels use
+// (stage_wave / plane3_wave), with that member's StageParams read from a
+// device table.  Members are restricted to affine geometry and the Rusanov
+// flux (the brick solves batched and served here); 2-D members take Nq 2..8,
+// 3-D members Nq 4..8.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "dg_plane3.hpp"
+
+namespace rv {
+
+namespace {
+
+constexpr int KINDS_PER_NQ = 4;  // (advection, Euler) x (double, float)
+
+template <int DIM>
+struct BatchRange {
+    static constexpr int NQ_MIN = DIM == 3 ? 4 : 2;
+    static constexpr int NQ_MAX = 8;
+};
+
+template <int DIM, int NQ, int LAW, typename R>
+__device__ __attribute__((always_inline)) inline void run_item(const StageParams &P, int64_t cell0, unsigned char *wl)
+{
+    if constexpr (DIM == 3)
+        plane3_wave<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>(P, cell0, reinterpret_cast<R *>(wl));
+    else
+        stage_wave<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>(P, cell0,
+                                                              reinterpret_cast<R *>(wl));
+}
+
+template <int DIM, int NQ>
+__device__ __attribute__((always_inline)) inline void run_nq(int sub, const StageParams &P, int64_t cell0, unsigned char *wl)
+{
+    switch (sub) {
+    case 0: run_item<DIM, NQ, LAW_ADVECTION, double>(P, cell0, wl); break;
+    case 1: run_item<DIM, NQ, LAW_ADVECTION, float>(P, cell0, wl); break;
+    case 2: run_item<DIM, NQ, LAW_EULER, double>(P, cell0, wl); break;
+    default: run_item<DIM, NQ, LAW_EULER, float>(P, cell0, wl); break;
+    }
+}
+
+template <int DIM, int NQ>
+__device__ __attribute__((always_inline)) inline void run_kind(int kind, const StageParams &P, int64_t cell0, unsigned char *wl)
+{
+    if constexpr (NQ > BatchRange<DIM>::NQ_MAX) {
+        (void)kind, (void)P, (void)cell0, (void)wl;
+    } else {
+        if (kind / KINDS_PER_NQ == NQ - BatchRange<DIM>::NQ_MIN)
+            run_nq<DIM, NQ>(kind % KINDS_PER_NQ, P, cell0, wl);
+        else
+            run_kind<DIM, NQ + 1>(kind, P, cell0, wl);
+    }
+}
+
+// Every wave exits after its last item: the loop bound is the list length.
+template <int DIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+dg_batch_kernel(const StageParams *__restrict__ members, const BatchItem *__restrict__ items,
+                int64_t nitems, int64_t nwaves, int slab_bytes)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
+    const int wave = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    unsigned char *wl = dg_lds + (size_t)wave * slab_bytes;
+    const int64_t first = wave_uniform(xcd_remap(blockIdx.x, gridDim.x) * wpb + wave);
+    for (int64_t it = first; it < nitems; it += nwaves) {
+        const BatchItem w = items[it];
+        run_kind<DIM, BatchRange<DIM>::NQ_MIN>(w.kind, members[w.member], w.cell0, wl);
+        // the next item reuses the slab
+        wave_lds_sync();
+    }
+}
+
+template <int DIM, int NQ, int LAW, typename R>
+void layout_of(int *epw, int *slab_bytes)
+{
+    if constexpr (DIM == 3) {
+        using T = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
+        *epw = T::EPW;
+        *slab_bytes = (int)(T::EPW * T::SLAB * sizeof(R));
+    } else {
+        using S = Stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
+        *epw = S::EPW;
+        *slab_bytes = (int)(S::EPW * S::NS * S::NP * sizeof(R));
+    }
+}
+
+template <int DIM, int NQ>
+bool layout_nq(int nq, int sub, int *epw, int *slab_bytes)
+{
+    if constexpr (NQ > BatchRange<DIM>::NQ_MAX) {
+        return false;
+    } else {
+        if (nq != NQ)
+            return layout_nq<DIM, NQ + 1>(nq, sub, epw, slab_bytes);
+        switch (sub) {
+        case 0: layout_of<DIM, NQ, LAW_ADVECTION, double>(epw, slab_bytes); break;
+        case 1: layout_of<DIM, NQ, LAW_ADVECTION, float>(epw, slab_bytes); break;
+        case 2: layout_of<DIM, NQ, LAW_EULER, double>(epw, slab_bytes); break;
+        default: layout_of<DIM, NQ, LAW_EULER, float>(epw, slab_bytes); break;
+        }
+        return true;
+    }
+}
+
+template <int DIM>
+const void *kernel_of()
+{
+    return reinterpret_cast<const void *>(&dg_batch_kernel<DIM>);
+}
+
+}  // namespace
+
+bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int *cells_per_wave,
+                int *slab_bytes)
+{
+    const int sub = (law == LAW_EULER ? 2 : 0) + (real_bytes == 4 ? 1 : 0);
+    if (law != LAW_EULER && law != LAW_ADVECTION)
+        return false;
+    if (real_bytes != 4 && real_bytes != 8)
+        return false;
+    bool ok = false;
+    if (dim == 2) {
+        ok = layout_nq<2, BatchRange<2>::NQ_MIN>(nq, sub, cells_per_wave, slab_bytes);
+        *kind = (nq - BatchRange<2>::NQ_MIN) * KINDS_PER_NQ + sub;
+    } else if (dim == 3) {
+        ok = layout_nq<3, BatchRange<3>::NQ_MIN>(nq, sub, cells_per_wave, slab_bytes);
+        *kind = (nq - BatchRange<3>::NQ_MIN) * KINDS_PER_NQ + sub;
+    }
+    return ok;
+}
+
+int batch_resident_waves(int dim, int wpb, int slab_bytes, int64_t *waves)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    const void *k = dim == 3 ? kernel_of<3>() : kernel_of<2>();
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * wpb,
+                                                     (size_t)wpb * slab_bytes) != hipSuccess)
+        return -1;
+    *waves = (int64_t)cus * (per_cu > 0 ? per_cu : 1) * wpb;
+    return 0;
+}
+
+int launch_batch(int dim, const StageParams *members, const BatchItem *items, int64_t nitems,
+                 int64_t nwaves, int slab_bytes, int wpb, void *stream)
+{
+    if (nitems <= 0)
+        return 0;
+    const int64_t blocks = (nwaves + wpb - 1) / wpb;
+    const int64_t launched = blocks * wpb;
+    const size_t lds = (size_t)wpb * slab_bytes;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dim == 3)
+        hipLaunchKernelGGL(dg_batch_kernel<3>, dim3((unsigned)blocks), dim3(64 * wpb), lds, st,
+                           members, items, nitems, launched, slab_bytes);
+    else
+        hipLaunchKernelGGL(dg_batch_kernel<2>, dim3((unsigned)blocks), dim3(64 * wpb), lds, st,
+                           members, items, nitems, launched, slab_bytes);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace rv

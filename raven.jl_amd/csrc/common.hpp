@@ -60,6 +60,23 @@ struct KernelKey {
     int dim, nq, law, real_bytes, geo, flux;
 };
 
+// Batched stages (batch.hip): one work-list item per wave's worth of cells of
+// one member; `kind` selects the member's (Nq, law, precision) routine.
+struct BatchItem {
+    int32_t member;
+    int32_t kind;
+    int64_t cell0;
+};
+// kind id, cells per wave and LDS bytes per wave of a batch member; false when
+// the batch kernels do not cover (dim, nq, law, precision)
+bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int *cells_per_wave,
+                int *slab_bytes);
+// waves of the batch kernel resident on the whole device at once
+int batch_resident_waves(int dim, int wpb, int slab_bytes, int64_t *waves);
+// one persistent launch over the work list (members: device table)
+int launch_batch(int dim, const StageParams *members, const BatchItem *items, int64_t nitems,
+                 int64_t nwaves, int slab_bytes, int wpb, void *stream);
+
 // Host-side grid generation (host_grid.cpp): Raven's brick GridManager +
 // generate restated for uniform bricks, and the LobattoCell 1-D operators.
 struct BrickSpec {

@@ -47,6 +47,33 @@ __device__ inline int64_t xcd_remap(int64_t b, int64_t nb)
     return x * q + (x < r ? x : r) + idx;
 }
 
+// Each wave works in its own LDS slab and a wavefront's LDS operations
+// complete in issue order, so ordering the compiler's memory operations is all
+// the synchronisation a slab needs (no s_barrier tying a block's waves).
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A value every lane of the wave holds, moved to scalar registers.
+__device__ inline int64_t wave_uniform(int64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Lane l's value of v, broadcast to the whole wave (two v_readlane_b32).
+__device__ inline double readlane_f64(double v, int l)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Volume index of node m of face f (faces ordered -x,+x,-y,+y,-z,+z; face
 // nodes over the remaining directions, first fastest: lobattocells.jl:2020-2050).
 template <int DIM, int NQ>
@@ -455,6 +482,33 @@ struct Stage {
     }
 };
 
+// The cells [cell0, cell0 + EPW) of one wave (cell0 is the same on every
+// lane): the direction sweeps meet in the wave's own LDS slabs `wlds`
+// (EPW * Ns * Np values).
+template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>
+__device__ __attribute__((always_inline)) inline void stage_wave(const StageParams &P, int64_t cell0, R *wlds)
+{
+    using S = Stage<DIM, NQ, LAW, R, GEO, FLUX>;
+    constexpr int SLAB = S::NS * S::NP;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / S::NP2;
+    const int pen = lane - slot * S::NP2;
+    const int64_t e = cell0 + slot;
+    const bool active = (slot < S::EPW) && (e < P.K);
+    R *slab = wlds + (slot < S::EPW ? slot : 0) * SLAB;
+    const Law<DIM, LAW, R> law(P);
+
+    S::template sweep<0>(P, law, active, e, pen, slab);
+    if constexpr (DIM > 1) {
+        wave_lds_sync();
+        S::template sweep<1>(P, law, active, e, pen, slab);
+    }
+    if constexpr (DIM > 2) {
+        wave_lds_sync();
+        S::template sweep<2>(P, law, active, e, pen, slab);
+    }
+}
+
 // Waves per block (cells of one block share nothing; the knob trades LDS
 // granularity against scheduling) come from blockDim.x; the LDS slabs are
 // dynamic shared memory sized by the launcher.
@@ -462,30 +516,13 @@ template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>
 __global__ void __launch_bounds__(256) dg_stage_kernel(const StageParams P)
 {
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX>;
-    constexpr int SLAB = S::NS * S::NP;
     extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
-    R *lds = reinterpret_cast<R *>(dg_lds);
-
-    const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    const int slot = lane / S::NP2;
-    const int pen = lane - slot * S::NP2;
     const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t e = (blk * wpb + wave) * S::EPW + slot;
-    const bool active = (slot < S::EPW) && (e < P.K);
-    R *slab = lds + (wave * S::EPW + (slot < S::EPW ? slot : 0)) * SLAB;
-    const Law<DIM, LAW, R> law(P);
-
-    S::template sweep<0>(P, law, active, e, pen, slab);
-    if constexpr (DIM > 1) {
-        __syncthreads();
-        S::template sweep<1>(P, law, active, e, pen, slab);
-    }
-    if constexpr (DIM > 2) {
-        __syncthreads();
-        S::template sweep<2>(P, law, active, e, pen, slab);
-    }
+    const int64_t cell0 = (blk * wpb + wave) * S::EPW;
+    R *wlds = reinterpret_cast<R *>(dg_lds) + wave * S::EPW * S::NS * S::NP;
+    stage_wave<DIM, NQ, LAW, R, GEO, FLUX>(P, cell0, wlds);
 }
 
 template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>

@@ -15,7 +15,10 @@
 //    (4 Nq^2 face nodes) are spread over the lanes, one node of each face per
 //    lane, with q- and the lift exchanged through LDS;
 //  * the accumulators start from a*dq_old (affine cells) so dq_old and q load
-//    together at the top; dq and q_out are stored at the end.
+//    together at the top; dq and q_out are stored at the end;
+//  * every load of a cell is issued before its first wait; the face records
+//    come in with the state (scalar loads when a wave owns one cell) and the
+//    neighbour states of all six faces are requested together.
 #pragma once
 
 #include "dg_kernels.hpp"
@@ -64,24 +67,6 @@ struct Plane3 {
     using S = Stage<3, NQ, LAW, R, GEO, FLUX>;
 };
 
-// Each wave works in its own LDS slab and a wavefront's LDS operations
-// complete in issue order, so ordering the compiler's memory operations is all
-// the synchronisation the slab needs (no s_barrier tying the block's waves).
-__device__ inline void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// A value every lane of the wave holds, moved to scalar registers.
-__device__ inline int64_t wave_uniform(int64_t v)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
 // q+ at node `pen` of a face from its neighbour record (conforming neighbour or
 // explicit offset); self-mapped faces (FACE_SELF) are filled from q- later.
 template <int NQ, int NS, typename R>
@@ -104,29 +89,25 @@ __device__ inline void load_neighbour(const StageParams &P, const R *qbase, cons
     }
 }
 
+// The cells [cell0, cell0 + EPW) of one wave (cell0 is the same on every
+// lane), working in the wave's own LDS slabs `wlds` (EPW * SLAB values).
 template <int NQ, int LAW, typename R, int GEO, int FLUX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-dg_plane3_kernel(const StageParams P)
+__device__ __attribute__((always_inline)) inline void plane3_wave(const StageParams &P, int64_t cell0, R *wlds)
 {
     using T = Plane3<NQ, LAW, R, GEO, FLUX>;
     using S = typename T::S;
     constexpr int NS = T::NS, NQ2 = T::NQ2, NP = T::NP, PAD = T::PAD;
     constexpr int PLANE = T::PLANE, PBUF = T::PBUF;
-    extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
-    R *lds = reinterpret_cast<R *>(dg_lds);
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wpb = blockDim.x >> 6;
     const int slot = lane / NQ2;
     const int l = lane - slot * NQ2;  // column index = i + NQ*j
     const int i = l % NQ, j = l / NQ;
-    const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    int64_t e = (blk * wpb + wave) * T::EPW + slot;
+    int64_t e = cell0 + slot;
     if constexpr (T::EPW == 1)
         e = wave_uniform(e);  // one cell per wave: per-cell data comes in scalar loads
     const bool active = (slot < T::EPW) && (e < P.K);
-    R *slab = lds + (wave * T::EPW + (slot < T::EPW ? slot : 0)) * T::SLAB;
+    R *slab = wlds + (slot < T::EPW ? slot : 0) * T::SLAB;
     const Law<3, LAW, R> law(P);
     const R *qbase = static_cast<const R *>(P.q_in);
     const bool dx_on = P.dir_mask & 1, dy_on = (P.dir_mask >> 1) & 1,
@@ -141,6 +122,9 @@ dg_plane3_kernel(const StageParams P)
     // complete before the first plane's flux
     const bool late_scale = GEO == GEO_AFFINE && P.read_dq && dz_on;
 
+    // the z-derivative weights M[kk*Nq + k], one per lane, broadcast with
+    // v_readlane inside the plane loop (no scalar loads or waits there)
+    const double ztab = ((GEO == GEO_AFFINE) ? P.Dw : P.Dt)[lane % NQ2];
     if (active) {
         // every load of the cell is issued before the first wait: derivative
         // weights, geometry factors, face records, then q and dq
@@ -237,10 +221,10 @@ dg_plane3_kernel(const StageParams P)
             buf[PLANE + s * NQ * PAD + i * PAD + j] = f2[s];  // row i, entry j
         }
         if (dz_on) {
-            const double *M = (GEO == GEO_AFFINE) ? P.Dw : P.Dt;
 #pragma unroll
             for (int kk = 0; kk < NQ; ++kk) {
-                const R w = R(M[kk * NQ + k]);  // contribution of plane k to output kk
+                // contribution of plane k to output kk
+                const R w = R(readlane_f64(ztab, kk * NQ + k));
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     R base = acc[s][kk];
@@ -428,6 +412,19 @@ dg_plane3_kernel(const StageParams P)
             }
         }
     }
+}
+
+template <int NQ, int LAW, typename R, int GEO, int FLUX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+dg_plane3_kernel(const StageParams P)
+{
+    using T = Plane3<NQ, LAW, R, GEO, FLUX>;
+    extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
+    const int wave = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    R *wlds = reinterpret_cast<R *>(dg_lds) + wave * T::EPW * T::SLAB;
+    plane3_wave<NQ, LAW, R, GEO, FLUX>(P, (blk * wpb + wave) * T::EPW, wlds);
 }
 
 template <int NQ, int LAW, typename R, int GEO, int FLUX>

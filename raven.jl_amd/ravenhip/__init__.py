@@ -4,6 +4,7 @@ Host-side mirror of Raven.jl's API for this path over libravenhip.so (HIP
 kernels for gfx950 behind the C ABI in include/ravenhip.h).
 """
 from ._lib import RavenHIPError, lib  # noqa: F401
+from .batch import Batch  # noqa: F401
 from .dgsem import (DGSEM, AdvectionLaw, CentralFlux, EulerLaw, RusanovFlux,  # noqa: F401
                     weightednorm)
 from .grids import GridManager, LobattoCell, brick, generate  # noqa: F401

@@ -106,6 +106,8 @@ def lib():
         "rv_get_info": (ctypes.c_int, [H, ctypes.POINTER(RvInfo)]),
         "rv_synchronize": (ctypes.c_int, [H]),
         "rv_stream": (vp, [H]),
+        "rv_set_waves_per_block": (ctypes.c_int, [H, ctypes.c_int]),
+        "rv_get_waves_per_block": (ctypes.c_int, [H]),
         "rv_rhs": (ctypes.c_int, [H, vp, vp, ctypes.c_double, ctypes.c_int]),
         "rv_lsrk_stage": (ctypes.c_int, [H, vp, vp, vp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double]),
@@ -122,6 +124,15 @@ def lib():
         "rv_lobatto": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p]),
         "rv_lsrk_coefficients": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p,
                                                 ctypes.POINTER(ctypes.c_int)]),
+        "rv_batch_create": (ctypes.c_int, [ctypes.POINTER(H), ctypes.c_int32,
+                                           ctypes.POINTER(H)]),
+        "rv_batch_destroy": (None, [H]),
+        "rv_batch_last_error": (ctypes.c_char_p, [H]),
+        "rv_batch_info": (ctypes.c_int, [H, c_i64p, c_i64p, c_i32p]),
+        "rv_batch_stage": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.POINTER(vp), c_f64p, c_f64p]),
+        "rv_batch_lsrk_run": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                             ctypes.c_int, c_f64p, ctypes.c_int64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -129,6 +140,13 @@ def lib():
         fn.argtypes = args
     _LIB = L
     return L
+
+
+def check_batch(rc, b):
+    if rc != RV_OK:
+        msg = lib().rv_batch_last_error(b)
+        raise RavenHIPError(rc, msg.decode() if msg else "")
+    return rc
 
 
 def check(rc, h=None):

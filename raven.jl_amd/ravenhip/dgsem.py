@@ -173,6 +173,36 @@ class DGSEM:
                                       self._check_state(dq), float(a), float(b_dt), float(time)),
                 self._h)
 
+    # -- launch shape ------------------------------------------------------
+    @property
+    def waves_per_block(self):
+        return L.lib().rv_get_waves_per_block(self._h)
+
+    @waves_per_block.setter
+    def waves_per_block(self, w):
+        L.check(L.lib().rv_set_waves_per_block(self._h, int(w)), self._h)
+
+    def autotune(self, q, choices=(1, 2, 4), repeats=5):
+        """Time one fused LSRK stage on state q for each launch shape and keep
+        the fastest; returns {waves_per_block: mean ms}.  q is not modified."""
+        torch = _torch()
+        out, dq = torch.empty_like(q), torch.zeros_like(q)
+        stream = torch.cuda.ExternalStream(self.stream) if self.stream else \
+            torch.cuda.default_stream()
+        times = {}
+        for w in choices:
+            self.waves_per_block = w
+            self.lsrk_stage(out, q, dq, 0.0, 0.0)  # warm-up
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(stream)
+            for _ in range(repeats):
+                self.lsrk_stage(out, q, dq, 0.0, 0.0)
+            ev[1].record(stream)
+            ev[1].synchronize()
+            times[w] = ev[0].elapsed_time(ev[1]) / repeats
+        self.waves_per_block = min(times, key=times.get)
+        return times
+
     # -- diagnostics ------------------------------------------------------
     def weightednorm_components(self, q):
         out = np.zeros(self.ns)

@@ -7,6 +7,8 @@ stage by stage through rv_lsrk_stage.
 """
 import ctypes
 
+import numpy as np
+
 from . import _lib as L
 
 class LSRK:
@@ -20,7 +22,6 @@ class LSRK:
     def coefficients(self):
         """(rka, rkb, rkc) of the scheme as Python floats, read from the
         library (coefficients_lsrk54/144, odesolvers.jl:163-244)."""
-        import numpy as np
         a, b, c = (np.zeros(14) for _ in range(3))
         n = ctypes.c_int(0)
         L.check(L.lib().rv_lsrk_coefficients(self.scheme, L.ptr(a, ctypes.c_double), L.ptr(b, ctypes.c_double),
@@ -54,9 +55,17 @@ def dostep_(q, solver, qtmp):
     point, stage times time + rkc*dt; q updated in place (qtmp is the
     ping-pong buffer; an odd stage count ends with a copy)."""
     a, b, c = solver.coefficients()
+    if solver.dg.dtype == "f32":
+        # LSRK{Float32}: coefficients and dt in the state type, rkb*dt in FP32
+        # (odesolvers.jl:41-48, 57)
+        f = np.float32
+        a = [float(f(v)) for v in a]
+        bdt = [float(f(v) * f(solver.dt)) for v in b]
+    else:
+        bdt = [v * solver.dt for v in b]
     cur, alt = q, qtmp
     for s in range(len(a)):
-        solver.dg.lsrk_stage(alt, cur, solver.dq, a[s], b[s] * solver.dt,
+        solver.dg.lsrk_stage(alt, cur, solver.dq, a[s], bdt[s],
                              solver.time + c[s] * solver.dt)
         cur, alt = alt, cur
     if cur is not q:

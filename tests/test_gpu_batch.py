@@ -1,0 +1,157 @@
+"""Batched solves (rv_batch_*): one persistent launch per stage over many
+operators of different degree, size, law and precision.  Every member's
+result must be bit-identical to what its own operator computes alone, and the
+FP64 members must agree with the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import problems as PB
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no HIP device")
+    return t
+
+
+@pytest.fixture(scope="module")
+def R(torch):
+    import ravenhip
+    return ravenhip
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def member(R, torch, dim, nq, n, law, dtype, seed):
+    """A periodic brick solve on [0, n/4]^dim with a smooth state plus noise."""
+    cell = R.LobattoCell(*([nq] * dim))
+    coords = [PB.julia_range(0, n / 4, n + 1)] * dim
+    grid = R.generate(R.GridManager(cell, R.brick(coords, (True,) * dim)))
+    rng = np.random.default_rng(seed)
+    u = list(rng.uniform(0.5, 1.5, dim))
+    lawobj = R.AdvectionLaw(dim, u) if law == "advection" else R.EulerLaw(dim)
+    dg = R.DGSEM(law=lawobj, grid=grid, surface_numericalflux=R.RusanovFlux(), dtype=dtype)
+    og = O.BrickGrid(nq, coords, (True,) * dim)
+    P = og.point_fields()
+    if law == "advection":
+        q = PB.advection_wave(P, dim, 0.0, u) + 0.01 * rng.standard_normal(og.K * og.Np)
+    elif dim == 2:
+        q = PB.euler_wave(P, 2, 0.0) + 1e-3 * rng.standard_normal(og.K * og.Np * 4)
+    else:
+        c = [0.5 * coords[0][-1]] * 2
+        q = PB.isentropic_vortex_3d(P, 0.0, centre=c,
+                                    noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    qd = torch.from_numpy(np.ascontiguousarray(q)).to(dg.torch_dtype).cuda()
+    qd = qd.reshape(dg.K, dg.ns, dg.Np).contiguous()
+    odg = O.DG(og, law, u=u if law == "advection" else None)
+    return dg, qd, q, odg
+
+
+MIX2 = [(3, 6, "advection", "f64"), (4, 5, "euler", "f64"), (5, 7, "advection", "f32"),
+        (6, 4, "euler", "f32"), (7, 3, "euler", "f64"), (8, 2, "advection", "f64"),
+        (2, 9, "euler", "f32"), (5, 8, "euler", "f64")]
+MIX3 = [(4, 3, "euler", "f64"), (5, 2, "advection", "f32"), (6, 2, "euler", "f32"),
+        (8, 2, "euler", "f64"), (7, 3, "advection", "f64")]
+
+
+@pytest.mark.parametrize("dim,mix", [(2, MIX2), (3, MIX3)])
+def test_batch_stage_equals_members_alone(torch, R, dim, mix):
+    say(f"batch stage dim={dim} members={len(mix)}")
+    ms = [member(R, torch, dim, nq, n, law, dt, seed) for seed, (nq, n, law, dt) in enumerate(mix)]
+    dgs = [m[0] for m in ms]
+    batch = R.Batch(dgs)
+    say(f"   {batch.info}")
+    rng = np.random.default_rng(7)
+    q_in = [m[1] for m in ms]
+    dq0 = [torch.from_numpy(1e-3 * rng.standard_normal(q.numel())).to(q.dtype).cuda()
+           .reshape(q.shape) for q in q_in]
+    a = [-0.5 - 0.1 * k for k in range(len(ms))]
+    bdt = [1e-3 * (k + 1) for k in range(len(ms))]
+    # alone
+    want = []
+    for dg, q, d0, ak, bk in zip(dgs, q_in, dq0, a, bdt):
+        qo, dq = torch.empty_like(q), d0.clone()
+        dg.lsrk_stage(qo, q, dq, ak, bk)
+        want.append((qo, dq))
+    # together, twice (the parameter ring is reused)
+    for rep in range(2):
+        q_out = [torch.empty_like(q) for q in q_in]
+        dq = [d.clone() for d in dq0]
+        batch.stage(q_out, q_in, dq, a, bdt)
+        torch.cuda.synchronize()
+        for k, (qo, d) in enumerate(zip(q_out, dq)):
+            assert torch.equal(qo, want[k][0]), f"member {k} q_out differs"
+            assert torch.equal(d, want[k][1]), f"member {k} dq differs"
+    batch.close()
+
+
+def test_batch_rhs_stage_matches_oracle(torch, R):
+    """A stage with a = 0, b_dt = 0 leaves q and returns L(q): FP64 members
+    against the oracle at the FP64 bar."""
+    mix = [(4, 5, "euler", "f64"), (7, 3, "advection", "f64"), (5, 6, "euler", "f64")]
+    ms = [member(R, torch, 2, nq, n, law, dt, seed) for seed, (nq, n, law, dt) in enumerate(mix)]
+    batch = R.Batch([m[0] for m in ms])
+    q_in = [m[1] for m in ms]
+    q_out = [torch.empty_like(q) for q in q_in]
+    dq = [torch.zeros_like(q) for q in q_in]
+    batch.stage(q_out, q_in, dq, [0.0] * 3, [0.0] * 3)
+    torch.cuda.synchronize()
+    for k, (dg, qd, q, odg) in enumerate(ms):
+        want = np.zeros(q.size)
+        odg.rhs(want, q.copy(), 0.0, increment=False)
+        got = dq[k].cpu().numpy().reshape(-1)
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        say(f"   member {k}: rel L2 {err:.2e}")
+        assert err <= 1e-12
+
+
+@pytest.mark.parametrize("dim,mix", [(2, MIX2[:5]), (3, MIX3[:3])])
+def test_batch_graph_run_equals_stage_loop(torch, R, dim, mix):
+    """rv_batch_lsrk_run (a captured hipGraph per step, replayed) against each
+    member stepping alone through its own stages; odd step counts exercise the
+    final copy back from the ping-pong buffer."""
+    ms = [member(R, torch, dim, nq, n, law, dt, 10 + s) for s, (nq, n, law, dt) in enumerate(mix)]
+    dgs = [m[0] for m in ms]
+    dts = [2e-3 * (1 + 0.25 * k) for k in range(len(ms))]
+    for nsteps in (3, 2):
+        want = []
+        for dg, q, dt in zip(dgs, [m[1] for m in ms], dts):
+            qa = q.clone()
+            solver = R.LSRK54(dg, qa, dt)
+            tmp = torch.empty_like(qa)
+            for _ in range(nsteps):
+                R.dostep_(qa, solver, tmp)
+            want.append(qa)
+        batch = R.Batch(dgs)
+        qs = [m[1].clone() for m in ms]
+        dqs = [torch.empty_like(q) for q in qs]
+        batch.lsrk_run(qs, dqs, dts, nsteps)
+        # a replay of the cached graphs from the same buffers
+        qs2 = qs
+        for k in range(len(ms)):
+            qs2[k].copy_(ms[k][1])
+        batch.lsrk_run(qs2, dqs, dts, nsteps)
+        torch.cuda.synchronize()
+        for k in range(len(ms)):
+            assert torch.equal(qs2[k], want[k]), f"member {k}, {nsteps} steps"
+        batch.close()
+
+
+def test_batch_rejects_unsupported_members(torch, R):
+    cell = R.LobattoCell(5, 5)
+    grid = R.generate(R.GridManager(cell, R.brick([PB.julia_range(0, 1, 4)] * 2, (True, True))))
+    dg = R.DGSEM(law=R.EulerLaw(2), grid=grid, surface_numericalflux=R.CentralFlux())
+    with pytest.raises(R.RavenHIPError):
+        R.Batch([dg])
+    cell3 = R.LobattoCell(3, 3, 3)
+    g3 = R.generate(R.GridManager(cell3, R.brick([PB.julia_range(0, 1, 3)] * 3, (True,) * 3)))
+    dg3 = R.DGSEM(law=R.EulerLaw(3), grid=g3, surface_numericalflux=R.RusanovFlux())
+    with pytest.raises(R.RavenHIPError):
+        R.Batch([dg3])  # 3-D batches start at Nq = 4

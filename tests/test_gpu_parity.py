@@ -368,3 +368,29 @@ def test_small_advection_100_steps_per_stage_rhs(torch, R):
     dg.synchronize()
     assert n == 100
     assert rel_l2(qd.cpu().numpy().reshape(-1), qo) <= RTOL_F64
+
+
+@pytest.mark.parametrize("nq,dtype", [(8, "f64"), (5, "f64"), (8, "f32")])
+def test_launch_shape_does_not_change_results(torch, R, nq, dtype):
+    """Every waves-per-block choice gives bit-identical stages, and the
+    autotuner keeps one of them."""
+    coords = [PB.julia_range(0, 10, 5)] * 3
+    grid, dg, og, odg = make_pair(R, 3, nq, coords, (True,) * 3, "euler", dtype=dtype)
+    q = to_dev(torch, dg, initial_state(og, "euler", 3))
+    rng = np.random.default_rng(3)
+    dq0 = to_dev(torch, dg, 1e-3 * rng.standard_normal(q.numel()))
+    outs = []
+    for w in (1, 2, 4):
+        dg.waves_per_block = w
+        assert dg.waves_per_block == w
+        qo, dq = torch.empty_like(q), dq0.clone()
+        dg.lsrk_stage(qo, q, dq, -0.4, 1e-3)
+        dg.synchronize()
+        outs.append((qo.cpu(), dq.cpu()))
+    for qo, dq in outs[1:]:
+        assert torch.equal(qo, outs[0][0]) and torch.equal(dq, outs[0][1])
+    times = dg.autotune(q)
+    say(f"   autotune Nq={nq} {dtype}: {times}")
+    assert dg.waves_per_block in (1, 2, 4) and set(times) == {1, 2, 4}
+    with pytest.raises(R.RavenHIPError):
+        dg.waves_per_block = 3
