@@ -195,6 +195,93 @@ def cpu_baseline(q0, dt, budget_s):
             "note": "C restatement of Raven's CPU path (oracle/), not Raven itself"}
 
 
+def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
+    """256 independent 2-D advection / Euler solves (degree 3-7, n x n periodic
+    cells with n uniform in 8..64, FP64 or FP32, seeded velocities and vortex
+    centres) advanced together: one persistent launch per stage, each LSRK54
+    step a replayed hipGraph.  Compared with the same solves run one after the
+    other through the plain per-operator path (rv_lsrk_run)."""
+    import ravenhip.serve as S
+    rng = np.random.default_rng(seed)
+    reqs = [S.SolveRequest(rid=k, arrival=0.0, dim=2,
+                           law=str(rng.choice(["advection", "euler"])),
+                           nq=int(rng.integers(3, 8)) + 1, n=int(rng.integers(8, 65)),
+                           T=1.0, dtype=str(rng.choice(["f64", "f32"])),
+                           seed=int(rng.integers(1 << 30))) for k in range(n_inst)]
+    ops = [S.make_operator(r) for r in reqs]
+    dgs = [op[1] for op in ops]
+    q0 = [S.initial_state(r, gm, dg) for r, (gm, dg) in zip(reqs, ops)]
+    dts = [r.dt() for r in reqs]
+    dofs = sum(r.dofs for r in reqs)
+    batch = R.Batch(dgs)
+    qs = [q.clone() for q in q0]
+    dqs = [torch.empty_like(q) for q in q0]
+    batch.lsrk_run(qs, dqs, dts, steps)  # builds and caches the step graphs
+    for q, q_init in zip(qs, q0):
+        q.copy_(q_init)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    batch.lsrk_run(qs, dqs, dts, steps)
+    ev[1].record()
+    torch.cuda.synchronize()
+    t_batch = ev[0].elapsed_time(ev[1]) / 1e3
+    finite = all(bool(torch.isfinite(q).all()) for q in qs)
+    # one after the other: solve! per instance, (steps - 1/4)*dt so that each
+    # takes exactly `steps` steps (the last one shortened)
+    solvers = [R.odesolvers.LSRK(dg, q, dt, 0.0, 54) for dg, q, dt in zip(dgs, q0, dts)]
+    qs2 = [q.clone() for q in q0]
+    R.solve_(qs2[0], (steps - 0.25) * dts[0], solvers[0])  # warm-up
+    qs2[0].copy_(q0[0])
+    solvers[0].time = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nsteps = 0
+    for q, dt, sv in zip(qs2, dts, solvers):
+        nsteps += R.solve_(q, (steps - 0.25) * dt, sv)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    info = batch.info
+    batch.close()
+    return {
+        "workload": f"{n_inst} independent 2-D periodic brick solves (advection and Euler, "
+                    "degree 3-7, 8^2..64^2 cells, FP64 and FP32), LSRK54",
+        "instances": n_inst, "steps": steps, "dof": int(dofs),
+        "batched_dof_updates_per_s": dofs * 5 * steps / t_batch,
+        "batched_ms_per_step": t_batch / steps * 1e3,
+        "sequential_dof_updates_per_s": dofs * 5 * steps / t_seq,
+        "sequential_ms_per_step": t_seq / steps * 1e3,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "sequential_steps_match": nsteps == steps * n_inst,
+        "finite": finite,
+        "work_items": info["items"], "persistent_waves": info["waves"],
+        "launch": "one persistent work-list kernel per stage, one hipGraph per step",
+    }
+
+
+def run_served(R, torch, rate, n_req=32, seed=7, budget_gb=64):
+    """A Poisson request stream (2-D / 3-D advection / Euler, degree 3-7,
+    1e3..1e5 cells log-uniform, T in [0.1, 1], FP32 / FP64) served with
+    continuous batching, and the same stream served one request at a time."""
+    import ravenhip.serve as S
+    served = S.random_requests(n_req, rate, seed=seed)
+    server = S.Server(hbm_budget=budget_gb << 30)
+    t0 = time.perf_counter()
+    server.run(served)
+    wall = time.perf_counter() - t0
+    plain = S.random_requests(n_req, rate, seed=seed)
+    t0 = time.perf_counter()
+    S.run_plain(plain)
+    wall_plain = time.perf_counter() - t0
+    a, b = S.summarize(served, wall), S.summarize(plain, wall_plain)
+    return {"rate_per_s": rate, "requests": n_req, "hbm_budget_gb": budget_gb,
+            "served": a, "one_at_a_time": b,
+            "throughput_ratio": a["throughput_dof_updates_per_s"] /
+            b["throughput_dof_updates_per_s"],
+            "peak_resident": server.peak_active,
+            "steps_match": all(x.steps == y.steps for x, y in zip(served, plain))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +292,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain state copy (calibrates profiler traffic counters)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 256-solve batched workload")
+    ap.add_argument("--no-served", action="store_true",
+                    help="skip the served request stream")
+    ap.add_argument("--serve-rates", default="8,32",
+                    help="request arrival rates (per second) of the served stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is "
@@ -300,6 +393,16 @@ def main():
         del r32
     del r64
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        log("bench: 256 batched 2-D solves")
+        out["batched"] = run_batched(R, torch)
+        torch.cuda.empty_cache()
+    if not args.no_served:
+        out["served"] = []
+        for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+            log(f"bench: served stream at {rate:g} requests/s")
+            out["served"].append(run_served(R, torch, rate))
+            torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(q0, dt, args.cpu_budget)

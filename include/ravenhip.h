@@ -230,6 +230,25 @@ int rv_brick_arrays(const rv_brick_desc *desc, double *points, double *metrics,
                     double *surface_metrics, int64_t *vmapM, int64_t *vmapP, int64_t *bc,
                     int64_t *dtoc, int64_t *ncontinuous);
 
+/* Volume and surface metrics of curved cells from their node coordinates, on
+ * the device: hexvolumemetrics! / quadvolumemetrics! (lobattocells.jl:
+ * 3038-3180, 2756-2813; Kopriva's curl-invariant form in 3-D) and
+ * hexsurfacemetrics! / quadsurfacemetrics! (:3253-3295, :2862-2900), in the
+ * layouts of rv_desc.  points (Np, dim, K), metrics and surface_metrics are
+ * DEVICE pointers; D (Nq x Nq column-major) and weights are host arrays.
+ * Runs on `stream` and returns when the results are complete.  This is the
+ * setup step generate(warp, gm) performs for warped grids
+ * (gridmanager.jl:1111-1166); dim 2 or 3, Nq <= 8. */
+int rv_curved_metrics(int dim, int nq, int64_t ncells, const double *points, const double *D,
+                      const double *weights, double *metrics, double *surface_metrics,
+                      void *stream);
+
+/* The integer lattice position of every cell of the brick at the finest level,
+ * ijk[e*dim + d] (cell e spans [lo + h*ijk, lo + h*(ijk+1)) along d): the
+ * quadrant coordinates generate() reads from the forest
+ * (gridmanager.jl:476-639), without per-node arrays. */
+int rv_brick_cells(const rv_brick_desc *desc, int32_t *ijk);
+
 /* LobattoCell 1-D operators (lobattocells.jl:1-47): nodes, weights and D
  * (column-major) for Nq points, rounded from 256-bit closed forms. */
 int rv_lobatto(int nq, double *points, double *weights, double *D);

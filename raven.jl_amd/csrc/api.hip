@@ -301,6 +301,159 @@ __global__ void maps_kernel(const FaceRec *faces, const int64_t *explicit_idx, i
     mapP[idx] = mp;
 }
 
+// Volume metrics of curved cells from node coordinates, one block of Np
+// threads per cell (thread = node).  3-D: hexvolumemetrics!
+// (lobattocells.jl:3038-3180), the curl-invariant form of Kopriva (2006)
+// eq. (37): a_r are reference derivatives of X x X_s, X x X_t, X x X_r, and
+// dRdX[r, :] = a_r / (2 det J).  2-D: quadvolumemetrics! (:2756-2813),
+// dRdX = inv(G) [X_r X_s]' with G the metric tensor.  The sums run in the
+// reference's loop order and no multiply-add is fused, so every entry is the
+// correctly rounded result of the same operation sequence as the host
+// restatement.  Output: (Np, dim*dim + 2, K) = (dRdX column-major, J, wJ).
+__global__ void __launch_bounds__(512)
+curved_volume_metrics_kernel(int dim, int nq, const double *points, const double *D,
+                             const double *w, double *metrics)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double mlds[];
+    const int64_t e = blockIdx.x;
+    const int np = dim == 3 ? nq * nq * nq : nq * nq;
+    const int n2 = nq * nq;
+    const int L = dim * dim + 2;
+    const int node = threadIdx.x;
+    const bool on = node < np;
+    double *X = mlds;             // [3][np]
+    double *cs = mlds + 3 * np;   // X x X_s
+    double *ct = mlds + 6 * np;   // X x X_t
+    double *cr = mlds + 9 * np;   // X x X_r
+    const double *pe = points + (int64_t)dim * np * e;
+    if (on)
+        for (int d = 0; d < dim; ++d)
+            X[d * np + node] = pe[d * np + node];
+    __syncthreads();
+    const int i = node % nq, j = (node / nq) % nq, k = dim == 3 ? node / n2 : 0;
+    double *M = metrics + (int64_t)L * np * e;
+    if (dim == 2) {
+        if (!on)
+            return;
+        double xr[2] = {-0.0, -0.0}, xs[2] = {-0.0, -0.0};
+        for (int m = 0; m < nq; ++m)
+            for (int d = 0; d < 2; ++d)
+                xr[d] += D[i + nq * m] * X[d * np + m + nq * j];
+        for (int m = 0; m < nq; ++m)
+            for (int d = 0; d < 2; ++d)
+                xs[d] += D[j + nq * m] * X[d * np + i + nq * m];
+        const double g11 = xr[0] * xr[0] + xr[1] * xr[1];
+        const double g12 = xr[0] * xs[0] + xr[1] * xs[1];
+        const double g22 = xs[0] * xs[0] + xs[1] * xs[1];
+        const double det = g11 * g22 - g12 * g12;
+        const double i11 = g22 / det, i12 = -g12 / det, i22 = g11 / det;
+        for (int b = 0; b < 2; ++b) {
+            M[(0 + 2 * b) * np + node] = i11 * xr[b] + i12 * xs[b];
+            M[(1 + 2 * b) * np + node] = i12 * xr[b] + i22 * xs[b];
+        }
+        const double J = fabs(xr[0] * xs[1] - xr[1] * xs[0]);
+        M[4 * np + node] = J;
+        M[5 * np + node] = w[i] * w[j] * J;
+        return;
+    }
+    double xr[3] = {-0.0, -0.0, -0.0}, xs[3] = {-0.0, -0.0, -0.0}, xt[3] = {-0.0, -0.0, -0.0};
+    if (on) {
+        for (int m = 0; m < nq; ++m)
+            for (int d = 0; d < 3; ++d)
+                xr[d] += D[i + nq * m] * X[d * np + m + nq * j + n2 * k];
+        for (int m = 0; m < nq; ++m)
+            for (int d = 0; d < 3; ++d)
+                xs[d] += D[j + nq * m] * X[d * np + i + nq * m + n2 * k];
+        for (int m = 0; m < nq; ++m)
+            for (int d = 0; d < 3; ++d)
+                xt[d] += D[k + nq * m] * X[d * np + i + nq * j + n2 * m];
+        const double x[3] = {X[node], X[np + node], X[2 * np + node]};
+        const double *v[3] = {xs, xt, xr};
+        double *c[3] = {cs, ct, cr};
+        for (int q = 0; q < 3; ++q) {
+            c[q][0 * np + node] = x[1] * v[q][2] - x[2] * v[q][1];
+            c[q][1 * np + node] = x[2] * v[q][0] - x[0] * v[q][2];
+            c[q][2 * np + node] = x[0] * v[q][1] - x[1] * v[q][0];
+        }
+    }
+    __syncthreads();
+    if (!on)
+        return;
+    double a1[3] = {-0.0, -0.0, -0.0}, a2[3] = {-0.0, -0.0, -0.0}, a3[3] = {-0.0, -0.0, -0.0};
+    for (int m = 0; m < nq; ++m)
+        for (int d = 0; d < 3; ++d)
+            a3[d] += D[i + nq * m] * cs[d * np + m + nq * j + n2 * k];
+    for (int m = 0; m < nq; ++m)
+        for (int d = 0; d < 3; ++d)
+            a2[d] -= D[i + nq * m] * ct[d * np + m + nq * j + n2 * k];
+    for (int m = 0; m < nq; ++m)
+        for (int d = 0; d < 3; ++d)
+            a1[d] += D[j + nq * m] * ct[d * np + i + nq * m + n2 * k];
+    for (int m = 0; m < nq; ++m)
+        for (int d = 0; d < 3; ++d)
+            a3[d] -= D[j + nq * m] * cr[d * np + i + nq * m + n2 * k];
+    for (int o = 0; o < nq; ++o)
+        for (int d = 0; d < 3; ++d)
+            a2[d] += D[k + nq * o] * cr[d * np + i + nq * j + n2 * o];
+    for (int o = 0; o < nq; ++o)
+        for (int d = 0; d < 3; ++d)
+            a1[d] -= D[k + nq * o] * cs[d * np + i + nq * j + n2 * o];
+    const double detJ = xr[0] * (xs[1] * xt[2] - xt[1] * xs[2]) -
+                        xs[0] * (xr[1] * xt[2] - xt[1] * xr[2]) +
+                        xt[0] * (xr[1] * xs[2] - xs[1] * xr[2]);
+    const double twodet = 2 * detJ;
+    const double *arow[3] = {a1, a2, a3};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            M[(a + 3 * b) * np + node] = arow[a][b] / twodet;
+    M[9 * np + node] = detJ;
+    M[10 * np + node] = w[i] * w[j] * w[k] * detJ;
+}
+
+// Surface metrics from the volume metrics at the face nodes
+// (hexsurfacemetrics! lobattocells.jl:3253-3295, quadsurfacemetrics!
+// :2862-2900): n = sn J dRdX[f/2, :], sJ = |n|, n /= sJ, wsJ = sJ * (face
+// weights).  Output (dim+2, 2dim*Nfp, K): n, sJ, wsJ per face-node slot.
+__global__ void curved_surface_metrics_kernel(int dim, int nq, int64_t K, const double *metrics,
+                                              const double *w, double *surf)
+{
+#pragma clang fp contract(off)
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = dim == 3 ? nq * nq * nq : nq * nq;
+    const int64_t nfp = dim == 3 ? nq * nq : nq;
+    const int64_t nfpt = 2 * dim * nfp;
+    if (idx >= nfpt * K)
+        return;
+    const int L = dim * dim + 2;
+    const int64_t e = idx / nfpt, slot = idx - e * nfpt;
+    const int f = (int)(slot / nfp);
+    const int64_t m = slot - f * nfp;
+    const int fg = f / 2;
+    const double sn = (f % 2 == 0) ? -1.0 : 1.0;
+    const int64_t nk = face_node_rt(dim, nq, f, m);
+    const double *mb = metrics + nk + np * L * e;
+    const double J = mb[np * (dim * dim)];
+    const double snJ = sn * J;
+    double nv[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < dim; ++c)
+        nv[c] = snJ * mb[np * (fg + dim * c)];
+    double ss = nv[0] * nv[0];
+    for (int c = 1; c < dim; ++c)
+        ss += nv[c] * nv[c];
+    const double sJ = sqrt(ss);
+    double *out = surf + (dim + 2) * idx;
+    for (int c = 0; c < dim; ++c)
+        out[c] = nv[c] / sJ;
+    out[dim] = sJ;
+    double wsJ = sJ;
+    if (dim == 2)
+        wsJ = sJ * w[m];
+    else
+        wsJ = sJ * w[m % nq] * w[m / nq];
+    out[dim + 1] = wsJ;
+}
+
 // ---------------------------------------------------------------------------
 
 int resolve_stream(rv_handle *h, void *stream)
@@ -1094,6 +1247,64 @@ int rv_brick_arrays(const rv_brick_desc *bd, double *points, double *metrics,
         if (ncontinuous)
             *ncontinuous = nc;
     }
+    return RV_OK;
+}
+
+int rv_curved_metrics(int dim, int nq, int64_t ncells, const double *points, const double *D,
+                      const double *weights, double *metrics, double *surface_metrics,
+                      void *stream)
+{
+    if ((dim != 2 && dim != 3) || nq < 2 || nq > 8 || ncells < 0 || !points || !D ||
+        !weights || !metrics || !surface_metrics)
+        return fail(nullptr, RV_E_ARG, "rv_curved_metrics: invalid argument");
+    if (ncells == 0)
+        return RV_OK;
+    const int np = dim == 3 ? nq * nq * nq : nq * nq;
+    const int nfp = dim == 3 ? nq * nq : nq;
+    // D and the weights are small host arrays: stage them on the device
+    double *dDw = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMallocAsync((void **)&dDw, sizeof(double) * (nq * nq + nq), st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dDw, D, sizeof(double) * nq * nq, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dDw + nq * nq, weights, sizeof(double) * nq, hipMemcpyHostToDevice,
+                           st);
+    if (e == hipSuccess) {
+        const int threads = (np + 63) / 64 * 64;
+        hipLaunchKernelGGL(curved_volume_metrics_kernel, dim3((unsigned)ncells), dim3(threads),
+                           sizeof(double) * 12 * np, st, dim, nq, points, dDw, dDw + nq * nq,
+                           metrics);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        const int64_t n = (int64_t)2 * dim * nfp * ncells;
+        hipLaunchKernelGGL(curved_surface_metrics_kernel, dim3((unsigned)((n + 255) / 256)),
+                           dim3(256), 0, st, dim, nq, (int64_t)ncells, metrics,
+                           dDw + nq * nq, surface_metrics);
+        e = hipGetLastError();
+    }
+    if (dDw)
+        (void)hipFreeAsync(dDw, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return fail(nullptr, RV_E_HIP, std::string("rv_curved_metrics: ") + hipGetErrorString(e));
+    return RV_OK;
+}
+
+int rv_brick_cells(const rv_brick_desc *bd, int32_t *ijk)
+{
+    BrickSpec b;
+    if (int rc = brick_spec(bd, &b))
+        return rc;
+    if (!ijk)
+        return fail(nullptr, RV_E_ARG, "null output");
+    const int64_t K = brick_numcells(b);
+    std::vector<int32_t> tree((size_t)K * b.dim), quad((size_t)K * b.dim);
+    brick_quadrants(b, tree.data(), quad.data());
+    for (int64_t k = 0; k < K * b.dim; ++k)
+        ijk[k] = (tree[k] << b.level) + quad[k];
     return RV_OK;
 }
 

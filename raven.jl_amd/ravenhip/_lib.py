@@ -122,6 +122,9 @@ def lib():
         "rv_brick_arrays": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), c_f64p, c_f64p, c_f64p,
                                            c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
         "rv_lobatto": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p]),
+        "rv_brick_cells": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), c_i32p]),
+        "rv_curved_metrics": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp,
+                                             c_f64p, c_f64p, vp, vp, vp]),
         "rv_lsrk_coefficients": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p,
                                                 ctypes.POINTER(ctypes.c_int)]),
         "rv_batch_create": (ctypes.c_int, [ctypes.POINTER(H), ctypes.c_int32,

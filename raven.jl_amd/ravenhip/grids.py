@@ -129,9 +129,12 @@ def _check_uniform(cg, desc):
 
 
 def generate(gm, warp=None):
-    """generate(gm) / generate(warp, gm) (gridmanager.jl:1084-1208)."""
-    if warp is not None:
-        raise NotImplementedError("warped grids are not supported yet")
+    """generate(gm) / generate(warp, gm) (gridmanager.jl:1084-1208).
+
+    warp(x, y[, z]) maps arrays of node coordinates to a tuple of arrays (the
+    vectorised form of Raven's point-wise warp); warped grids take the curved
+    metric path (gridmanager.jl:1160-1166), which runs on the device
+    (rv_curved_metrics)."""
     desc = gm.brick_desc()
     _check_uniform(gm.coarsegrid, desc)
     lib = L.lib()
@@ -154,4 +157,29 @@ def generate(gm, warp=None):
                                 L.ptr(surface, c.c_double), L.ptr(vmapM, c.c_int64),
                                 L.ptr(vmapP, c.c_int64), L.ptr(bc, c.c_int64),
                                 L.ptr(dtoc, c.c_int64), c.byref(nc)))
+    if warp is not None:
+        X = warp(*[points[:, d, :] for d in range(dim)])
+        points = np.ascontiguousarray(
+            np.stack([np.broadcast_to(np.asarray(x, dtype=np.float64), (K, Np)) for x in X],
+                     axis=1))
+        metrics, surface = curved_metrics(gm.cell, points)
     return Grid(gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc.value)
+
+
+def curved_metrics(cell, points):
+    """Volume and surface metrics of curved cells (rv_curved_metrics) for
+    host node coordinates (K, dim, Np); returns host arrays in Raven's
+    layout."""
+    import torch
+    K, dim, Np = points.shape
+    nfpt = 2 * dim * cell.nq ** (dim - 1)
+    pd = torch.from_numpy(np.ascontiguousarray(points)).cuda()
+    md = torch.empty((K, dim * dim + 2, Np), dtype=torch.float64, device="cuda")
+    sd = torch.empty((K, nfpt, dim + 2), dtype=torch.float64, device="cuda")
+    c = L.ctypes
+    D = np.ascontiguousarray(cell.D_colmajor)
+    L.check(L.lib().rv_curved_metrics(dim, cell.nq, K, c.c_void_p(pd.data_ptr()),
+                                      L.ptr(D, c.c_double), L.ptr(cell.weights_1d, c.c_double),
+                                      c.c_void_p(md.data_ptr()), c.c_void_p(sd.data_ptr()),
+                                      c.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return md.cpu().numpy(), sd.cpu().numpy()

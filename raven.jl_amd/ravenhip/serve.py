@@ -1,0 +1,327 @@
+"""Serving many solves on one GPU with continuous batching.
+
+A request is one brick solve (2-D or 3-D, advection or Euler, degree 3-7,
+FP32 or FP64) to a final time T.  The server keeps every admitted solve's
+state resident in device memory, admits waiting requests in arrival order as
+soon as their device footprint fits the budget, and advances all admitted
+solves of one dimension together: each LSRK54 step is five batched stage
+launches (rv_batch_stage) over the current mix, with every member's own dt and
+its final step shortened to land on its T exactly as ``solve!`` does
+(odesolvers.jl:9-30).  Requests that arrive while others run join at the next
+step; finished ones leave and free their memory.  Each member's stages are
+the ones it would get alone, so its result does not depend on the mix.
+
+``run_plain`` serves the same stream one request at a time through the plain
+per-operator path (rv_lsrk_run) for comparison.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .batch import Batch
+from .dgsem import DGSEM, AdvectionLaw, EulerLaw, RusanovFlux
+from .grids import GridManager, LobattoCell, brick
+
+GAMMA = 1.4
+CFL = 0.125
+
+
+@dataclass
+class SolveRequest:
+    """One solve: a periodic brick of n^dim unit cells, final time T."""
+    rid: int
+    arrival: float          # seconds after the stream starts
+    dim: int
+    law: str                # "advection" or "euler"
+    nq: int
+    n: int                  # cells per direction
+    T: float
+    dtype: str              # "f64" or "f32"
+    seed: int = 0
+    # filled in by the server
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps: int = 0
+    result: object = field(default=None, repr=False)
+
+    @property
+    def ns(self):
+        return 1 if self.law == "advection" else self.dim + 2
+
+    @property
+    def cells(self):
+        return self.n ** self.dim
+
+    @property
+    def dofs(self):
+        return self.cells * self.nq ** self.dim * self.ns
+
+    def velocity(self):
+        rng = np.random.default_rng(self.seed)
+        u = rng.uniform(-1.0, 1.0, self.dim)
+        return u / np.linalg.norm(u)
+
+    def speed(self):
+        # |u| for advection; |u0| + c0 of the vortex background for Euler
+        return 1.0 if self.law == "advection" else 0.5 + math.sqrt(GAMMA)
+
+    def dt(self):
+        """CFL rule of Raven's tests, cfl * h / N / speed (h = 1 here)."""
+        return CFL * 1.0 / (self.nq - 1) / self.speed()
+
+    def state_bytes(self):
+        return self.dofs * (4 if self.dtype == "f32" else 8)
+
+    def footprint(self):
+        """Device bytes while resident: q, dq and the ping-pong copy, plus the
+        operator's face records and per-cell geometry."""
+        return 3 * self.state_bytes() + self.cells * (6 * 8 + 3 * 8) + (1 << 16)
+
+
+def make_operator(req, stream=None):
+    cell = LobattoCell(*([req.nq] * req.dim))
+    gm = GridManager(cell, _brick(req))
+    law = AdvectionLaw(req.dim, list(req.velocity())) if req.law == "advection" \
+        else EulerLaw(req.dim, GAMMA)
+    return gm, DGSEM.brick(gm, law, RusanovFlux(), dtype=req.dtype, stream=stream)
+
+
+def _brick(req):
+    coords = [np.linspace(0.0, float(req.n), req.n + 1)] * req.dim
+    return brick(coords, (True,) * req.dim)
+
+
+def initial_state(req, gm, dg):
+    """A smooth initial state evaluated on the device from the cells' lattice
+    positions and the Gauss-Lobatto points: a Gaussian pulse (advection) or an
+    isentropic vortex (Euler, extruded in z for 3-D) at a seeded centre."""
+    import torch
+    desc = gm.brick_desc()
+    ijk = np.zeros((dg.K, req.dim), dtype=np.int32)
+    L.check(L.lib().rv_brick_cells(L.ctypes.byref(desc), L.ptr(ijk, L.ctypes.c_int32)))
+    xi = torch.tensor((gm.cell.points_1d + 1.0) / 2.0, dtype=torch.float64, device="cuda")
+    nq, dim = req.nq, req.dim
+    base = torch.from_numpy(ijk.astype(np.float64)).cuda()           # (K, dim)
+    # node m = i + nq*j (+ nq^2*k): coordinate d uses index (m // nq^d) % nq
+    m = torch.arange(nq ** dim, device="cuda")
+    X = [base[:, d:d + 1] + xi[(m // nq ** d) % nq][None, :] for d in range(dim)]  # (K, Np)
+    rng = np.random.default_rng(req.seed + 1)
+    c = rng.uniform(0.3, 0.7, dim) * req.n
+    if req.law == "advection":
+        r2 = sum((X[d] - c[d]) ** 2 for d in range(dim))
+        q = torch.exp(-r2 / (2.0 * (req.n / 8.0) ** 2))[:, None, :]
+    else:
+        # isentropic vortex (test/balancelaws/euler/isentropicvortex.jl:18-34),
+        # radius scaled to n/10, background velocity 0.5 along x
+        beta, u0 = 5.0, 0.5
+        s = req.n / 10.0
+        dx, dy = (X[0] - c[0]) / s, (X[1] - c[1]) / s
+        f = 1.0 - (dx * dx + dy * dy)
+        ef = torch.exp(f / 2.0)
+        u = u0 - beta / (2.0 * math.pi) * ef * dy
+        v = beta / (2.0 * math.pi) * ef * dx
+        Tt = 1.0 - (GAMMA - 1.0) * beta ** 2 / (8.0 * GAMMA * math.pi ** 2) * torch.exp(f)
+        rho = Tt ** (1.0 / (GAMMA - 1.0))
+        p = rho ** GAMMA
+        comps = [rho, rho * u, rho * v] + ([torch.zeros_like(rho)] if dim == 3 else [])
+        E = p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v)
+        q = torch.stack(comps + [E], dim=1)
+    return q.to(dg.torch_dtype).contiguous()
+
+
+class _Active:
+    """A resident solve: operator, states and its own clock (rv_lsrk_run's
+    arithmetic, FP32 time for FP32 states)."""
+
+    def __init__(self, req, a, b, stream):
+        import torch
+        self.req = req
+        self.gm, self.dg = make_operator(req, stream)
+        q = initial_state(req, self.gm, self.dg)
+        self.bufs = [q, torch.empty_like(q)]
+        self.dq = torch.zeros_like(q)
+        self.cur = 0
+        f32 = req.dtype == "f32"
+        rnd = (lambda v: float(np.float32(v))) if f32 else float
+        self.f32 = f32
+        self.a = [rnd(v) for v in a]
+        self.b = [rnd(v) for v in b]
+        self.time = rnd(0.0)
+        self.dt = rnd(req.dt())
+        self.T = rnd(req.T)
+        self.done = False
+
+    def next_step(self):
+        """dt of the coming step and whether it is the last (odesolvers.jl:21-25)."""
+        if self.f32:
+            f = np.float32
+            tdt = float(f(self.time) + f(self.dt))
+        else:
+            tdt = self.time + self.dt
+        if tdt >= self.T:
+            step = float(np.float32(self.T) - np.float32(self.time)) if self.f32 \
+                else self.T - self.time
+            return step, True
+        return self.dt, False
+
+    def bdt(self, s, step):
+        if self.f32:
+            return float(np.float32(self.b[s]) * np.float32(step))
+        return self.b[s] * step
+
+    def advance_clock(self, step, last):
+        if self.f32:
+            self.time = float(np.float32(self.time) + np.float32(step))
+        else:
+            self.time = self.time + step
+        self.done = last
+
+
+class Server:
+    """Continuous-batching server over one device.
+
+    hbm_budget -- device bytes the resident solves may hold together
+    """
+
+    def __init__(self, hbm_budget, scheme=L.RV_LSRK54):
+        from .odesolvers import LSRK
+        self.hbm_budget = int(hbm_budget)
+
+        class _NoDG:
+            def GridArray(self, fill):
+                return None
+        a, b, _ = LSRK(_NoDG(), None, 1.0, 0.0, scheme).coefficients()
+        self.a, self.b = a, b
+        self._batches = {}
+        self.peak_bytes = 0       # largest footprint held at once
+        self.peak_active = 0      # most solves resident at once
+        self.steps_run = 0        # batched steps (one per dimension present)
+
+    def _batch(self, dim, members):
+        """The batch of the current mix of one dimension (rebuilt when the
+        mix changes)."""
+        key = tuple(id(m) for m in members)
+        cached = self._batches.get(dim)
+        if cached is None or cached[0] != key:
+            self._batches[dim] = (key, Batch([m.dg for m in members]))
+        return self._batches[dim][1]
+
+    def _step(self, dim, members):
+        """One LSRK step of every member: len(a) batched stage launches."""
+        bt = self._batch(dim, members)
+        plan = [m.next_step() for m in members]
+        for s in range(len(self.a)):
+            q_in = [m.bufs[m.cur] for m in members]
+            q_out = [m.bufs[1 - m.cur] for m in members]
+            dq = [m.dq for m in members]
+            a = [m.a[s] for m in members]
+            bdt = [m.bdt(s, st) for m, (st, _) in zip(members, plan)]
+            bt.stage(q_out, q_in, dq, a, bdt)
+            for m in members:
+                m.cur = 1 - m.cur
+        for m, (st, last) in zip(members, plan):
+            m.advance_clock(st, last)
+            m.req.steps += 1
+
+    def run(self, requests, keep_results=False, stream=None):
+        """Serve the requests (arrival times relative to the call); returns
+        the requests with admitted / finished times filled in."""
+        import torch
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting, active = [], []
+        used = 0
+        t0 = time.perf_counter()
+        k = 0
+        while k < len(pending) or waiting or active:
+            now = time.perf_counter() - t0
+            while k < len(pending) and pending[k].arrival <= now:
+                waiting.append(pending[k])
+                k += 1
+            # admission in arrival order while the footprint fits
+            while waiting and (used + waiting[0].footprint() <= self.hbm_budget or not active):
+                r = waiting.pop(0)
+                r.admitted = time.perf_counter() - t0
+                used += r.footprint()
+                active.append(_Active(r, self.a, self.b, stream))
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.peak_active = max(self.peak_active, len(active))
+            if not active:
+                if k < len(pending):
+                    time.sleep(max(0.0, min(pending[k].arrival - now, 0.01)))
+                continue
+            for dim in (2, 3):
+                group = [m for m in active if m.req.dim == dim]
+                if group:
+                    self._step(dim, group)
+                    self.steps_run += 1
+            torch.cuda.synchronize()
+            now = time.perf_counter() - t0
+            still = []
+            for m in active:
+                if m.done:
+                    m.req.finished = now
+                    m.req.result = m.bufs[m.cur] if keep_results else None
+                    used -= m.req.footprint()
+                else:
+                    still.append(m)
+            active = still
+        self._batches.clear()
+        return requests
+
+
+def run_plain(requests, keep_results=False, stream=None, scheme=L.RV_LSRK54):
+    """The same stream served one request at a time (FIFO) by the plain path:
+    one operator, rv_lsrk_run, then the next request."""
+    import torch
+    from .odesolvers import LSRK, solve_
+    pending = sorted(requests, key=lambda r: r.arrival)
+    t0 = time.perf_counter()
+    for r in pending:
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        r.admitted = time.perf_counter() - t0
+        gm, dg = make_operator(r, stream)
+        q = initial_state(r, gm, dg)
+        solver = LSRK(dg, q, r.dt(), 0.0, scheme)
+        r.steps = solve_(q, r.T, solver)
+        torch.cuda.synchronize()
+        r.finished = time.perf_counter() - t0
+        r.result = q if keep_results else None
+        del solver, dg, gm
+    return requests
+
+
+def random_requests(n, rate, seed=0, dims=(2, 3), cells=(1e3, 1e5), degrees=(3, 7),
+                    T=(0.1, 1.0)):
+    """Poisson arrivals at `rate` per second; each request a 2-D or 3-D
+    advection or Euler solve, degree uniform in `degrees`, cell count
+    log-uniform in `cells`, T uniform, FP32 or FP64."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        dim = int(rng.choice(dims))
+        ncell = math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1])))
+        side = max(2, int(round(ncell ** (1.0 / dim))))
+        out.append(SolveRequest(rid=i, arrival=t, dim=dim,
+                                law=str(rng.choice(["advection", "euler"])),
+                                nq=int(rng.integers(degrees[0], degrees[1] + 1)) + 1,
+                                n=side, T=float(rng.uniform(*T)),
+                                dtype=str(rng.choice(["f32", "f64"])),
+                                seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def summarize(requests, wall):
+    tts = np.array([r.finished - r.arrival for r in requests])
+    dof_updates = sum(r.dofs * r.steps * 5 for r in requests)
+    return {"requests": len(requests), "wall_s": wall,
+            "throughput_dof_updates_per_s": dof_updates / wall,
+            "p50_time_to_solution_s": float(np.percentile(tts, 50)),
+            "p99_time_to_solution_s": float(np.percentile(tts, 99)),
+            "mean_time_to_solution_s": float(tts.mean())}

@@ -394,3 +394,58 @@ def test_launch_shape_does_not_change_results(torch, R, nq, dtype):
     assert dg.waves_per_block in (1, 2, 4) and set(times) == {1, 2, 4}
     with pytest.raises(R.RavenHIPError):
         dg.waves_per_block = 3
+
+
+def _warp3(x, y, z):
+    """A smooth warp of the brick (the kind generate(warp, gm) takes)."""
+    return (x + 0.05 * np.sin(np.pi * y) * np.sin(np.pi * z),
+            y + 0.04 * np.sin(np.pi * x) * np.cos(np.pi * z),
+            z + 0.03 * np.sin(2 * np.pi * x) * np.sin(np.pi * y))
+
+
+def _warp2(x, y):
+    return (x + 0.05 * np.sin(np.pi * y), y + 0.04 * np.sin(np.pi * x) * np.cos(np.pi * y))
+
+
+@pytest.mark.parametrize("dim,nq", [(3, 4), (3, 6), (3, 8), (2, 5), (2, 8)])
+def test_curved_metrics_match_oracle(torch, R, dim, nq):
+    """Device-side curved metrics (rv_curved_metrics) against the oracle's
+    restatement of hexvolumemetrics!/quadvolumemetrics! and the surface
+    metrics: the same operation sequence, so equal to the last bit."""
+    coords = [PB.julia_range(-1, 1, 4)] * dim
+    warp = _warp3 if dim == 3 else _warp2
+    og = O.BrickGrid(nq, coords, (True,) * dim, warp=warp)
+    grid = R.generate(R.GridManager(R.LobattoCell(*([nq] * dim)), R.brick(coords, (True,) * dim)),
+                      warp=warp)
+    assert np.array_equal(grid.points.reshape(-1), og.points)
+    gm, om = grid.metrics.reshape(-1), og.metrics
+    gs, os_ = grid.surface.reshape(-1), og.surf
+    dm = np.max(np.abs(gm - om) / np.maximum(np.abs(om), 1e-300))
+    ds = np.max(np.abs(gs - os_) / np.maximum(np.abs(os_), 1e-300))
+    say(f"   curved metrics dim={dim} nq={nq}: max rel diff volume {dm:.2e} surface {ds:.2e}")
+    assert np.array_equal(gm, om) and np.array_equal(gs, os_)
+
+
+@pytest.mark.parametrize("dim,nq,law,dtype", [(3, 6, "euler", "f64"), (2, 7, "euler", "f64"),
+                                              (3, 5, "advection", "f64"), (3, 6, "euler", "f32")])
+def test_rhs_on_warped_grid_matches_oracle(torch, R, dim, nq, law, dtype):
+    coords = [PB.julia_range(-1, 1, 4)] * dim
+    warp = _warp3 if dim == 3 else _warp2
+    og = O.BrickGrid(nq, coords, (True,) * dim, warp=warp)
+    grid = R.generate(R.GridManager(R.LobattoCell(*([nq] * dim)), R.brick(coords, (True,) * dim)),
+                      warp=warp)
+    u = [1.0, 0.5, 0.25][:dim] if law == "advection" else None
+    lawobj = R.AdvectionLaw(dim, u) if law == "advection" else R.EulerLaw(dim)
+    dg = R.DGSEM(law=lawobj, grid=grid, surface_numericalflux=R.RusanovFlux(), dtype=dtype)
+    assert dg.info.geometry == 2
+    odg = O.DG(og, law, u=u)
+    q = initial_state(og, law, dim)
+    want = np.zeros(q.size)
+    odg.rhs(want, q.copy(), 0.0, increment=False)
+    qd = to_dev(torch, dg, q)
+    dqd = dg.GridArray(0.0)
+    dg(dqd, qd, 0.0, increment=False)
+    dg.synchronize()
+    err = rel_l2(dqd.cpu().numpy().astype(np.float64).reshape(-1), want)
+    say(f"   warped {dim}-D {law} Nq={nq} {dtype}: rel L2 {err:.2e}")
+    assert err <= (RTOL_F64 if dtype == "f64" else 1e-4)
