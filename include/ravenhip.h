@@ -168,6 +168,32 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
  * may be NULL; *nstages receives 5 or 14.  No handle, no device work. */
 int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstages);
 
+/* --- adaptive forests ---------------------------------------------------- */
+
+/* The leaves of GridManager(cell, brick(...); min_level) after adapt!(gm,
+ * flags) calls (gridmanager.jl:30-91, 160-172), without p4est: the brick's
+ * trees in tree order, Morton order inside a tree, refinement by one level per
+ * call (flag 0x10 = AdaptRefine; 0 keeps a leaf; coarsening is not supported)
+ * followed by 2:1 balance across faces, edges and corners.  Host only. */
+typedef struct rv_forest rv_forest;
+int rv_forest_create(const rv_brick_desc *desc, rv_forest **out);
+void rv_forest_destroy(rv_forest *f);
+int64_t rv_forest_numcells(const rv_forest *f);
+int rv_forest_adapt(rv_forest *f, const uint8_t *flags);
+/* tree lattice coordinates (K x dim), levels (K) and coordinates inside the
+ * tree at each leaf's level (K x dim); any pointer may be NULL */
+int rv_forest_cells(const rv_forest *f, int32_t *tree, int32_t *level, int32_t *quad);
+/* Grid arrays generate() builds for the forest (gridmanager.jl:1084-1208):
+ * brick points (Np, dim, K) from the leaves' levels and coordinates
+ * (lobattocells.jl:1590-1674), vmapM / vmapP (1-based), boundary codes, and
+ * per (face, cell) kinds: 0 conforming or boundary, 1 hanging on a coarser
+ * neighbour, 2 covered by finer neighbours.  Non-conforming faces are
+ * self-mapped in vmapP as in Raven (lobattocells.jl:2072-2074).  Curved or
+ * refined cells take their metrics from rv_curved_metrics.  Any output
+ * pointer may be NULL. */
+int rv_forest_arrays(const rv_forest *f, int32_t nq, double *points, int64_t *vmapM,
+                     int64_t *vmapP, int64_t *bc, int32_t *face_kind);
+
 /* --- batches of independent solves -------------------------------------- */
 
 /* Many operators advanced together: ONE persistent launch per LSRK stage walks

@@ -1308,6 +1308,72 @@ int rv_brick_cells(const rv_brick_desc *bd, int32_t *ijk)
     return RV_OK;
 }
 
+}  // extern "C"
+
+struct rv_forest {
+    Forest *f = nullptr;
+    int dim = 0;
+};
+
+extern "C" {
+
+int rv_forest_create(const rv_brick_desc *bd, rv_forest **out)
+{
+    if (!out)
+        return fail(nullptr, RV_E_ARG, "null output");
+    *out = nullptr;
+    BrickSpec b;
+    if (int rc = brick_spec(bd, &b))
+        return rc;
+    if (b.dim < 2)
+        return fail(nullptr, RV_E_UNSUPPORTED, "adaptive forests need dim 2 or 3");
+    rv_forest *f = new rv_forest();
+    f->f = forest_new(b);
+    f->dim = b.dim;
+    *out = f;
+    return RV_OK;
+}
+
+void rv_forest_destroy(rv_forest *f)
+{
+    if (f) {
+        forest_free(f->f);
+        delete f;
+    }
+}
+
+int64_t rv_forest_numcells(const rv_forest *f)
+{
+    return f ? forest_size(f->f) : fail(nullptr, RV_E_ARG, "null forest");
+}
+
+int rv_forest_adapt(rv_forest *f, const uint8_t *flags)
+{
+    if (!f || !flags)
+        return fail(nullptr, RV_E_ARG, "null argument");
+    if (forest_adapt(f->f, flags) != 0)
+        return fail(nullptr, RV_E_UNSUPPORTED,
+                    "adapt flags must be 0 (keep) or 0x10 (refine) below the finest level");
+    return RV_OK;
+}
+
+int rv_forest_cells(const rv_forest *f, int32_t *tree, int32_t *level, int32_t *quad)
+{
+    if (!f)
+        return fail(nullptr, RV_E_ARG, "null forest");
+    forest_cells(f->f, tree, level, quad);
+    return RV_OK;
+}
+
+int rv_forest_arrays(const rv_forest *f, int32_t nq, double *points, int64_t *vmapM,
+                     int64_t *vmapP, int64_t *bc, int32_t *face_kind)
+{
+    if (!f || nq < 2 || nq > 32)
+        return fail(nullptr, RV_E_ARG, "invalid argument");
+    forest_arrays(f->f, nq, points, vmapM, vmapP, bc, face_kind);
+    return RV_OK;
+}
+
 int rv_lobatto(int nq, double *points, double *weights, double *D)
 {
     if (nq < 2 || nq > 32 || !points || !weights || !D)

@@ -100,4 +100,20 @@ int64_t brick_dtoc(const BrickSpec &b, int64_t *dtoc);
 void brick_affine(const BrickSpec &b, double *drdx, double *J);
 void brick_neighbours(const BrickSpec &b, int64_t *nbr);
 
+// Adaptive brick forest (forest.cpp): leaves after adapt! with 2:1 balance.
+struct Forest;
+Forest *forest_new(const BrickSpec &b);
+void forest_free(Forest *f);
+int64_t forest_size(const Forest *f);
+// flags per leaf: 0 keep, 0x10 refine; -1 on any other flag
+int forest_adapt(Forest *f, const uint8_t *flags);
+// tree lattice coordinates (K x dim), levels (K), coordinates inside the tree
+// at each leaf's level (K x dim); any pointer may be null
+void forest_cells(const Forest *f, int32_t *tree, int32_t *level, int32_t *quad);
+// points (Np, dim, K), vmapM / vmapP (2dim*Nfp, K) 1-based, boundary codes
+// (2dim, K) and face kinds (2dim, K: 0 conforming or boundary, 1 hanging on a
+// coarser neighbour, 2 covered by finer neighbours); any pointer may be null
+void forest_arrays(const Forest *f, int nq, double *points, int64_t *vmapM, int64_t *vmapP,
+                   int64_t *bc, int32_t *kind);
+
 }  // namespace rv

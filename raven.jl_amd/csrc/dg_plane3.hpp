@@ -17,8 +17,9 @@
 //  * the accumulators start from a*dq_old (affine cells) so dq_old and q load
 //    together at the top; dq and q_out are stored at the end;
 //  * every load of a cell is issued before its first wait; the face records
-//    come in with the state (scalar loads when a wave owns one cell) and the
-//    neighbour states of all six faces are requested together.
+//    come in with the state (scalar loads when a wave owns one cell); the
+//    x-face neighbour states are read with the state and parked in LDS, the
+//    y- and z-face ones are requested together after the plane loop.
 #pragma once
 
 #include "dg_kernels.hpp"
@@ -62,7 +63,9 @@ struct Plane3 {
     static constexpr int PLANE = NS * NQ * PAD;      // one flux component of one plane
     static constexpr int PBUF = 2 * PLANE;           // f1 and f2 of one plane
     static constexpr int FACEBUF = 4 * NS * NQ2;     // q- / lift of the four x,y faces
-    static constexpr int SLAB = (2 * PBUF > FACEBUF) ? 2 * PBUF : FACEBUF;
+    static constexpr int SLAB0 = (2 * PBUF > FACEBUF) ? 2 * PBUF : FACEBUF;
+    static constexpr int XFB = 2 * NS * NQ2;         // x-face neighbour states, read early
+    static constexpr int SLAB = SLAB0 + XFB;
     static_assert(EPW >= 1, "Nq^2 must not exceed the wave size");
     using S = Stage<3, NQ, LAW, R, GEO, FLUX>;
 };
@@ -114,6 +117,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                dz_on = (P.dir_mask >> 2) & 1;
 
     R q[NS][NQ], acc[NS][NQ];
+    R xq[2][NS];       // x-face neighbour states until they are parked in LDS
     R cx[NQ], cy[NQ];  // this lane's x / y derivative weights
     R sz = R(1);
     FaceRec rec[6];
@@ -170,6 +174,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     acc[s][k] = R(0);
+        }
+        // the x neighbours are processed at about the same time (they are
+        // next to this cell in the forest order), so their face nodes are
+        // read now, while those lines are still in L2; an x face touches every
+        // cache line of the neighbour's state
+        if (dx_on) {
+            load_neighbour<NQ, NS, R>(P, qbase, rec[0], l, xq[0]);
+            load_neighbour<NQ, NS, R>(P, qbase, rec[1], l, xq[1]);
         }
         const R fx = dx_on ? sx : R(0), fy = dy_on ? sy : R(0);
 #pragma unroll
@@ -267,6 +279,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     };
     if (active)
         plane_flux(0);
+    if (active && dx_on) {
+        R *xf = slab + T::SLAB0;
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                xf[(f * NS + s) * NQ2 + l] = xq[f][s];
+    }
     wave_lds_sync();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -278,15 +298,23 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
         wave_lds_sync();
     }
 
-    // ---- neighbour states of all six faces, issued together: node l of each
-    // x / y face (one node per lane) and the ends of the lane's own column for
-    // the two z faces
+    // ---- neighbour states of the y and z faces, issued together: node l of
+    // each y face (one node per lane) and the ends of the lane's own column
+    // for the two z faces; the x faces come from LDS
     R qp[6][NS];
     if (active) {
 #pragma unroll
-        for (int f = 0; f < 6; ++f)
+        for (int f = 2; f < 6; ++f)
             if ((P.dir_mask >> (f >> 1)) & 1)
                 load_neighbour<NQ, NS, R>(P, qbase, rec[f], l, qp[f]);
+        if (dx_on) {
+            const R *xf = slab + T::SLAB0;
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    qp[f][s] = xf[(f * NS + s) * NQ2 + l];
+        }
     }
 
     // ---- x and y faces: owners publish q-, every lane evaluates one node of

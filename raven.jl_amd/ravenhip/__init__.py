@@ -7,7 +7,7 @@ from ._lib import RavenHIPError, lib  # noqa: F401
 from .batch import Batch  # noqa: F401
 from .dgsem import (DGSEM, AdvectionLaw, CentralFlux, EulerLaw, RusanovFlux,  # noqa: F401
                     weightednorm)
-from .grids import GridManager, LobattoCell, brick, generate  # noqa: F401
+from .grids import AdaptNone, AdaptRefine, GridManager, LobattoCell, brick, generate  # noqa: F401
 from .odesolvers import LSRK54, LSRK144, dostep_, solve_  # noqa: F401
 
 lib()  # fail at import when the native library is missing

@@ -123,6 +123,13 @@ def lib():
                                            c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
         "rv_lobatto": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p]),
         "rv_brick_cells": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), c_i32p]),
+        "rv_forest_create": (ctypes.c_int, [ctypes.POINTER(RvBrickDesc), ctypes.POINTER(H)]),
+        "rv_forest_destroy": (None, [H]),
+        "rv_forest_numcells": (ctypes.c_int64, [H]),
+        "rv_forest_adapt": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_uint8)]),
+        "rv_forest_cells": (ctypes.c_int, [H, c_i32p, c_i32p, c_i32p]),
+        "rv_forest_arrays": (ctypes.c_int, [H, ctypes.c_int32, c_f64p, c_i64p, c_i64p, c_i64p,
+                                            c_i32p]),
         "rv_curved_metrics": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp,
                                              c_f64p, c_f64p, vp, vp, vp]),
         "rv_lsrk_coefficients": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p,

@@ -95,7 +95,7 @@ class DGSEM:
             cell = grid.referencecell
             keep = [np.ascontiguousarray(a) for a in (
                 cell.D_colmajor, cell.weights_1d, grid.metrics, grid.surface)]
-            ikeep = [np.ascontiguousarray(a, dtype=np.int64) for a in (
+            ikeep = [None if a is None else np.ascontiguousarray(a, dtype=np.int64) for a in (
                 grid.vmapM, grid.vmapP, grid.boundarycodes, grid.dtoc)]
             d.D, d.weights, d.metrics, d.surface_metrics = (L.ptr(a, ctypes.c_double) for a in keep)
             d.vmapM, d.vmapP, d.boundary_codes, d.dtoc = (L.ptr(a, ctypes.c_int64) for a in ikeep)

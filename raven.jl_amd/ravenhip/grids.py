@@ -62,6 +62,9 @@ def brick(coordinates, periodic=None):
     return BrickGrid(coordinates, periodic)
 
 
+AdaptNone, AdaptRefine = 0x00, 0x10  # gridmanager.jl:27
+
+
 class GridManager:
     def __init__(self, cell, coarsegrid, min_level=0):
         if cell.dim != coarsegrid.dim:
@@ -69,6 +72,71 @@ class GridManager:
         self.cell = cell
         self.coarsegrid = coarsegrid
         self.min_level = int(min_level)
+        self._forest = None
+
+    def __del__(self):
+        if getattr(self, "_forest", None) is not None:
+            L.lib().rv_forest_destroy(self._forest)
+            self._forest = None
+
+    def _ensure_forest(self):
+        if self._forest is None:
+            desc = self.brick_desc()
+            _check_uniform(self.coarsegrid, desc)
+            f = L.H()
+            L.check(L.lib().rv_forest_create(L.ctypes.byref(desc), L.ctypes.byref(f)))
+            self._forest = f
+        return self._forest
+
+    def __len__(self):
+        if self._forest is None:
+            return int(L.lib().rv_brick_numcells(L.ctypes.byref(self.brick_desc())))
+        return int(L.lib().rv_forest_numcells(self._forest))
+
+    @property
+    def adapted(self):
+        return self._forest is not None
+
+    def adapt(self, flags):
+        """adapt!(gm, flags) (gridmanager.jl:160-172): refine the leaves flagged
+        AdaptRefine by one level, then balance the forest 2:1."""
+        f = self._ensure_forest()
+        fl = np.ascontiguousarray(flags, dtype=np.uint8)
+        if fl.size != len(self):
+            raise ValueError("one flag per cell")
+        L.check(L.lib().rv_forest_adapt(f, fl.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_uint8))))
+
+    def cells(self):
+        """(tree, level, quad): tree lattice coordinates (K, dim), levels (K)
+        and each leaf's coordinates inside its tree at its level (K, dim)."""
+        dim = self.cell.dim
+        K = len(self)
+        tree = np.zeros((K, dim), dtype=np.int32)
+        level = np.zeros(K, dtype=np.int32)
+        quad = np.zeros((K, dim), dtype=np.int32)
+        if self._forest is None:
+            desc = self.brick_desc()
+            ijk = np.zeros((K, dim), dtype=np.int32)
+            L.check(L.lib().rv_brick_cells(L.ctypes.byref(desc), L.ptr(ijk, L.ctypes.c_int32)))
+            tree[:] = ijk >> self.min_level
+            quad[:] = ijk - (tree << self.min_level)
+            level[:] = self.min_level
+        else:
+            L.check(L.lib().rv_forest_cells(self._forest, L.ptr(tree, L.ctypes.c_int32),
+                                            L.ptr(level, L.ctypes.c_int32),
+                                            L.ptr(quad, L.ctypes.c_int32)))
+        return tree, level, quad
+
+    def cell_centres(self):
+        """Centres of the leaves in brick coordinates (before any warp)."""
+        tree, level, quad = self.cells()
+        cg = self.coarsegrid
+        out = np.zeros(tree.shape)
+        for d in range(cg.dim):
+            v = cg.coordinates[d]
+            frac = (quad[:, d] + 0.5) / (1 << level).astype(np.float64)
+            out[:, d] = v[tree[:, d]] + (v[tree[:, d] + 1] - v[tree[:, d]]) * frac
+        return out
 
     def brick_desc(self, law=L.RV_LAW_ADVECTION, dtype=L.RV_F64, flux=L.RV_FLUX_RUSANOV,
                    gamma=1.4, velocity=(0.0, 0.0, 0.0), stream=None):
@@ -134,7 +202,10 @@ def generate(gm, warp=None):
     warp(x, y[, z]) maps arrays of node coordinates to a tuple of arrays (the
     vectorised form of Raven's point-wise warp); warped grids take the curved
     metric path (gridmanager.jl:1160-1166), which runs on the device
-    (rv_curved_metrics)."""
+    (rv_curved_metrics), as do the refined cells of an adapted forest
+    (gm.adapt); those grids carry no continuous numbering (dtoc is None)."""
+    if gm.adapted:
+        return _generate_forest(gm, warp)
     desc = gm.brick_desc()
     _check_uniform(gm.coarsegrid, desc)
     lib = L.lib()
@@ -164,6 +235,33 @@ def generate(gm, warp=None):
                      axis=1))
         metrics, surface = curved_metrics(gm.cell, points)
     return Grid(gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc.value)
+
+
+def _generate_forest(gm, warp):
+    f = gm._forest
+    dim, nq = gm.cell.dim, gm.cell.nq
+    K = len(gm)
+    Np, nfp = nq ** dim, nq ** (dim - 1)
+    nfpt = 2 * dim * nfp
+    points = np.zeros((K, dim, Np))
+    vmapM = np.zeros((K, nfpt), dtype=np.int64)
+    vmapP = np.zeros((K, nfpt), dtype=np.int64)
+    bc = np.zeros((K, 2 * dim), dtype=np.int64)
+    kind = np.zeros((K, 2 * dim), dtype=np.int32)
+    c = L.ctypes
+    L.check(L.lib().rv_forest_arrays(f, nq, L.ptr(points, c.c_double), L.ptr(vmapM, c.c_int64),
+                                     L.ptr(vmapP, c.c_int64), L.ptr(bc, c.c_int64),
+                                     L.ptr(kind, c.c_int32)))
+    if warp is not None:
+        X = warp(*[points[:, d, :] for d in range(dim)])
+        points = np.ascontiguousarray(
+            np.stack([np.broadcast_to(np.asarray(x, dtype=np.float64), (K, Np)) for x in X],
+                     axis=1))
+    metrics, surface = curved_metrics(gm.cell, points)
+    grid = Grid(gm, points, metrics, surface, vmapM, vmapP, bc, None, 0)
+    grid.face_kind = kind
+    grid.levels = gm.cells()[1]
+    return grid
 
 
 def curved_metrics(cell, points):

@@ -14,6 +14,7 @@ the ones it would get alone, so its result does not depend on the mix.
 ``run_plain`` serves the same stream one request at a time through the plain
 per-operator path (rv_lsrk_run) for comparison.
 """
+import itertools
 import math
 import time
 from dataclasses import dataclass, field
@@ -136,8 +137,11 @@ class _Active:
     """A resident solve: operator, states and its own clock (rv_lsrk_run's
     arithmetic, FP32 time for FP32 states)."""
 
+    _serial = itertools.count()
+
     def __init__(self, req, a, b, stream):
         import torch
+        self.key = next(_Active._serial)  # unique for the process (ids are reused)
         self.req = req
         self.gm, self.dg = make_operator(req, stream)
         q = initial_state(req, self.gm, self.dg)
@@ -203,7 +207,7 @@ class Server:
     def _batch(self, dim, members):
         """The batch of the current mix of one dimension (rebuilt when the
         mix changes)."""
-        key = tuple(id(m) for m in members)
+        key = tuple(m.key for m in members)
         cached = self._batches.get(dim)
         if cached is None or cached[0] != key:
             self._batches[dim] = (key, Batch([m.dg for m in members]))

@@ -449,3 +449,76 @@ def test_rhs_on_warped_grid_matches_oracle(torch, R, dim, nq, law, dtype):
     err = rel_l2(dqd.cpu().numpy().astype(np.float64).reshape(-1), want)
     say(f"   warped {dim}-D {law} Nq={nq} {dtype}: rel L2 {err:.2e}")
     assert err <= (RTOL_F64 if dtype == "f64" else 1e-4)
+
+
+class _ArrayGrid:
+    """The arrays the oracle's operator reads, taken from a generated grid."""
+
+    def __init__(self, grid):
+        self.dim, self.nq, self.K = grid.dim, grid.nq, grid.K
+        self.Np = grid.Np
+        self.x1d, self.w1d, self.D = O.gll(grid.nq)
+        self.metrics = np.ascontiguousarray(grid.metrics).reshape(-1)
+        self.surf = np.ascontiguousarray(grid.surface).reshape(-1)
+        self.vmapM = np.ascontiguousarray(grid.vmapM).reshape(-1)
+        self.vmapP = np.ascontiguousarray(grid.vmapP).reshape(-1)
+        self.bc = np.ascontiguousarray(grid.boundarycodes).reshape(-1)
+
+    def point_fields(self):
+        return self.points
+
+
+def _adapted_grid(R, dim, nq, n, rounds, radius, warp):
+    gm = R.GridManager(R.LobattoCell(*([nq] * dim)),
+                       R.brick([PB.julia_range(0, 10, n + 1)] * dim, (True,) * dim))
+    for r in range(rounds):
+        c = gm.cell_centres()
+        flags = np.where((np.linalg.norm(c - 5.0, axis=1) < radius) & (gm.cells()[1] <= r),
+                         R.AdaptRefine, R.AdaptNone)
+        gm.adapt(flags)
+    return R.generate(gm, warp=warp)
+
+
+def _warp10(x, y, z):
+    """A smooth warp of [0,10]^3 (periodic in each direction)."""
+    k = 2 * np.pi / 10
+    return (x + 0.3 * np.sin(k * y) * np.sin(k * z), y + 0.25 * np.sin(k * x) * np.cos(k * z),
+            z + 0.2 * np.sin(k * x) * np.sin(k * y))
+
+
+@pytest.mark.parametrize("nq,dtype", [(6, "f64"), (4, "f64"), (6, "f32")])
+def test_rhs_on_adapted_warped_grid_matches_oracle(torch, R, nq, dtype):
+    """Refined 2:1-balanced forest with a warp: hanging faces are self-mapped
+    as in Raven, so the oracle fed the same arrays is the reference."""
+    grid = _adapted_grid(R, 3, nq, 6, 2, 2.1, _warp10)
+    say(f"   adapted grid: K={grid.K}, levels {np.bincount(grid.levels)}, "
+        f"hanging faces {int(np.sum(grid.face_kind == 1))}")
+    dg = R.DGSEM(law=R.EulerLaw(3), grid=grid, surface_numericalflux=R.RusanovFlux(), dtype=dtype)
+    ag = _ArrayGrid(grid)
+    ag.points = grid.points
+    odg = O.DG(ag, "euler")
+    rng = np.random.default_rng(11)
+    P = grid.points
+    q = PB.isentropic_vortex_3d(P, 0.0, centre=(5.0, 5.0),
+                                noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    want = np.zeros(q.size)
+    odg.rhs(want, q.copy(), 0.0, increment=False)
+    qd = to_dev(torch, dg, q)
+    dqd = dg.GridArray(0.0)
+    dg(dqd, qd, 0.0, increment=False)
+    dg.synchronize()
+    err = rel_l2(dqd.cpu().numpy().astype(np.float64).reshape(-1), want)
+    say(f"   adapted warped Nq={nq} {dtype}: rel L2 {err:.2e}")
+    assert err <= (RTOL_F64 if dtype == "f64" else 1e-4)
+    if dtype == "f64":
+        # two whole LSRK54 steps
+        dt = 0.125 * (10 / 6 / 4) / (nq - 1) / (1 + np.sqrt(1.4))
+        qo, dqo = q.copy(), np.zeros(q.size)
+        qg = to_dev(torch, dg, q)
+        solver = R.LSRK54(dg, qg, dt)
+        tmp = torch.empty_like(qg)
+        for st in range(2):
+            odg.lsrk_step(qo, dqo, st * dt, dt)
+            R.dostep_(qg, solver, tmp)
+        dg.synchronize()
+        assert rel_l2(qg.cpu().numpy().reshape(-1), qo) <= RTOL_F64
