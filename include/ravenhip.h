@@ -142,6 +142,21 @@ void *rv_stream(const rv_handle *h);
 int rv_set_waves_per_block(rv_handle *h, int waves_per_block);
 int rv_get_waves_per_block(const rv_handle *h);
 
+/* Couple the hanging faces through mortars instead of Raven's self-mapping
+ * (lobattocells.jl:2072-2074): face kinds (2dim, K) as rv_forest_arrays gives
+ * them and partners as from rv_forest_mortars.  On each fine sub-face the coarse trace
+ * is interpolated with tohalves_1d (lobattocells.jl:15-16), the numerical flux
+ * is evaluated with the fine face's normal and weight, and the coarse cell
+ * receives the transpose projection, so the scheme stays conservative.  Every
+ * rv_rhs / rv_lsrk_stage / rv_lsrk_run then runs one more kernel after the
+ * stage kernel.  Needs general geometry.  No counterpart in Raven's RHS, which
+ * builds these structures but does not read them. */
+int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *partner);
+
+/* tohalves_1d (lobattocells.jl:15-16): row-major [a][b] = l_b((x_a -+ 1)/2),
+ * computed in quad precision and rounded. */
+int rv_lobatto_halves(int nq, double *lower, double *upper);
+
 /* --- the hot path ------------------------------------------------------ */
 
 /* dg(dq, q, t; increment) (dgsem.jl:84-126):
@@ -193,6 +208,14 @@ int rv_forest_cells(const rv_forest *f, int32_t *tree, int32_t *level, int32_t *
  * pointer may be NULL. */
 int rv_forest_arrays(const rv_forest *f, int32_t nq, double *points, int64_t *vmapM,
                      int64_t *vmapP, int64_t *bc, int32_t *face_kind);
+/* Partners of the non-conforming faces, (4, 2dim, K) int64 (column-major as
+ * the other face arrays: entry k of face f of cell e at [k + 4 (f + 2dim e)]),
+ * 0-based cells: a hanging face holds (coarse cell, sub-face s1 + 2 s2, -1, -1)
+ * where s1, s2 are the halves along the face's first and second tangential
+ * direction; a covered face holds its 2^(dim-1) finer cells in sub-face
+ * order; conforming faces hold -1.  These are the facts Raven keeps in
+ * vmapNC / nctoface / nctypes (lobattocells.jl:2208-2275). */
+int rv_forest_mortars(const rv_forest *f, int64_t *partner);
 
 /* --- batches of independent solves -------------------------------------- */
 

@@ -976,6 +976,108 @@ void orc_dg_rhs(orc_dg *dg, double *dq, const double *q, double t, int increment
             }
 }
 
+/* Mortar coupling of hanging faces: restatement of the library's opt-in
+ * extension (Raven itself leaves these faces self-mapped and has no mortar in
+ * its right-hand side, lobattocells.jl:2072-2074).  Adds to dq, for every cell
+ * and face in order: on a hanging face (kind 1) -MJI wsJ_f nf_s, with nf_s the
+ * Rusanov flux between the fine state and the coarse trace interpolated with
+ * tohalves_1d (lower/upper, row-major [a][b]); on a covered face (kind 2) the
+ * transpose projection of the fine faces' wsJ_f nf_s onto the coarse nodes,
+ * times MJI.  `surf_full` holds the surface metrics with the lift weights of
+ * these faces (the operator's own `surf` has them zeroed).  partner as
+ * rv_forest_mortars: (4, 2dim, K). */
+void orc_dg_mortar_add(const orc_dg *dg, const int32_t *kind, const int64_t *partner,
+                       const double *lower, const double *upper, const double *surf_full,
+                       const double *q, double *dq)
+{
+    const int dim = dg->dim, nq = dg->nq, ns = dg->ns, nf = 2 * dim;
+    const int64_t Np = dg->Np, nfp = dg->nfp, nfpt = nf * nfp;
+    const int nsub = 1 << (dim - 1);
+    const double *H[2] = {lower, upper};
+    double tr[5 * 64], g[4 * 5 * 64];
+    for (int64_t e = 0; e < dg->K; ++e)
+        for (int f = 0; f < nf; ++f) {
+            const int k = kind[f + nf * e];
+            if (k == 0)
+                continue;
+            const int64_t *pt = partner + 4 * (f + nf * e);
+            /* the trace to interpolate: coarse neighbour's face f^1 or own face f */
+            const int64_t src = k == 1 ? pt[0] : e;
+            const int fsrc = k == 1 ? (f ^ 1) : f;
+            for (int64_t m = 0; m < nfp; ++m) {
+                int64_t node = face_volume_node(dim, nq, fsrc, m);
+                for (int s = 0; s < ns; ++s)
+                    tr[s * nfp + m] = q[node + Np * (s + (int64_t)ns * src)];
+            }
+            const int first = k == 1 ? (int)pt[1] : 0, last = k == 1 ? (int)pt[1] : nsub - 1;
+            for (int sub = first; sub <= last; ++sub) {
+                const int s1 = sub & 1, s2 = (sub >> 1) & 1;
+                const int64_t fine = k == 1 ? e : pt[sub];
+                const int ffine = k == 1 ? f : (f ^ 1);
+                for (int64_t m = 0; m < nfp; ++m) {
+                    const int a = (int)(m % nq), b = (int)(m / nq);
+                    double qi[5], qf[5], nv[3], wsJ, nflx[5];
+                    for (int s = 0; s < ns; ++s) {
+                        double v = 0.0;
+                        if (dim == 2) {
+                            for (int i = 0; i < nq; ++i)
+                                v = v + H[s1][a * nq + i] * tr[s * nfp + i];
+                        } else {
+                            for (int jj = 0; jj < nq; ++jj) {
+                                double inner = 0.0;
+                                for (int i = 0; i < nq; ++i)
+                                    inner = inner + H[s1][a * nq + i] * tr[s * nfp + i + nq * jj];
+                                v = v + H[s2][b * nq + jj] * inner;
+                            }
+                        }
+                        qi[s] = v;
+                    }
+                    const int64_t fnode = face_volume_node(dim, nq, ffine, m);
+                    for (int s = 0; s < ns; ++s)
+                        qf[s] = q[fnode + Np * (s + (int64_t)ns * fine)];
+                    const double *sg = surf_full + (dim + 2) * (ffine * nfp + m + nfpt * fine);
+                    for (int c = 0; c < dim; ++c)
+                        nv[c] = sg[c];
+                    wsJ = sg[dim + 1];
+                    rusanov(dg, nv, qf, qi, nflx);
+                    for (int s = 0; s < ns; ++s) {
+                        const double w = wsJ * nflx[s];
+                        if (k == 1) {
+                            const double mji = dg->MJI[fnode + Np * e];
+                            dq[fnode + Np * (s + (int64_t)ns * e)] += -(mji * w);
+                        } else {
+                            g[(sub * ns + s) * nfp + m] = w;
+                        }
+                    }
+                }
+            }
+            if (k == 2)
+                for (int64_t m = 0; m < nfp; ++m) {
+                    const int i = (int)(m % nq), j = (int)(m / nq);
+                    const int64_t node = face_volume_node(dim, nq, f, m);
+                    for (int s = 0; s < ns; ++s) {
+                        double v = 0.0;
+                        for (int sub = 0; sub < nsub; ++sub) {
+                            const int s1 = sub & 1, s2 = (sub >> 1) & 1;
+                            if (dim == 2) {
+                                for (int a = 0; a < nq; ++a)
+                                    v = v + H[s1][a * nq + i] * g[(sub * ns + s) * nfp + a];
+                            } else {
+                                for (int b = 0; b < nq; ++b) {
+                                    double inner = 0.0;
+                                    for (int a = 0; a < nq; ++a)
+                                        inner = inner +
+                                                H[s1][a * nq + i] * g[(sub * ns + s) * nfp + a + nq * b];
+                                    v = v + H[s2][b * nq + j] * inner;
+                                }
+                            }
+                        }
+                        dq[node + Np * (s + (int64_t)ns * e)] += dg->MJI[node + Np * e] * v;
+                    }
+                }
+        }
+}
+
 void orc_dg_lsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double time,
                       double dt)
 {

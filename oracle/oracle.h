@@ -104,6 +104,14 @@ void orc_dg_lsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double time
 int64_t orc_dg_solve(orc_dg *dg, int scheme, double *q, double dt, double t0,
                      double timeend, double *t_out);
 
+/* Mortar coupling of hanging faces (the library's opt-in extension; Raven has
+ * none): adds the mortar fluxes to dq.  kind (2dim, K), partner (4, 2dim, K)
+ * as rv_forest_mortars, tohalves_1d lower/upper row-major, surf_full the
+ * surface metrics with the non-conforming faces' lift weights kept. */
+void orc_dg_mortar_add(const orc_dg *dg, const int32_t *kind, const int64_t *partner,
+                       const double *lower, const double *upper, const double *surf_full,
+                       const double *q, double *dq);
+
 /* weightednorm(dg, q) component norms (measures.jl:6-14), p = 2. */
 void orc_dg_weightednorm(const orc_dg *dg, const double *q, double *comp);
 

@@ -54,6 +54,8 @@ def lib():
         L.orc_dg_ns.argtypes = [ctypes.c_void_p]
         L.orc_dg_ns.restype = ctypes.c_int
         L.orc_dg_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_dg_mortar_add.argtypes = [ctypes.c_void_p, i32p, i64p, f64p, f64p, f64p, f64p,
+                                         f64p]
         L.orc_dg_rhs.argtypes = [ctypes.c_void_p, f64p, f64p, ctypes.c_double, ctypes.c_int]
         L.orc_dg_lsrk_step.argtypes = [ctypes.c_void_p, ctypes.c_int, f64p, f64p,
                                        ctypes.c_double, ctypes.c_double]
@@ -169,6 +171,15 @@ class DG:
         t = ctypes.c_double(0)
         n = lib().orc_dg_solve(self.h, scheme, q, dt, t0, timeend, ctypes.byref(t))
         return n, t.value
+
+    def mortar_add(self, kind, partner, lower, upper, surf_full, q, dq):
+        """Add the mortar fluxes of hanging faces to dq (orc_dg_mortar_add)."""
+        self._mortar_keep = [np.ascontiguousarray(kind, dtype=np.int32).reshape(-1),
+                             np.ascontiguousarray(partner, dtype=np.int64).reshape(-1),
+                             np.ascontiguousarray(lower, dtype=np.float64).reshape(-1),
+                             np.ascontiguousarray(upper, dtype=np.float64).reshape(-1),
+                             np.ascontiguousarray(surf_full, dtype=np.float64).reshape(-1)]
+        lib().orc_dg_mortar_add(self.h, *self._mortar_keep, q, dq)
 
     def weightednorm(self, q):
         comp = np.zeros(self.ns)

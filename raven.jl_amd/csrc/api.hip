@@ -67,6 +67,12 @@ struct rv_handle {
     int faces_self = 0, faces_explicit = 0;
     bool is_brick = false;
     BrickSpec brick{};
+    // mortar coupling of hanging faces (rv_enable_mortars)
+    MortarLauncher mortar = nullptr;
+    int64_t mortar_cells = 0;
+    int64_t *m_cells = nullptr, *m_partner = nullptr;
+    int32_t *m_kind = nullptr;
+    double *m_sgeo = nullptr, *m_halves = nullptr;
 };
 
 namespace {
@@ -162,9 +168,10 @@ void free_handle(rv_handle *h)
 {
     if (!h)
         return;
-    void *bufs[] = {h->faces, h->explicit_idx, h->affine, h->vgeo, h->mji, h->sgeo, h->Dt,
-                    h->Dw, h->winv, h->MJ, h->partials, h->qtmp, h->dtoc, h->ctod_ptr,
-                    h->ctod_rows};
+    void *bufs[] = {h->faces,  h->explicit_idx, h->affine,   h->vgeo,     h->mji,
+                    h->sgeo,   h->Dt,           h->Dw,       h->winv,     h->MJ,
+                    h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
+                    h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -454,6 +461,19 @@ __global__ void curved_surface_metrics_kernel(int dim, int nq, int64_t K, const 
     out[dim + 1] = wsJ;
 }
 
+// Lift weight 0 on the non-conforming faces: the stage kernels then leave
+// them out and the mortar kernel supplies their flux.
+__global__ void zero_lift_kernel(double *sgeo, const int32_t *kind, int64_t K, int nf, int dim,
+                                 int64_t nfp)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= K * nf * nfp)
+        return;
+    const int64_t ef = idx / nfp, m = idx - ef * nfp;
+    if (kind[ef] != 0)
+        sgeo[(ef * (dim + 1) + dim) * nfp + m] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 
 int resolve_stream(rv_handle *h, void *stream)
@@ -733,6 +753,20 @@ int launch(rv_handle *h, const StageParams &P)
         hipError_t e = hipGetLastError();
         return fail(h, RV_E_HIP, std::string("stage launch failed: ") + hipGetErrorString(e));
     }
+    if (h->mortar) {
+        MortarParams M{};
+        M.stage = P;
+        M.ncells = h->mortar_cells;
+        M.cells = h->m_cells;
+        M.kind = h->m_kind;
+        M.partner = h->m_partner;
+        M.sgeo = h->m_sgeo;
+        M.halves = h->m_halves;
+        if (h->mortar(M, h->stream) != 0) {
+            hipError_t e = hipGetLastError();
+            return fail(h, RV_E_HIP, std::string("mortar launch failed: ") + hipGetErrorString(e));
+        }
+    }
     return RV_OK;
 }
 
@@ -969,6 +1003,60 @@ int rv_set_waves_per_block(rv_handle *h, int waves_per_block)
 }
 
 int rv_get_waves_per_block(const rv_handle *h) { return h ? h->wpb : 0; }
+
+int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *partner)
+{
+    if (!h || !face_kind || !partner)
+        return fail(h, RV_E_ARG, "null argument");
+    if (h->geo != GEO_GENERAL)
+        return fail(h, RV_E_UNSUPPORTED, "mortars need per-node (general) geometry");
+    if (h->mortar)
+        return fail(h, RV_E_STATE, "mortars are already enabled");
+    const int nf = 2 * h->dim, nsub = 1 << (h->dim - 1);
+    std::vector<int64_t> cells;
+    for (int64_t e = 0; e < h->K; ++e) {
+        bool any = false;
+        for (int f = 0; f < nf; ++f) {
+            const int k = face_kind[e * nf + f];
+            const int64_t *p = partner + 4 * (e * nf + f);
+            if (k < 0 || k > 2)
+                return fail(h, RV_E_ARG, "face kind must be 0, 1 or 2");
+            if (k == 1 && (p[0] < 0 || p[0] >= h->K || p[1] < 0 || p[1] >= nsub))
+                return fail(h, RV_E_ARG, "hanging face without a valid coarse partner");
+            if (k == 2)
+                for (int s = 0; s < nsub; ++s)
+                    if (p[s] < 0 || p[s] >= h->K)
+                        return fail(h, RV_E_ARG, "covered face without its finer partners");
+            any = any || k != 0;
+        }
+        if (any)
+            cells.push_back(e);
+    }
+    MortarLauncher ml = find_mortar_launcher(h->dim, h->nq, h->law, h->dtype == RV_F32 ? 4 : 8,
+                                             h->flux);
+    if (!ml)
+        return fail(h, RV_E_UNSUPPORTED, "no mortar kernel for this (dim, Nq)");
+    std::vector<double> halves(2 * h->nq * h->nq);
+    lobatto_halves(h->nq, halves.data(), halves.data() + h->nq * h->nq);
+    const size_t nsg = (size_t)h->K * nf * (h->dim + 1) * h->nfp;
+    int rc;
+    if ((rc = upload(h, &h->m_cells, cells.data(), std::max<size_t>(cells.size(), 1))) ||
+        (rc = upload(h, &h->m_kind, face_kind, (size_t)h->K * nf)) ||
+        (rc = upload(h, &h->m_partner, partner, (size_t)h->K * nf * 4)) ||
+        (rc = upload(h, &h->m_halves, halves.data(), halves.size())) ||
+        (rc = dalloc(h, &h->m_sgeo, nsg)))
+        return rc;
+    HIP_TRY(h, hipMemcpyAsync(h->m_sgeo, h->sgeo, nsg * sizeof(double), hipMemcpyDeviceToDevice,
+                              h->stream));
+    const int64_t n = h->K * nf * h->nfp;
+    hipLaunchKernelGGL(zero_lift_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       h->stream, h->sgeo, h->m_kind, h->K, nf, h->dim, h->nfp);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->mortar_cells = (int64_t)cells.size();
+    h->mortar = ml;
+    return RV_OK;
+}
 
 int rv_rhs(rv_handle *h, void *dq, const void *q, double t, int increment)
 {
@@ -1362,6 +1450,22 @@ int rv_forest_cells(const rv_forest *f, int32_t *tree, int32_t *level, int32_t *
     if (!f)
         return fail(nullptr, RV_E_ARG, "null forest");
     forest_cells(f->f, tree, level, quad);
+    return RV_OK;
+}
+
+int rv_forest_mortars(const rv_forest *f, int64_t *partner)
+{
+    if (!f || !partner)
+        return fail(nullptr, RV_E_ARG, "null argument");
+    forest_mortars(f->f, partner);
+    return RV_OK;
+}
+
+int rv_lobatto_halves(int nq, double *lower, double *upper)
+{
+    if (nq < 2 || nq > 32 || !lower || !upper)
+        return fail(nullptr, RV_E_ARG, "invalid argument");
+    lobatto_halves(nq, lower, upper);
     return RV_OK;
 }
 

@@ -60,6 +60,19 @@ struct KernelKey {
     int dim, nq, law, real_bytes, geo, flux;
 };
 
+// Mortar coupling of hanging faces (mortar.hip), launched after a stage.
+struct MortarParams {
+    StageParams stage;       // the stage just launched (states, law, mji)
+    int64_t ncells;          // cells with non-conforming faces
+    const int64_t *cells;
+    const int32_t *kind;     // (K, 2dim): 0 conforming, 1 hanging, 2 covered
+    const int64_t *partner;  // (K, 2dim, 4), see forest_mortars
+    const double *sgeo;      // surface geometry with the lift weights kept
+    const double *halves;    // tohalves_1d, (2, Nq, Nq) row-major
+};
+using MortarLauncher = int (*)(const MortarParams &, void *stream);
+MortarLauncher find_mortar_launcher(int dim, int nq, int law, int real_bytes, int flux);
+
 // Batched stages (batch.hip): one work-list item per wave's worth of cells of
 // one member; `kind` selects the member's (Nq, law, precision) routine.
 struct BatchItem {
@@ -91,6 +104,9 @@ void brick_quadrants(const BrickSpec &b, int32_t *tree, int32_t *quad);
 // rounded lattice lo + (hi-lo)*i/n like Julia's range(lo, stop=hi, length=n+1))
 void brick_vertices(const BrickSpec &b, int d, double *v);
 void lobatto_operators(int nq, double *x, double *w, double *D);
+// tohalves_1d (lobattocells.jl:15-16): interpolation from the GLL points to
+// the GLL points of the lower / upper half, row-major [a][b] = l_b(y_a)
+void lobatto_halves(int nq, double *lower, double *upper);
 // full Raven-layout arrays (any pointer may be null)
 void brick_arrays(const BrickSpec &b, double *points, double *metrics, double *surf,
                   int64_t *vmapM, int64_t *vmapP, int64_t *bc);
@@ -115,5 +131,9 @@ void forest_cells(const Forest *f, int32_t *tree, int32_t *level, int32_t *quad)
 // coarser neighbour, 2 covered by finer neighbours); any pointer may be null
 void forest_arrays(const Forest *f, int nq, double *points, int64_t *vmapM, int64_t *vmapP,
                    int64_t *bc, int32_t *kind);
+// partners of the non-conforming faces, (K, 2dim, 4): a hanging face holds
+// (coarse cell, sub-face index s1 + 2*s2, -1, -1); a covered face holds the
+// 2^(dim-1) finer cells in sub-face order; conforming faces hold -1
+void forest_mortars(const Forest *f, int64_t *partner);
 
 }  // namespace rv

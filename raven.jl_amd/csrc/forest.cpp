@@ -344,4 +344,54 @@ void forest_arrays(const Forest *f, int nq, double *points, int64_t *vmapM, int6
     }
 }
 
+void forest_mortars(const Forest *f, int64_t *partner)
+{
+    const BrickSpec &b = f->b;
+    const int dim = b.dim, nf = 2 * dim;
+    const int64_t K = forest_size(f);
+    for (int64_t e = 0; e < K; ++e) {
+        const Leaf &l = f->leaves[e];
+        for (int fc = 0; fc < nf; ++fc) {
+            int64_t *out = partner + 4 * (fc + nf * e);
+            for (int k = 0; k < 4; ++k)
+                out[k] = -1;
+            const int d = fc / 2;
+            int tang[2] = {0, 0}, nt = 0;
+            for (int c = 0; c < dim; ++c)
+                if (c != d)
+                    tang[nt++] = c;
+            int delta[3] = {0, 0, 0};
+            delta[d] = (fc & 1) ? 1 : -1;
+            Leaf o;
+            if (!f->shifted(l, delta, &o))
+                continue;
+            int32_t lev = 0;
+            const int64_t c = f->containing(o, &lev);
+            if (c >= 0 && lev < l.level) {
+                // hanging: the sub-face of the coarse face this leaf covers
+                const Leaf &cl = f->leaves[c];
+                int sub = 0;
+                for (int k = 0; k < nt; ++k)
+                    sub |= (l.c[tang[k]] - 2 * cl.c[tang[k]]) << k;
+                out[0] = c;
+                out[1] = sub;
+            } else if (c < 0) {
+                // covered: the finer leaves touching this face, sub-face order
+                for (int sub = 0; sub < (1 << nt); ++sub) {
+                    Leaf ch = o;
+                    ch.level = o.level + 1;
+                    for (int cc = 0; cc < dim; ++cc)
+                        ch.c[cc] = 2 * o.c[cc];
+                    ch.c[d] += (fc & 1) ? 0 : 1;
+                    for (int k = 0; k < nt; ++k)
+                        ch.c[tang[k]] += (sub >> k) & 1;
+                    int32_t lv = 0;
+                    const int64_t fine = f->containing(ch, &lv);
+                    out[sub] = (fine >= 0 && lv == ch.level) ? fine : -2;
+                }
+            }
+        }
+    }
+}
+
 }  // namespace rv

@@ -234,14 +234,15 @@ void brick_vertices(const BrickSpec &b, int d, double *v)
         v[i] = (i == 0) ? b.lo[d] : (i == n) ? b.hi[d] : (double)((lo * (n - i) + hi * i) / n);
 }
 
-void lobatto_operators(int nq, double *xo, double *wo, double *Do)
+// Gauss-Lobatto nodes in quad precision: Newton on P_n'(x) from the
+// Chebyshev-Lobatto nodes, then exact symmetry (and an exact zero centre).
+void gll_nodes_quad(int nq, std::vector<__float128> &x)
 {
     const int n = nq - 1;
-    std::vector<__float128> x(nq), pn(nq);
+    x.assign(nq, 0);
     x[0] = -1;
     x[n] = 1;
     for (int i = 1; i < n; ++i) {
-        // Newton on P_n'(x) from the Chebyshev-Lobatto node
         __float128 xi = -cosq(M_PIq * i / n);
         for (int it = 0; it < 100; ++it) {
             __float128 dp;
@@ -257,6 +258,13 @@ void lobatto_operators(int nq, double *xo, double *wo, double *Do)
         x[n - i] = -x[i];
     if (nq % 2)
         x[n / 2] = 0;
+}
+
+void lobatto_operators(int nq, double *xo, double *wo, double *Do)
+{
+    const int n = nq - 1;
+    std::vector<__float128> x, pn(nq);
+    gll_nodes_quad(nq, x);
     for (int i = 0; i < nq; ++i) {
         __float128 dp;
         pn[i] = legendre(n, x[i], &dp);
@@ -274,6 +282,26 @@ void lobatto_operators(int nq, double *xo, double *wo, double *Do)
                 v = (__float128)n * (n + 1) / 4;
             Do[a + nq * c] = (double)v;
         }
+}
+
+void lobatto_halves(int nq, double *lower, double *upper)
+{
+    std::vector<__float128> x;
+    gll_nodes_quad(nq, x);
+    // I[a][b] = l_b(y_a), y_a = (x_a -+ 1) / 2 (spectralinterpolation, tohalves)
+    for (int h = 0; h < 2; ++h) {
+        double *out = h ? upper : lower;
+        for (int a = 0; a < nq; ++a) {
+            const __float128 y = (x[a] + (h ? 1 : -1)) / 2;
+            for (int b = 0; b < nq; ++b) {
+                __float128 v = 1;
+                for (int c = 0; c < nq; ++c)
+                    if (c != b)
+                        v *= (y - x[c]) / (x[b] - x[c]);
+                out[a * nq + b] = (double)v;
+            }
+        }
+    }
 }
 
 void brick_affine(const BrickSpec &b, double *drdx, double *J)

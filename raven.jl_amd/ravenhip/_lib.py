@@ -130,6 +130,9 @@ def lib():
         "rv_forest_cells": (ctypes.c_int, [H, c_i32p, c_i32p, c_i32p]),
         "rv_forest_arrays": (ctypes.c_int, [H, ctypes.c_int32, c_f64p, c_i64p, c_i64p, c_i64p,
                                             c_i32p]),
+        "rv_forest_mortars": (ctypes.c_int, [H, c_i64p]),
+        "rv_lobatto_halves": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p]),
+        "rv_enable_mortars": (ctypes.c_int, [H, c_i32p, c_i64p]),
         "rv_curved_metrics": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp,
                                              c_f64p, c_f64p, vp, vp, vp]),
         "rv_lsrk_coefficients": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p,

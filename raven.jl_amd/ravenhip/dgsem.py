@@ -173,6 +173,15 @@ class DGSEM:
                                       self._check_state(dq), float(a), float(b_dt), float(time)),
                 self._h)
 
+    def enable_mortars(self, grid=None):
+        """Couple the hanging faces of an adapted grid through mortars
+        (rv_enable_mortars) instead of Raven's self-mapped faces."""
+        g = grid if grid is not None else self.grid
+        kind = np.ascontiguousarray(g.face_kind, dtype=np.int32)
+        partner = np.ascontiguousarray(g.mortar_partner, dtype=np.int64)
+        L.check(L.lib().rv_enable_mortars(self._h, L.ptr(kind, ctypes.c_int32),
+                                          L.ptr(partner, ctypes.c_int64)), self._h)
+
     # -- launch shape ------------------------------------------------------
     @property
     def waves_per_block(self):

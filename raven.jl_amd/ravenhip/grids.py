@@ -258,8 +258,11 @@ def _generate_forest(gm, warp):
             np.stack([np.broadcast_to(np.asarray(x, dtype=np.float64), (K, Np)) for x in X],
                      axis=1))
     metrics, surface = curved_metrics(gm.cell, points)
+    partner = np.zeros((K, 2 * dim, 4), dtype=np.int64)
+    L.check(L.lib().rv_forest_mortars(f, L.ptr(partner, c.c_int64)))
     grid = Grid(gm, points, metrics, surface, vmapM, vmapP, bc, None, 0)
     grid.face_kind = kind
+    grid.mortar_partner = partner
     grid.levels = gm.cells()[1]
     return grid
 

@@ -522,3 +522,52 @@ def test_rhs_on_adapted_warped_grid_matches_oracle(torch, R, nq, dtype):
             R.dostep_(qg, solver, tmp)
         dg.synchronize()
         assert rel_l2(qg.cpu().numpy().reshape(-1), qo) <= RTOL_F64
+
+
+@pytest.mark.parametrize("nq,dtype", [(6, "f64"), (4, "f64"), (5, "f32")])
+def test_mortar_stage_on_adapted_grid_matches_oracle(torch, R, nq, dtype):
+    """Mortar-coupled hanging faces (rv_enable_mortars) against the oracle's
+    restatement: one fused LSRK stage (dq and q_out) on a warped 2:1 forest,
+    and discrete conservation of the coupled right-hand side on the device."""
+    grid = _adapted_grid(R, 3, nq, 6, 2, 2.1, _warp10)
+    dg = R.DGSEM(law=R.EulerLaw(3), grid=grid, surface_numericalflux=R.RusanovFlux(), dtype=dtype)
+    dg.enable_mortars(grid)
+    K, nfp = grid.K, nq * nq
+    ag = _ArrayGrid(grid)
+    ag.points = grid.points
+    surf = np.array(grid.surface).reshape(K, 6, nfp, 5)
+    surf_full = surf.reshape(-1).copy()
+    surf[grid.face_kind != 0, :, 4] = 0.0
+    ag.surf = surf.reshape(-1)
+    odg = O.DG(ag, "euler")
+    lower, upper = np.zeros(nq * nq), np.zeros(nq * nq)
+    R._lib.check(R.lib().rv_lobatto_halves(nq, R._lib.ptr(lower, R._lib.ctypes.c_double),
+                                           R._lib.ptr(upper, R._lib.ctypes.c_double)))
+    rng = np.random.default_rng(12)
+    P = grid.points
+    q = PB.isentropic_vortex_3d(P, 0.0, centre=(5.0, 5.0),
+                                noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    dq0 = 1e-3 * rng.standard_normal(q.size)
+    a, bdt = -0.6, 2e-3
+    L_o = np.zeros(q.size)
+    odg.rhs(L_o, q.copy(), 0.0, increment=False)
+    odg.mortar_add(grid.face_kind, grid.mortar_partner, lower, upper, surf_full, q, L_o)
+    want_dq = a * dq0 + L_o
+    want_q = q + bdt * want_dq
+    qd, qo, dqd = to_dev(torch, dg, q), None, to_dev(torch, dg, dq0)
+    qo = torch.empty_like(qd)
+    dg.lsrk_stage(qo, qd, dqd, a, bdt)
+    Ld = dg.GridArray(0.0)
+    dg(Ld, qd, 0.0, increment=False)
+    dg.synchronize()
+    tol = RTOL_F64 if dtype == "f64" else 1e-4
+    e1 = rel_l2(dqd.cpu().numpy().astype(np.float64).reshape(-1), want_dq)
+    e2 = rel_l2(qo.cpu().numpy().astype(np.float64).reshape(-1), want_q)
+    e3 = rel_l2(Ld.cpu().numpy().astype(np.float64).reshape(-1), L_o)
+    say(f"   mortar Nq={nq} {dtype}: dq {e1:.2e}, q_out {e2:.2e}, rhs {e3:.2e}")
+    assert e1 <= tol and e2 <= tol and e3 <= tol
+    if dtype == "f64":
+        tot = dg.integrals(Ld)
+        scale = dg.integrals(torch.abs(Ld).contiguous())
+        say(f"   sum MJ dq = {tot}")
+        assert np.all(np.abs(tot) <= 1e-11 * scale)
