@@ -259,6 +259,85 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     }
 
 
+def run_adaptive(R, torch, steps=200, nq=6, seed=99):
+    """3-D Euler on a 16^3 brick over [0,10]^3 with a seeded smooth warp,
+    refined 2:1-balanced to level 2 inside the sphere of radius 2 at the
+    centre, degree 5: FP32 state and flux arithmetic (FP64 metrics) against
+    the all-FP64 run after `steps` LSRK54 steps, with hanging faces coupled
+    through mortars, and Raven's self-mapped hanging faces for comparison."""
+    import problems as PB
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(0.15, 0.3, 3)
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    k = 2 * np.pi / 10
+
+    def warp(x, y, z):
+        return (x + amp[0] * np.sin(k * y + ph[0]) * np.sin(k * z),
+                y + amp[1] * np.sin(k * x + ph[1]) * np.sin(k * z),
+                z + amp[2] * np.sin(k * x) * np.sin(k * y + ph[2]))
+
+    cell = R.LobattoCell(nq, nq, nq)
+    gm = R.GridManager(cell, R.brick([np.linspace(0, 10, 17)] * 3, (True,) * 3))
+    for r in range(2):
+        c = gm.cell_centres()
+        flags = np.where((np.linalg.norm(c - 5.0, axis=1) < 2.0) & (gm.cells()[1] <= r),
+                         R.AdaptRefine, R.AdaptNone)
+        gm.adapt(flags)
+    t0 = time.perf_counter()
+    grid = R.generate(gm, warp=warp)
+    t_grid = time.perf_counter() - t0
+    P = grid.points
+    q0 = PB.isentropic_vortex_3d(P, 0.0, gamma=GAMMA, centre=(5.0, 5.0),
+                                 noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    hmin = 10 / 16 / 4
+    dt = 0.125 * hmin / (nq - 1) / (1.0 + np.sqrt(GAMMA))
+    out = {"workload": "3-D Euler, 16^3 periodic brick on [0,10]^3, seeded smooth warp, refined "
+                       "2:1-balanced to level 2 inside r=2 at the centre, degree 5, isentropic "
+                       "vortex + 1e-3 density noise, LSRK54",
+           "cells": int(grid.K), "levels": np.bincount(grid.levels).tolist(),
+           "hanging_faces": int(np.sum(grid.face_kind == 1)), "steps": steps, "dt": dt,
+           "grid_setup_s": t_grid}
+    for mortar in (True, False):
+        res = {}
+        for dtype in ("f64", "f32"):
+            dg = R.DGSEM(law=R.EulerLaw(3, GAMMA), grid=grid,
+                         surface_numericalflux=R.RusanovFlux(), dtype=dtype)
+            if mortar:
+                dg.enable_mortars(grid)
+            q = torch.from_numpy(q0).to(dg.torch_dtype).cuda().reshape(dg.K, dg.ns, dg.Np)
+            q = q.contiguous()
+            i0 = dg.integrals(q)
+            solver = R.LSRK54(dg, q, dt)
+            R.solve_(q, dt * 2, solver)  # warm-up (two steps), then restart
+            q.copy_(torch.from_numpy(q0).to(dg.torch_dtype).reshape(q.shape))
+            solver.time = 0.0
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            n = R.solve_(q, (steps - 0.25) * dt, solver)
+            ev[1].record()
+            torch.cuda.synchronize()
+            sec = ev[0].elapsed_time(ev[1]) / 1e3
+            i1 = dg.integrals(q)
+            res[dtype] = dict(q=q.double().cpu().numpy().reshape(-1), sec=sec, steps=n,
+                              mass_drift=float((i1[0] - i0[0]) / i0[0]),
+                              energy_drift=float((i1[4] - i0[4]) / i0[4]))
+            del dg, q, solver
+        q64, q32 = res["f64"]["q"], res["f32"]["q"]
+        key = "mortar" if mortar else "self_mapped_hanging_faces"
+        dofs = grid.K * nq ** 3 * 5
+        out[key] = {
+            "fp32_rel_l2_vs_fp64": float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64)),
+            "fp64_dof_updates_per_s": dofs * 5 * res["f64"]["steps"] / res["f64"]["sec"],
+            "fp32_dof_updates_per_s": dofs * 5 * res["f32"]["steps"] / res["f32"]["sec"],
+            "mass_drift_fp64": res["f64"]["mass_drift"], "mass_drift_fp32": res["f32"]["mass_drift"],
+            "energy_drift_fp64": res["f64"]["energy_drift"],
+            "energy_drift_fp32": res["f32"]["energy_drift"],
+            "steps_run": res["f64"]["steps"],
+        }
+    return out
+
+
 def run_served(R, torch, rate, n_req=32, seed=7, budget_gb=64):
     """A Poisson request stream (2-D / 3-D advection / Euler, degree 3-7,
     1e3..1e5 cells log-uniform, T in [0.1, 1], FP32 / FP64) served with
@@ -296,6 +375,8 @@ def main():
                     help="skip the 256-solve batched workload")
     ap.add_argument("--no-served", action="store_true",
                     help="skip the served request stream")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the adaptive warped degree-5 FP32/FP64 workload")
     ap.add_argument("--serve-rates", default="8,32",
                     help="request arrival rates (per second) of the served stream")
     args = ap.parse_args()
@@ -393,6 +474,10 @@ def main():
         del r32
     del r64
     torch.cuda.empty_cache()
+    if not args.no_adaptive:
+        log("bench: adaptive warped 16^3 degree-5 workload")
+        out["adaptive"] = run_adaptive(R, torch)
+        torch.cuda.empty_cache()
     if not args.no_batched:
         log("bench: 256 batched 2-D solves")
         out["batched"] = run_batched(R, torch)
