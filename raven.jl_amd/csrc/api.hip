@@ -737,6 +737,7 @@ StageParams stage_params(rv_handle *h)
     P.affine = h->affine;
     P.vgeo = h->vgeo;
     P.mji = h->mji;
+    P.mj = h->MJ;
     P.sgeo = h->sgeo;
     P.Dt = h->Dt;
     P.Dw = h->Dw;

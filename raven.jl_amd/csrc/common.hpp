@@ -43,6 +43,7 @@ struct StageParams {
     // general geometry (FP64): wJ*dRdX rows, MJI, face normals and wsJ
     const double *vgeo;          // (K, dim, dim, Np): [e][d][c][node] = wJ*dRdX[d,c]
     const double *mji;           // (K, Np)
+    const double *mj;            // (K, Np) = wJ
     const double *sgeo;          // (K, 2dim, dim+1, Nfp): n_c ..., wsJ
     // operators (FP64, device)
     const double *Dt;            // [i*Nq + n] = D[n,i]            (general)

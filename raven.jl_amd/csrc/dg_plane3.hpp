@@ -123,8 +123,10 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     FaceRec rec[6];
     const R a = R(P.a), bdt = R(P.b_dt);
     // a*dq_old is formed at the first z scatter so that the dq loads need not
-    // complete before the first plane's flux
-    const bool late_scale = GEO == GEO_AFFINE && P.read_dq && dz_on;
+    // complete before the first plane's flux; general cells accumulate
+    // MJ * dq (the final update multiplies by MJI), so their factor is a*MJ
+    const bool late_scale = P.read_dq && dz_on;
+    R zscale[NQ];  // general cells: a * MJ of the column's nodes
 
     // the z-derivative weights M[kk*Nq + k], one per lane, broadcast with
     // v_readlane inside the plane loop (no scalar loads or waits there)
@@ -155,18 +157,21 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int s = 0; s < NS; ++s)
                 q[s][k] = qe[s * NP + k * NQ2];
         const R *dqe = static_cast<const R *>(P.dq) + e * (NS * NP) + l;
-        if (GEO == GEO_AFFINE && P.read_dq) {
+        if (P.read_dq) {
 #pragma unroll
             for (int k = 0; k < NQ; ++k)
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     acc[s][k] = dqe[s * NP + k * NQ2];
+#pragma unroll
+            for (int k = 0; k < NQ; ++k)
+                zscale[k] = GEO == GEO_AFFINE ? a : a * R(P.mj[e * NP + l + k * NQ2]);
             if (!late_scale) {
 #pragma unroll
                 for (int k = 0; k < NQ; ++k)
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
-                        acc[s][k] = a * acc[s][k];
+                        acc[s][k] = zscale[k] * acc[s][k];
             }
         } else {
 #pragma unroll
@@ -196,6 +201,18 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     struct alignas(16) Vec {
         R v[T::VEC];
     };
+    // general cells: wJ*dRdX of the next plane, requested one plane ahead
+    double gnext[3][3];
+    auto load_geo = [&](const int k) {
+        if constexpr (GEO == GEO_GENERAL) {
+            const int node = l + k * NQ2;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    gnext[d][c] = P.vgeo[((e * 3 + d) * 3 + c) * NP + node];
+        }
+    };
     auto plane_flux = [&](const int k) {
         R *buf = slab + (k & 1) * PBUF;
         R qn[NS];
@@ -213,13 +230,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 f3[s] = sz * Fa[2][s];
             }
         } else {
-            const int node = l + k * NQ2;
             R g[3][3];
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    g[d][c] = R(P.vgeo[((e * 3 + d) * 3 + c) * NP + node]);
+                    g[d][c] = R(gnext[d][c]);
+            if (k + 1 < NQ)
+                load_geo(k + 1);
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 f1[s] = g[0][0] * Fa[0][s] + g[0][1] * Fa[1][s] + g[0][2] * Fa[2][s];
@@ -241,7 +259,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 for (int s = 0; s < NS; ++s) {
                     R base = acc[s][kk];
                     if (k == 0 && late_scale)
-                        base = a * base;
+                        base = zscale[kk] * base;
                     acc[s][kk] = base + w * f3[s];
                 }
             }
@@ -277,8 +295,10 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             acc[s][k] = v;
         }
     };
-    if (active)
+    if (active) {
+        load_geo(0);
         plane_flux(0);
+    }
     if (active && dx_on) {
         R *xf = slab + T::SLAB0;
 #pragma unroll
@@ -429,11 +449,8 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 R v = acc[s][k];
-                if constexpr (GEO == GEO_GENERAL) {
+                if constexpr (GEO == GEO_GENERAL)
                     v = mji * v;
-                    if (P.read_dq)
-                        v = a * dqo[s * NP + k * NQ2] + v;
-                }
                 dqo[s * NP + k * NQ2] = v;
                 if (P.write_q)
                     qo[s * NP + k * NQ2] = q[s][k] + bdt * v;
