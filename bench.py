@@ -10,6 +10,7 @@ build runs on one GPU.
 """
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -326,13 +327,20 @@ def run_adaptive(R, torch, steps=200, nq=6, seed=99):
         q64, q32 = res["f64"]["q"], res["f32"]["q"]
         key = "mortar" if mortar else "self_mapped_hanging_faces"
         dofs = grid.K * nq ** 3 * 5
+        finite = bool(np.isfinite(q64).all() and np.isfinite(q32).all())
+
+        def num(v):
+            return float(v) if np.isfinite(v) else None
         out[key] = {
-            "fp32_rel_l2_vs_fp64": float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64)),
+            "diverged": not finite,
+            "fp32_rel_l2_vs_fp64": num(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+            if finite else None,
             "fp64_dof_updates_per_s": dofs * 5 * res["f64"]["steps"] / res["f64"]["sec"],
             "fp32_dof_updates_per_s": dofs * 5 * res["f32"]["steps"] / res["f32"]["sec"],
-            "mass_drift_fp64": res["f64"]["mass_drift"], "mass_drift_fp32": res["f32"]["mass_drift"],
-            "energy_drift_fp64": res["f64"]["energy_drift"],
-            "energy_drift_fp32": res["f32"]["energy_drift"],
+            "mass_drift_fp64": num(res["f64"]["mass_drift"]),
+            "mass_drift_fp32": num(res["f32"]["mass_drift"]),
+            "energy_drift_fp64": num(res["f64"]["energy_drift"]),
+            "energy_drift_fp32": num(res["f32"]["energy_drift"]),
             "steps_run": res["f64"]["steps"],
         }
     return out
@@ -353,11 +361,14 @@ def run_served(R, torch, rate, n_req=32, seed=7, budget_gb=64):
     S.run_plain(plain)
     wall_plain = time.perf_counter() - t0
     a, b = S.summarize(served, wall), S.summarize(plain, wall_plain)
-    return {"rate_per_s": rate, "requests": n_req, "hbm_budget_gb": budget_gb,
+    return {"rate_per_s": rate if math.isfinite(rate) else "burst", "requests": n_req,
+            "hbm_budget_gb": budget_gb,
             "served": a, "one_at_a_time": b,
             "throughput_ratio": a["throughput_dof_updates_per_s"] /
             b["throughput_dof_updates_per_s"],
             "peak_resident": server.peak_active,
+            "server_host_s": {k: round(v, 3) for k, v in server.host_time.items()},
+            "server_steps": server.steps_run,
             "steps_match": all(x.steps == y.steps for x, y in zip(served, plain))}
 
 
@@ -377,8 +388,9 @@ def main():
                     help="skip the served request stream")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="skip the adaptive warped degree-5 FP32/FP64 workload")
-    ap.add_argument("--serve-rates", default="8,32",
-                    help="request arrival rates (per second) of the served stream")
+    ap.add_argument("--serve-rates", default="8,32,burst",
+                    help="request arrival rates (per second) of the served stream; "
+                         "'burst': every request arrives at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is "
@@ -484,8 +496,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_served:
         out["served"] = []
-        for rate in [float(x) for x in args.serve_rates.split(",") if x]:
-            log(f"bench: served stream at {rate:g} requests/s")
+        for tok in [x.strip() for x in args.serve_rates.split(",") if x.strip()]:
+            rate = float("inf") if tok == "burst" else float(tok)
+            log(f"bench: served stream at {tok} requests/s")
             out["served"].append(run_served(R, torch, rate))
             torch.cuda.empty_cache()
     if not args.no_cpu_baseline:

@@ -237,6 +237,16 @@ int rv_batch_info(const rv_batch *b, int64_t *nitems, int64_t *nwaves, int32_t *
  *   dq[m] = a[m]*dq[m] + L_m(q_in[m]);  q_out[m] = q_in[m] + b_dt[m]*dq[m]. */
 int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, void *const *dq,
                    const double *a, const double *b_dt);
+/* One whole LSRK step of every member m with its own step size dt[m] (its
+ * final, shortened step included), ping-ponging between q0[m] and q1[m]: the
+ * state is read from q1[m] when cur[m] is 1, else from q0[m], and after an odd
+ * number of stages it ends in the other buffer.  dq[m] carries over between
+ * steps (zero it before the first).  Members with dt[m] == 0 are left
+ * untouched and launch nothing, so finished solves need not leave the batch.
+ * The stage tables of the step go to the device in one copy, then each stage
+ * launches once (2-D) or once per kind of the members stepping (3-D). */
+int rv_batch_step(rv_batch *b, void *const *q0, void *const *q1, void *const *dq,
+                  const int32_t *cur, const double *dt, int scheme);
 /* nsteps whole LSRK steps of every member with its own dt[m]: dq[m] is zeroed,
  * each step replays a captured hipGraph of the scheme's stages, and q[m] holds
  * the result (the library ping-pongs through buffers of its own).  The graphs

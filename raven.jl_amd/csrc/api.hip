@@ -2,9 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -57,6 +59,7 @@ struct rv_handle {
     double *affine = nullptr, *vgeo = nullptr, *mji = nullptr, *sgeo = nullptr;
     double *Dt = nullptr, *Dw = nullptr, *winv = nullptr;
     double *MJ = nullptr;
+    double *cellJ = nullptr;     // brick handles: per-cell J that MJ is filled from
     double *partials = nullptr;  // reduction scratch
     void *qtmp = nullptr;        // ping-pong state of rv_lsrk_run
     int64_t *dtoc = nullptr;     // 0-based
@@ -171,7 +174,8 @@ void free_handle(rv_handle *h)
     void *bufs[] = {h->faces,  h->explicit_idx, h->affine,   h->vgeo,     h->mji,
                     h->sgeo,   h->Dt,           h->Dw,       h->winv,     h->MJ,
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
-                    h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves};
+                    h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
+                    h->cellJ};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -461,6 +465,29 @@ __global__ void curved_surface_metrics_kernel(int dim, int nq, int64_t K, const 
     out[dim + 1] = wsJ;
 }
 
+// MJ of affine bricks: w_i w_j (w_k) J_e at every node.
+struct WeightsArg {
+    double w[32];
+};
+__global__ void affine_mj_kernel(double *MJ, const double *J, WeightsArg wa, int dim, int nq,
+                                 int64_t K)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t np = nq;
+    for (int d = 1; d < dim; ++d)
+        np *= nq;
+    if (idx >= np * K)
+        return;
+    const int64_t e = idx / np;
+    int64_t rem = idx - e * np;
+    double wp = 0;
+    for (int c = 0; c < dim; ++c) {
+        wp = c == 0 ? wa.w[rem % nq] : wp * wa.w[rem % nq];
+        rem /= nq;
+    }
+    MJ[idx] = wp * J[e];
+}
+
 // Lift weight 0 on the non-conforming faces: the stage kernels then leave
 // them out and the mortar kernel supplies their flux.
 __global__ void zero_lift_kernel(double *sgeo, const int32_t *kind, int64_t K, int nf, int dim,
@@ -732,16 +759,16 @@ StageParams stage_params(rv_handle *h)
     StageParams P{};
     P.K = h->K;
     P.dir_mask = h->dir_mask;
-    P.faces = h->faces;
-    P.explicit_idx = h->explicit_idx;
-    P.affine = h->affine;
-    P.vgeo = h->vgeo;
-    P.mji = h->mji;
-    P.mj = h->MJ;
-    P.sgeo = h->sgeo;
-    P.Dt = h->Dt;
-    P.Dw = h->Dw;
-    P.winv = h->winv;
+    P.faces = as_global(h->faces);
+    P.explicit_idx = as_global(h->explicit_idx);
+    P.affine = as_global(h->affine);
+    P.vgeo = as_global(h->vgeo);
+    P.mji = as_global(h->mji);
+    P.mj = as_global(h->MJ);
+    P.sgeo = as_global(h->sgeo);
+    P.Dt = as_global(h->Dt);
+    P.Dw = as_global(h->Dw);
+    P.winv = as_global(h->winv);
     P.gamma = h->gamma;
     for (int c = 0; c < 3; ++c)
         P.u[c] = h->u[c];
@@ -758,11 +785,11 @@ int launch(rv_handle *h, const StageParams &P)
         MortarParams M{};
         M.stage = P;
         M.ncells = h->mortar_cells;
-        M.cells = h->m_cells;
-        M.kind = h->m_kind;
-        M.partner = h->m_partner;
-        M.sgeo = h->m_sgeo;
-        M.halves = h->m_halves;
+        M.cells = as_global(h->m_cells);
+        M.kind = as_global(h->m_kind);
+        M.partner = as_global(h->m_partner);
+        M.sgeo = as_global(h->m_sgeo);
+        M.halves = as_global(h->m_halves);
         if (h->mortar(M, h->stream) != 0) {
             hipError_t e = hipGetLastError();
             return fail(h, RV_E_HIP, std::string("mortar launch failed: ") + hipGetErrorString(e));
@@ -899,7 +926,7 @@ int rv_create_brick(const rv_brick_desc *bd, rv_handle **out)
     h->geo = GEO_AFFINE;
     int rc = resolve_stream(h, bd->stream);
     const int64_t K = h->K, Np = h->Np, nf = 2 * dim;
-    std::vector<double> x(nq), w(nq), D(nq * nq), drdx(K * dim), J(K), MJ(K * Np);
+    std::vector<double> x(nq), w(nq), D(nq * nq), drdx(K * dim), J(K);
     std::vector<int64_t> nbr(K * nf);
     std::vector<FaceRec> recs(K * nf);
     if (!rc) {
@@ -907,15 +934,6 @@ int rv_create_brick(const rv_brick_desc *bd, rv_handle **out)
         brick_affine(b, drdx.data(), J.data());
         brick_neighbours(b, nbr.data());
         for (int64_t e = 0; e < K; ++e) {
-            for (int64_t n = 0; n < Np; ++n) {
-                int64_t rem = n;
-                double wp = 0;
-                for (int c = 0; c < dim; ++c) {
-                    wp = c == 0 ? w[rem % nq] : wp * w[rem % nq];
-                    rem /= nq;
-                }
-                MJ[e * Np + n] = wp * J[e];
-            }
             for (int f = 0; f < nf; ++f) {
                 const int64_t en = nbr[e * nf + f];
                 recs[e * nf + f] = en < 0 ? FaceRec{FACE_SELF, 1} : FaceRec{(int32_t)en, f ^ 1};
@@ -923,12 +941,26 @@ int rv_create_brick(const rv_brick_desc *bd, rv_handle **out)
                     h->faces_self++;
             }
         }
+        // MJ = w_i w_j (w_k) J per node, filled on the device from the
+        // per-cell J by a kernel queued on the handle's stream (ordered before
+        // every later use of MJ, which runs on that stream too); set-up neither
+        // waits for the stream nor frees memory, so it does not stall work
+        // already queued on the device.  The continuous numbering is built on
+        // the first rv_gather / rv_scatter (solving never needs it).
         if (!(rc = upload(h, &h->affine, drdx.data(), drdx.size())) &&
-            !(rc = upload(h, &h->MJ, MJ.data(), MJ.size())) &&
+            !(rc = upload(h, &h->cellJ, J.data(), J.size())) &&
+            !(rc = dalloc(h, &h->MJ, K * Np)) &&
             !(rc = upload(h, &h->faces, recs.data(), recs.size()))) {
-            std::vector<int64_t> dtoc(K * Np);
-            const int64_t nc = brick_dtoc(b, dtoc.data());
-            rc = build_gatherscatter(h, dtoc.data(), nc);
+            WeightsArg wa{};
+            for (int i = 0; i < nq; ++i)
+                wa.w[i] = w[i];
+            const int64_t n = K * Np;
+            if (n > 0)
+                hipLaunchKernelGGL(affine_mj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                                   0, h->stream, h->MJ, h->cellJ, wa, dim, nq, K);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                rc = fail(h, RV_E_HIP, std::string("MJ fill: ") + hipGetErrorString(e));
             if (!rc)
                 rc = finish_setup(h, D.data(), w.data());
         }
@@ -1065,9 +1097,9 @@ int rv_rhs(rv_handle *h, void *dq, const void *q, double t, int increment)
     if (!h || !dq || !q)
         return fail(h, RV_E_ARG, "null argument");
     StageParams P = stage_params(h);
-    P.q_in = q;
+    P.q_in = as_global(q);
     P.q_out = nullptr;
-    P.dq = dq;
+    P.dq = as_global(dq);
     P.a = 1.0;
     P.b_dt = 0.0;
     P.read_dq = increment ? 1 : 0;
@@ -1084,9 +1116,9 @@ int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double 
     if (q_out == q_in)
         return fail(h, RV_E_ARG, "q_out must not alias q_in");
     StageParams P = stage_params(h);
-    P.q_in = q_in;
-    P.q_out = q_out;
-    P.dq = dq;
+    P.q_in = as_global(q_in);
+    P.q_out = as_global(q_out);
+    P.dq = as_global(dq);
     P.a = a;
     P.b_dt = b_dt;
     P.read_dq = a != 0.0 ? 1 : 0;
@@ -1202,9 +1234,9 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
             final_step = true;
         }
         for (int s = 0; s < nst; ++s) {
-            P.q_in = cur;
-            P.q_out = alt;
-            P.dq = dq;
+            P.q_in = as_global(cur);
+            P.q_out = as_global(alt);
+            P.dq = as_global(dq);
             P.a = rnd(A[s]);
             P.b_dt = f32 ? (double)((float)rnd(B[s]) * (float)step_dt) : B[s] * step_dt;
             P.read_dq = P.a != 0.0 ? 1 : 0;
@@ -1248,12 +1280,26 @@ int rv_integrals(rv_handle *h, const void *q, double *out)
     return reduce(h, q, 1, out);
 }
 
+namespace {
+// GatherScatter tables: given to rv_create, or built from the brick on first use
+int ensure_gatherscatter(rv_handle *h)
+{
+    if (h->dtoc)
+        return RV_OK;
+    if (!h->is_brick)
+        return fail(h, RV_E_STATE, "no dtoc was given to rv_create");
+    std::vector<int64_t> dtoc(h->K * h->Np);
+    const int64_t nc = brick_dtoc(h->brick, dtoc.data());
+    return build_gatherscatter(h, dtoc.data(), nc);
+}
+}  // namespace
+
 int rv_gather(rv_handle *h, void *dest, const void *src, int nf)
 {
     if (!h || !dest || !src || nf < 1)
         return fail(h, RV_E_ARG, "invalid argument");
-    if (!h->ctod_ptr)
-        return fail(h, RV_E_STATE, "no dtoc was given to rv_create");
+    if (int rc = ensure_gatherscatter(h))
+        return rc;
     const int64_t blocks = (h->nc + 255) / 256;
     if (h->dtype == RV_F64)
         hipLaunchKernelGGL(gather_kernel<double>, dim3((unsigned)std::max<int64_t>(blocks, 1)),
@@ -1273,8 +1319,8 @@ int rv_scatter(rv_handle *h, void *dest, const void *src, int nf)
 {
     if (!h || !dest || !src || nf < 1)
         return fail(h, RV_E_ARG, "invalid argument");
-    if (!h->dtoc)
-        return fail(h, RV_E_STATE, "no dtoc was given to rv_create");
+    if (int rc = ensure_gatherscatter(h))
+        return rc;
     const int64_t n = h->Np * h->K;
     const int64_t blocks = std::max<int64_t>((n + 255) / 256, 1);
     if (h->dtype == RV_F64)
@@ -1493,7 +1539,11 @@ int rv_lobatto(int nq, double *points, double *weights, double *D)
 // Batches: many operators, one persistent launch per stage (batch.hip).
 // ===========================================================================
 struct rv_batch {
-    static constexpr int RING = 4;  // parameter tables in flight for rv_batch_stage
+    static constexpr int RING = 4;   // parameter tables in flight (rv_batch_stage / _step)
+    static constexpr int MAXST = 14;  // stages of the longest scheme (LSRK144)
+    // a kind with at least this many items (waves) fills the device on its
+    // own: 256 CUs x 4 waves
+    static constexpr int64_t OWN_LAUNCH_ITEMS = 1024;
     std::string err;
     int dim = 0, wpb = 2, slab_bytes = 0;
     int64_t nitems = 0, nwaves = 0;
@@ -1511,6 +1561,23 @@ struct rv_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     std::vector<const void *> graph_key;   // pointers, scheme and dt the graphs were built for
     int64_t device_bytes = 0;
+    // Kinds with enough items to fill the device on their own (every 3-D
+    // kind: one kernel holding all 3-D kinds would run out of registers)
+    // launch once per kind, with the kind's own slab and the single
+    // operator's launch shape.  The remaining small kinds share one mixed
+    // persistent launch over the tail of the work list [tail_first, nitems).
+    // Members are sorted by (small kind, kind), so a kind's members follow
+    // each other and the items of the members still stepping form a few
+    // contiguous runs.
+    struct KindRange {
+        int kind, slab_bytes, wpb;
+        int mbegin, mend;                  // positions in `order`
+    };
+    std::vector<KindRange> kind_ranges;
+    int64_t tail_first = 0;                // first item of the mixed launch
+    int tail_mbegin = 0;                   // position in `order` of its first member
+    std::vector<int> order;                // members sorted by (small kind, kind)
+    std::vector<int64_t> mfirst, mcount;   // each member's items
 };
 
 namespace {
@@ -1564,9 +1631,49 @@ void free_batch(rv_batch *b)
             return bfail(b, RV_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
     } while (0)
 
-int batch_launch(rv_batch *b, const StageParams *table, hipStream_t st)
+// Stages 0..nst-1 of the batch, stage s's table at table + s * n.  active
+// (optional) marks the members that step.  Kind by kind, all stages of the
+// kind's stepping members run back to back (a member's state stays in the
+// caches from one stage to the next, as when it runs alone); then the mixed
+// tail runs each stage once, or not at all when none of its members steps
+// (the others' items return at once, their K being 0).
+int batch_launch(rv_batch *b, const StageParams *table, int nst, hipStream_t st,
+                 const std::vector<char> *active = nullptr)
 {
-    if (launch_batch(b->dim, table, b->items, b->nitems, b->nwaves, b->slab_bytes, b->wpb, st)) {
+    int rc = 0;
+    const size_t n = b->members.size();
+    for (const auto &r : b->kind_ranges) {
+        // runs of consecutive stepping members: [first item, count)
+        std::vector<std::pair<int64_t, int64_t>> runs;
+        int64_t first = -1, count = 0;
+        for (int p = r.mbegin; p <= r.mend; ++p) {
+            const int m = p < r.mend ? b->order[p] : -1;
+            if (m < 0 || (active && !(*active)[m])) {
+                if (count > 0)
+                    runs.push_back({first, count});
+                first = -1, count = 0;
+                continue;
+            }
+            if (first < 0)
+                first = b->mfirst[m];
+            count += b->mcount[m];
+        }
+        for (int s = 0; s < nst && rc == 0; ++s)
+            for (const auto &run : runs)
+                if ((rc = launch_batch_kind(b->dim, r.kind, table + s * n, b->items, run.first,
+                                            run.second, r.slab_bytes, r.wpb, st)))
+                    break;
+        if (rc)
+            break;
+    }
+    const int64_t ntail = b->nitems - b->tail_first;
+    bool any = ntail > 0 && !active;
+    for (int p = b->tail_mbegin; ntail > 0 && active && p < (int)n; ++p)
+        any = any || (*active)[b->order[p]];
+    for (int s = 0; s < nst && any && rc == 0; ++s)
+        rc = launch_batch(b->dim, table + s * n, b->items + b->tail_first, ntail, b->nwaves,
+                          b->slab_bytes, b->wpb, st);
+    if (rc) {
         hipError_t e = hipGetLastError();
         return bfail(b, RV_E_HIP, std::string("batch launch failed: ") + hipGetErrorString(e));
     }
@@ -1586,7 +1693,7 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
         return bfail(nullptr, RV_E_ARG, "a batch needs at least one member");
     rv_batch *b = new rv_batch();
     b->members.assign(members, members + n);
-    std::vector<int> kind(n), epw(n);
+    std::vector<int> kind(n), epw(n), slab(n);
     for (int m = 0; m < n; ++m) {
         rv_handle *h = members[m];
         if (!h) {
@@ -1611,19 +1718,54 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
             free_batch(b);
             return bfail(nullptr, RV_E_UNSUPPORTED, "no batch kernel for this (dim, Nq)");
         }
-        b->slab_bytes = std::max(b->slab_bytes, sb);
+        slab[m] = (sb + 15) / 16 * 16;
     }
-    // work list: members grouped by kind, each member's cells in order
-    std::vector<int> order(n);
+    // items per kind decide which kinds launch on their own
+    std::map<int, int64_t> kind_items;
+    for (int m = 0; m < n; ++m)
+        kind_items[kind[m]] += (b->members[m]->K + epw[m] - 1) / epw[m];
+    std::vector<char> small(n);
+    for (int m = 0; m < n; ++m)
+        small[m] = b->dim != 3 && kind_items[kind[m]] < rv_batch::OWN_LAUNCH_ITEMS;
+    // work list: members sorted by (small kind, kind), each member's cells in order
+    std::vector<int> &order = b->order;
+    order.resize(n);
     for (int m = 0; m < n; ++m)
         order[m] = m;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kind[x] < kind[y]; });
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return small[x] != small[y] ? small[x] < small[y] : kind[x] < kind[y];
+    });
     std::vector<BatchItem> items;
-    for (int m : order)
+    b->mfirst.assign(n, 0);
+    b->mcount.assign(n, 0);
+    b->tail_mbegin = n;
+    b->tail_first = -1;
+    b->slab_bytes = 16;
+    for (int p = 0; p < n; ++p) {
+        const int m = order[p];
+        if (small[m] && b->tail_first < 0) {
+            b->tail_first = (int64_t)items.size();
+            b->tail_mbegin = p;
+        }
+        if (small[m])
+            b->slab_bytes = std::max(b->slab_bytes, slab[m]);
+        b->mfirst[m] = (int64_t)items.size();
         for (int64_t c = 0; c < b->members[m]->K; c += epw[m])
             items.push_back(BatchItem{m, kind[m], c});
+        b->mcount[m] = (int64_t)items.size() - b->mfirst[m];
+    }
     b->nitems = (int64_t)items.size();
-    b->slab_bytes = (b->slab_bytes + 15) / 16 * 16;
+    if (b->tail_first < 0)
+        b->tail_first = b->nitems;
+    // a kind launching on its own takes its first member's launch shape (the
+    // single operator's, autotuned or default)
+    for (int p = 0; p < b->tail_mbegin; ++p) {
+        const int m = order[p];
+        if (p == 0 || kind[m] != kind[order[p - 1]])
+            b->kind_ranges.push_back({kind[m], slab[m], b->members[m]->wpb, p, p + 1});
+        else
+            b->kind_ranges.back().mend = p + 1;
+    }
     auto bail = [&](const char *what, hipError_t e) {
         free_batch(b);
         return bfail(nullptr, RV_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -1634,23 +1776,21 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
     if ((e = hipMemcpy(b->items, items.data(), items.size() * sizeof(BatchItem),
                        hipMemcpyHostToDevice)))
         return bail("hipMemcpy", e);
-    if ((e = hipHostMalloc((void **)&b->host_ring, sizeof(StageParams) * n * rv_batch::RING)))
+    const size_t ring_entries = (size_t)n * rv_batch::MAXST * rv_batch::RING;
+    if ((e = hipHostMalloc((void **)&b->host_ring, sizeof(StageParams) * ring_entries)))
         return bail("hipHostMalloc", e);
-    if ((e = hipMalloc(&b->dev_ring, sizeof(StageParams) * n * rv_batch::RING)))
+    if ((e = hipMalloc(&b->dev_ring, sizeof(StageParams) * ring_entries)))
         return bail("hipMalloc", e);
     for (auto &ev : b->ring_ev)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)))
             return bail("hipEventCreate", e);
-    if ((e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking)))
-        return bail("hipStreamCreate", e);
-    b->device_bytes = (int64_t)(items.size() * sizeof(BatchItem) +
-                                sizeof(StageParams) * n * rv_batch::RING);
+    b->device_bytes = (int64_t)(items.size() * sizeof(BatchItem) + sizeof(StageParams) * ring_entries);
     int64_t resident = 0;
     if (batch_resident_waves(b->dim, b->wpb, b->slab_bytes, &resident) != 0 || resident <= 0) {
         free_batch(b);
         return bfail(nullptr, RV_E_HIP, "occupancy query of the batch kernel failed");
     }
-    b->nwaves = std::min<int64_t>(resident, std::max<int64_t>(b->nitems, 1));
+    b->nwaves = std::min<int64_t>(resident, std::max<int64_t>(b->nitems - b->tail_first, 1));
     *out = b;
     return RV_OK;
 }
@@ -1682,8 +1822,8 @@ int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, voi
         return bfail(b, RV_E_ARG, "null argument");
     const int n = (int)b->members.size();
     const int slot = b->ring;
-    StageParams *host = b->host_ring + (size_t)slot * n;
-    StageParams *dev = b->dev_ring + (size_t)slot * n;
+    StageParams *host = b->host_ring + (size_t)slot * n * rv_batch::MAXST;
+    StageParams *dev = b->dev_ring + (size_t)slot * n * rv_batch::MAXST;
     // the pinned slot is rewritten only after its previous upload completed
     BHIP_TRY(b, hipEventSynchronize(b->ring_ev[slot]));
     for (int m = 0; m < n; ++m) {
@@ -1692,9 +1832,9 @@ int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, voi
         if (q_out[m] == q_in[m])
             return bfail(b, RV_E_ARG, "q_out must not alias q_in");
         StageParams P = stage_params(b->members[m]);
-        P.q_in = q_in[m];
-        P.q_out = q_out[m];
-        P.dq = dq[m];
+        P.q_in = as_global(q_in[m]);
+        P.q_out = as_global(q_out[m]);
+        P.dq = as_global(dq[m]);
         P.a = a[m];
         P.b_dt = b_dt[m];
         P.read_dq = a[m] != 0.0 ? 1 : 0;
@@ -1705,7 +1845,55 @@ int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, voi
                                b->stream));
     BHIP_TRY(b, hipEventRecord(b->ring_ev[slot], b->stream));
     b->ring = (slot + 1) % rv_batch::RING;
-    return batch_launch(b, dev, b->stream);
+    return batch_launch(b, dev, 1, b->stream);
+}
+
+int rv_batch_step(rv_batch *b, void *const *q0, void *const *q1, void *const *dq,
+                  const int32_t *cur, const double *dt, int scheme)
+{
+    if (!b || !q0 || !q1 || !dq || !cur || !dt)
+        return bfail(b, RV_E_ARG, "null argument");
+    const double *A, *B, *C;
+    const int nst = lsrk_tables(scheme, &A, &B, &C);
+    if (nst == 0 || nst > rv_batch::MAXST)
+        return bfail(b, RV_E_ARG, "unknown LSRK scheme");
+    (void)C;
+    const int n = (int)b->members.size();
+    const int slot = b->ring;
+    StageParams *host = b->host_ring + (size_t)slot * n * rv_batch::MAXST;
+    StageParams *dev = b->dev_ring + (size_t)slot * n * rv_batch::MAXST;
+    BHIP_TRY(b, hipEventSynchronize(b->ring_ev[slot]));
+    std::vector<char> active(n);
+    for (int m = 0; m < n; ++m) {
+        rv_handle *h = b->members[m];
+        const bool f32 = h->dtype == RV_F32;
+        const bool on = dt[m] != 0.0;
+        active[m] = on;
+        if (on && (!q0[m] || !q1[m] || !dq[m] || q0[m] == q1[m] || (cur[m] & ~1)))
+            return bfail(b, RV_E_ARG, "invalid state pointers or parity");
+        StageParams base = stage_params(h);
+        if (!on)
+            base.K = 0;  // the member's items do nothing this step
+        for (int s = 0; s < nst; ++s) {
+            StageParams P = base;
+            void *bufs[2] = {q0[m], q1[m]};
+            const int c = (cur[m] + s) & 1;
+            P.q_in = as_global(bufs[c]);
+            P.q_out = as_global(bufs[1 - c]);
+            P.dq = as_global(dq[m]);
+            // LSRK{FT}: coefficients and dt in the state type (odesolvers.jl:41-48)
+            P.a = f32 ? (double)(float)A[s] : A[s];
+            P.b_dt = f32 ? (double)((float)B[s] * (float)dt[m]) : B[s] * dt[m];
+            P.read_dq = P.a != 0.0 ? 1 : 0;
+            P.write_q = 1;
+            host[(size_t)s * n + m] = P;
+        }
+    }
+    BHIP_TRY(b, hipMemcpyAsync(dev, host, sizeof(StageParams) * n * nst, hipMemcpyHostToDevice,
+                               b->stream));
+    BHIP_TRY(b, hipEventRecord(b->ring_ev[slot], b->stream));
+    b->ring = (slot + 1) % rv_batch::RING;
+    return batch_launch(b, dev, nst, b->stream, &active);
 }
 
 int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, const double *dt,
@@ -1742,6 +1930,8 @@ int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, 
         key.push_back(reinterpret_cast<const void *>((intptr_t)bits));
     }
     const int parities = (nst % 2) ? 2 : 1;
+    if (!b->capture)
+        BHIP_TRY(b, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
     if (key != b->graph_key) {
         drop_graphs(b);
         if (b->dev_tables) {
@@ -1757,9 +1947,9 @@ int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, 
                     void *bufs[2] = {q[m], b->qtmp[m]};
                     const int cur = (p + s) % 2;
                     StageParams P = stage_params(h);
-                    P.q_in = bufs[cur];
-                    P.q_out = bufs[1 - cur];
-                    P.dq = dq[m];
+                    P.q_in = as_global(bufs[cur]);
+                    P.q_out = as_global(bufs[1 - cur]);
+                    P.dq = as_global(dq[m]);
                     // coefficients rounded to the state type as in rv_lsrk_run
                     P.a = f32 ? (double)(float)A[s] : A[s];
                     P.b_dt = f32 ? (double)((float)B[s] * (float)dt[m]) : B[s] * dt[m];
@@ -1773,9 +1963,7 @@ int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, 
         for (int p = 0; p < parities; ++p) {
             hipGraph_t g = nullptr;
             BHIP_TRY(b, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-            int rc = RV_OK;
-            for (int s = 0; s < nst && rc == RV_OK; ++s)
-                rc = batch_launch(b, b->dev_tables + ((size_t)p * nst + s) * n, b->capture);
+            int rc = batch_launch(b, b->dev_tables + (size_t)p * nst * n, nst, b->capture);
             hipError_t ec = hipStreamEndCapture(b->capture, &g);
             if (rc != RV_OK) {
                 if (g)

@@ -1,5 +1,7 @@
 // Batched fused LSRK stages: many independent operators (different Nq, law,
-// precision and cell count) advanced by ONE persistent launch per stage.
+// precision and cell count) advanced by one persistent launch per stage (2-D)
+// or one launch per kind present, each with that kind's own LDS slab and the
+// single operator's launch shape (3-D).
 //
 // The work list holds one item per wave's worth of cells of one member
 // (member, kind, first cell); items are grouped by kind so that the waves
@@ -11,6 +13,8 @@
 // flux (the brick solves batched and served here); 2-D members take Nq 2..8,
 // 3-D members Nq 4..8.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "common.hpp"
 #include "dg_plane3.hpp"
@@ -26,6 +30,26 @@ struct BatchRange {
     static constexpr int NQ_MIN = DIM == 3 ? 4 : 2;
     static constexpr int NQ_MAX = 8;
 };
+
+// LDS of one wave for a kind: the single-operator kernel's slab, in elements
+template <int DIM, int NQ, int LAW, typename R>
+constexpr int kind_slab_elems()
+{
+    if constexpr (DIM == 3) {
+        using T = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
+        return T::EPW * T::SLAB;
+    } else {
+        using S = Stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
+        return S::EPW * S::NS * S::NP;
+    }
+}
+
+// ... and in bytes, rounded up to 16 (the mixed kernel's slab stride)
+template <int DIM, int NQ, int LAW, typename R>
+constexpr int kind_slab_bytes()
+{
+    return (int)((kind_slab_elems<DIM, NQ, LAW, R>() * sizeof(R) + 15) / 16 * 16);
+}
 
 template <int DIM, int NQ, int LAW, typename R>
 __device__ __attribute__((always_inline)) inline void run_item(const StageParams &P, int64_t cell0, unsigned char *wl)
@@ -70,7 +94,12 @@ dg_batch_kernel(const StageParams *__restrict__ members, const BatchItem *__rest
     extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
     const int wave = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    unsigned char *wl = dg_lds + (size_t)wave * slab_bytes;
+    // slab_bytes is a multiple of 16: 16-byte LDS accesses stay available
+    unsigned char *wl = static_cast<unsigned char *>(
+        __builtin_assume_aligned(dg_lds + (size_t)wave * slab_bytes, 16));
+    // consecutive waves take consecutive items, so the waves running at
+    // the same time share kinds and neighbouring cells, and every XCD sees
+    // the whole mix of kinds (cheap and costly items alike)
     const int64_t first = wave_uniform(xcd_remap(blockIdx.x, gridDim.x) * wpb + wave);
     for (int64_t it = first; it < nitems; it += nwaves) {
         const BatchItem w = items[it];
@@ -80,18 +109,81 @@ dg_batch_kernel(const StageParams *__restrict__ members, const BatchItem *__rest
     }
 }
 
+// One kind only: a wave per item of the range [first, first + count), the
+// same code and register allocation as the single-operator kernel.  3-D
+// batches launch one of these per kind present.
+template <int DIM, int NQ, int LAW, typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+dg_batch_kind_kernel(const StageParams *__restrict__ members, const BatchItem *__restrict__ items,
+                     int64_t first, int64_t count, int slab_bytes)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
+    const int wave = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int64_t k = wave_uniform(xcd_remap(blockIdx.x, gridDim.x) * wpb + wave);
+    if (k >= count)
+        return;
+    const BatchItem w = items[first + k];
+    // the kind's own slab at the single-operator kernel's offset: the same
+    // known alignment gives the same LDS access widths (16-byte reads where
+    // that kernel has them; for FP32 those measured slower)
+    (void)slab_bytes;
+    run_item<DIM, NQ, LAW, R>(members[w.member], w.cell0,
+                              reinterpret_cast<unsigned char *>(
+                                  reinterpret_cast<R *>(dg_lds) +
+                                  wave * kind_slab_elems<DIM, NQ, LAW, R>()));
+}
+
+template <int DIM, int NQ>
+int launch_kind_nq(int sub, const StageParams *members, const BatchItem *items, int64_t first,
+                   int64_t count, int slab_bytes, int wpb, hipStream_t st)
+{
+    const int64_t blocks = (count + wpb - 1) / wpb;
+    const size_t lds = (size_t)wpb * slab_bytes;
+    const dim3 g((unsigned)blocks), b(64 * wpb);
+    switch (sub) {
+    case 0:
+        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_ADVECTION, double>), g, b, lds, st,
+                           members, items, first, count, slab_bytes);
+        break;
+    case 1:
+        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_ADVECTION, float>), g, b, lds, st,
+                           members, items, first, count, slab_bytes);
+        break;
+    case 2:
+        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_EULER, double>), g, b, lds, st,
+                           members, items, first, count, slab_bytes);
+        break;
+    default:
+        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_EULER, float>), g, b, lds, st,
+                           members, items, first, count, slab_bytes);
+        break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int DIM, int NQ>
+int launch_kind(int kind, const StageParams *members, const BatchItem *items, int64_t first,
+                int64_t count, int slab_bytes, int wpb, hipStream_t st)
+{
+    if constexpr (NQ > BatchRange<DIM>::NQ_MAX) {
+        return -1;
+    } else {
+        if (kind / KINDS_PER_NQ == NQ - BatchRange<DIM>::NQ_MIN)
+            return launch_kind_nq<DIM, NQ>(kind % KINDS_PER_NQ, members, items, first, count,
+                                           slab_bytes, wpb, st);
+        return launch_kind<DIM, NQ + 1>(kind, members, items, first, count, slab_bytes, wpb, st);
+    }
+}
+
 template <int DIM, int NQ, int LAW, typename R>
 void layout_of(int *epw, int *slab_bytes)
 {
-    if constexpr (DIM == 3) {
-        using T = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
-        *epw = T::EPW;
-        *slab_bytes = (int)(T::EPW * T::SLAB * sizeof(R));
-    } else {
-        using S = Stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
-        *epw = S::EPW;
-        *slab_bytes = (int)(S::EPW * S::NS * S::NP * sizeof(R));
-    }
+    if constexpr (DIM == 3)
+        *epw = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
+    else
+        *epw = Stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
+    *slab_bytes = kind_slab_bytes<DIM, NQ, LAW, R>();
 }
 
 template <int DIM, int NQ>
@@ -169,6 +261,19 @@ int launch_batch(int dim, const StageParams *members, const BatchItem *items, in
         hipLaunchKernelGGL(dg_batch_kernel<2>, dim3((unsigned)blocks), dim3(64 * wpb), lds, st,
                            members, items, nitems, launched, slab_bytes);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_batch_kind(int dim, int kind, const StageParams *members, const BatchItem *items,
+                      int64_t first, int64_t count, int slab_bytes, int wpb, void *stream)
+{
+    if (count <= 0)
+        return 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (dim == 3)
+        return launch_kind<3, BatchRange<3>::NQ_MIN>(kind, members, items, first, count,
+                                                      slab_bytes, wpb, st);
+    return launch_kind<2, BatchRange<2>::NQ_MIN>(kind, members, items, first, count, slab_bytes,
+                                                  wpb, st);
 }
 
 }  // namespace rv

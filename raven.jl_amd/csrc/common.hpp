@@ -24,31 +24,49 @@ struct FaceRec {
     int32_t y;
 };
 
+// Device code reads the arrays below through global-memory (address space
+// 1) pointers, also when a StageParams comes out of a device table (batches):
+// a pointer loaded from memory is otherwise a generic one, compiled to flat
+// loads that also count against the LDS wait counter.  The host pass sees
+// plain pointers; the layout is the same.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RV_GLOBAL __attribute__((address_space(1)))
+#else
+#define RV_GLOBAL
+#endif
+
+// A host-side device pointer as the parameter structs hold it.
+template <typename T>
+inline T RV_GLOBAL *as_global(T *p)
+{
+    return (T RV_GLOBAL *)p;
+}
+
 // Arguments of one fused DG stage launch.  Pointers are device pointers.
 struct StageParams {
     int64_t K;            // cells
-    const void *q_in;     // state read (R)
-    void *q_out;          // state written (R), nullptr in RHS mode
-    void *dq;             // register (R)
+    const void RV_GLOBAL *q_in;  // state read (R)
+    void RV_GLOBAL *q_out;       // state written (R), nullptr in RHS mode
+    void RV_GLOBAL *dq;          // register (R)
     double a;             // dq = a*dq + L
     double b_dt;          // q_out = q_in + b_dt*dq
     int read_dq;          // load dq (a != 0 or increment)
     int write_q;          // stage mode
     int dir_mask;         // active directions (bit d)
     int reserved;
-    const FaceRec *faces;        // (2dim, K)
-    const int64_t *explicit_idx; // explicit face-node offsets
+    const FaceRec RV_GLOBAL *faces;        // (2dim, K)
+    const int64_t RV_GLOBAL *explicit_idx; // explicit face-node offsets
     // affine geometry: per cell, per direction: dRdX[d,d] (FP64)
-    const double *affine;        // (K, dim)
+    const double RV_GLOBAL *affine;        // (K, dim)
     // general geometry (FP64): wJ*dRdX rows, MJI, face normals and wsJ
-    const double *vgeo;          // (K, dim, dim, Np): [e][d][c][node] = wJ*dRdX[d,c]
-    const double *mji;           // (K, Np)
-    const double *mj;            // (K, Np) = wJ
-    const double *sgeo;          // (K, 2dim, dim+1, Nfp): n_c ..., wsJ
+    const double RV_GLOBAL *vgeo;  // (K, dim, dim, Np): [e][d][c][node] = wJ*dRdX[d,c]
+    const double RV_GLOBAL *mji;   // (K, Np)
+    const double RV_GLOBAL *mj;    // (K, Np) = wJ
+    const double RV_GLOBAL *sgeo;  // (K, 2dim, dim+1, Nfp): n_c ..., wsJ
     // operators (FP64, device)
-    const double *Dt;            // [i*Nq + n] = D[n,i]            (general)
-    const double *Dw;            // [i*Nq + n] = D[n,i] w_n / w_i  (affine)
-    const double *winv;          // [i] = 1/w_i
+    const double RV_GLOBAL *Dt;    // [i*Nq + n] = D[n,i]            (general)
+    const double RV_GLOBAL *Dw;    // [i*Nq + n] = D[n,i] w_n / w_i  (affine)
+    const double RV_GLOBAL *winv;  // [i] = 1/w_i
     double gamma;
     double u[3];
 };
@@ -65,11 +83,11 @@ struct KernelKey {
 struct MortarParams {
     StageParams stage;       // the stage just launched (states, law, mji)
     int64_t ncells;          // cells with non-conforming faces
-    const int64_t *cells;
-    const int32_t *kind;     // (K, 2dim): 0 conforming, 1 hanging, 2 covered
-    const int64_t *partner;  // (K, 2dim, 4), see forest_mortars
-    const double *sgeo;      // surface geometry with the lift weights kept
-    const double *halves;    // tohalves_1d, (2, Nq, Nq) row-major
+    const int64_t RV_GLOBAL *cells;
+    const int32_t RV_GLOBAL *kind;     // (K, 2dim): 0 conforming, 1 hanging, 2 covered
+    const int64_t RV_GLOBAL *partner;  // (K, 2dim, 4), see forest_mortars
+    const double RV_GLOBAL *sgeo;      // surface geometry with the lift weights kept
+    const double RV_GLOBAL *halves;    // tohalves_1d, (2, Nq, Nq) row-major
 };
 using MortarLauncher = int (*)(const MortarParams &, void *stream);
 MortarLauncher find_mortar_launcher(int dim, int nq, int law, int real_bytes, int flux);
@@ -90,6 +108,9 @@ int batch_resident_waves(int dim, int wpb, int slab_bytes, int64_t *waves);
 // one persistent launch over the work list (members: device table)
 int launch_batch(int dim, const StageParams *members, const BatchItem *items, int64_t nitems,
                  int64_t nwaves, int slab_bytes, int wpb, void *stream);
+// one launch for the items [first, first + count) of a single kind
+int launch_batch_kind(int dim, int kind, const StageParams *members, const BatchItem *items,
+                      int64_t first, int64_t count, int slab_bytes, int wpb, void *stream);
 
 // Host-side grid generation (host_grid.cpp): Raven's brick GridManager +
 // generate restated for uniform bricks, and the LobattoCell 1-D operators.

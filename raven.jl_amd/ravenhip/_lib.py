@@ -144,6 +144,8 @@ def lib():
         "rv_batch_info": (ctypes.c_int, [H, c_i64p, c_i64p, c_i32p]),
         "rv_batch_stage": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(vp), c_f64p, c_f64p]),
+        "rv_batch_step": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                         ctypes.POINTER(vp), c_i32p, c_f64p, ctypes.c_int]),
         "rv_batch_lsrk_run": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                              ctypes.c_int, c_f64p, ctypes.c_int64]),
     }

@@ -77,3 +77,21 @@ class Batch:
         L.check_batch(L.lib().rv_batch_lsrk_run(self._b, _ptrs(q), _ptrs(dq), int(scheme),
                                                 L.ptr(dv, ctypes.c_double), int(nsteps)),
                       self._b)
+
+    def step(self, q0, q1, dq, cur, dt, scheme=L.RV_LSRK54):
+        """One whole LSRK step of every member with its own dt, ping-ponging
+        between q0[m] and q1[m] (read from q1[m] when cur[m] is 1); members with
+        dt[m] == 0 are left untouched."""
+        n = len(self.dgs)
+        dv = np.ascontiguousarray(dt, dtype=np.float64)
+        cv = np.ascontiguousarray(cur, dtype=np.int32)
+        if dv.size != n or cv.size != n:
+            raise ValueError("one dt and one parity per member")
+        on = [k for k in range(n) if dv[k] != 0.0]
+        for k in on:
+            for t in (q0[k], q1[k], dq[k]):
+                self.dgs[k]._check_state(t)
+        ptrs = [(ctypes.c_void_p * n)(*[t.data_ptr() if t is not None else None for t in ts])
+                for ts in (q0, q1, dq)]
+        L.check_batch(L.lib().rv_batch_step(self._b, *ptrs, L.ptr(cv, ctypes.c_int32),
+                                            L.ptr(dv, ctypes.c_double), int(scheme)), self._b)

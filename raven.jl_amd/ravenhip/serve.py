@@ -4,12 +4,12 @@ A request is one brick solve (2-D or 3-D, advection or Euler, degree 3-7,
 FP32 or FP64) to a final time T.  The server keeps every admitted solve's
 state resident in device memory, admits waiting requests in arrival order as
 soon as their device footprint fits the budget, and advances all admitted
-solves of one dimension together: each LSRK54 step is five batched stage
-launches (rv_batch_stage) over the current mix, with every member's own dt and
-its final step shortened to land on its T exactly as ``solve!`` does
-(odesolvers.jl:9-30).  Requests that arrive while others run join at the next
-step; finished ones leave and free their memory.  Each member's stages are
-the ones it would get alone, so its result does not depend on the mix.
+solves of one dimension together: each LSRK54 step is one rv_batch_step call
+over the current mix, with every member's own dt and its final step shortened
+to land on its T exactly as ``solve!`` does (odesolvers.jl:9-30).  Requests
+that arrive while others run join at the next step; finished ones leave and
+free their memory.  Each member's stages are the ones it would get alone, so
+its result does not depend on the mix.
 
 ``run_plain`` serves the same stream one request at a time through the plain
 per-operator path (rv_lsrk_run) for comparison.
@@ -78,8 +78,9 @@ class SolveRequest:
 
     def footprint(self):
         """Device bytes while resident: q, dq and the ping-pong copy, plus the
-        operator's face records and per-cell geometry."""
-        return 3 * self.state_bytes() + self.cells * (6 * 8 + 3 * 8) + (1 << 16)
+        operator's FP64 MJ, face records and per-cell geometry."""
+        return (3 * self.state_bytes() + self.cells * self.nq ** self.dim * 8
+                + self.cells * (6 * 8 + 4 * 8) + (1 << 16))
 
 
 def make_operator(req, stream=None):
@@ -184,95 +185,195 @@ class _Active:
         self.done = last
 
 
+class _Group:
+    """The resident solves of one dimension and their batch.  Finished solves
+    stay in the batch with step 0 until the next admission rebuilds it."""
+
+    def __init__(self, members):
+        import ctypes
+        self.members = list(members)
+        self.batch = Batch([m.dg for m in self.members])
+        n = len(self.members)
+        self.cur = np.zeros(n, dtype=np.int32)
+        self.dt = np.zeros(n, dtype=np.float64)
+        vp = ctypes.c_void_p * n
+        self.q0 = vp(*[m.bufs[0].data_ptr() for m in self.members])
+        self.q1 = vp(*[m.bufs[1].data_ptr() for m in self.members])
+        self.dq = vp(*[m.dq.data_ptr() for m in self.members])
+
+    def step(self, scheme, nstages):
+        """One LSRK step of every unfinished member: one C call, one launch
+        per stage.  Returns the members that finished with it."""
+        plan = []
+        for k, m in enumerate(self.members):
+            if m.done:
+                self.dt[k] = 0.0
+                plan.append(None)
+                continue
+            st, last = m.next_step()
+            self.dt[k] = st
+            self.cur[k] = m.cur
+            plan.append((st, last))
+        L.check_batch(L.lib().rv_batch_step(self.batch._b, self.q0, self.q1, self.dq,
+                                            L.ptr(self.cur, L.ctypes.c_int32),
+                                            L.ptr(self.dt, L.ctypes.c_double), scheme),
+                      self.batch._b)
+        finished = []
+        for m, p in zip(self.members, plan):
+            if p is None:
+                continue
+            st, last = p
+            m.cur = (m.cur + nstages) % 2
+            m.advance_clock(st, last)
+            m.req.steps += 1
+            if last:
+                finished.append(m)
+        return finished
+
+
 class Server:
     """Continuous-batching server over one device.
 
-    hbm_budget -- device bytes the resident solves may hold together
+    hbm_budget  -- device bytes the resident solves may hold together
+    admit_slice -- host seconds one pass of the loop may spend admitting
+                   (at least one request is admitted when none is resident)
+
+    Solves step on a stream of the server's own; operator set-up and initial
+    states are prepared on the default stream, which does not wait for it, so
+    admission overlaps the device's work on the solves already running.  Each
+    pass of the loop admits arrived requests in arrival order while their
+    footprint fits (for at most admit_slice seconds), then issues one LSRK step
+    of every resident solve: one rv_batch_step call per dimension present.  The
+    host never waits for the device between steps (the batch's parameter ring
+    keeps it at most a few steps ahead); a solve's completion is seen through an
+    event recorded after its last step, and its states go back to the
+    allocator then.  Operators and batches that are no longer used are
+    destroyed when the server is idle or done, since freeing device memory
+    waits for the device.
     """
 
-    def __init__(self, hbm_budget, scheme=L.RV_LSRK54):
+    def __init__(self, hbm_budget, scheme=L.RV_LSRK54, admit_slice=0.004):
         from .odesolvers import LSRK
         self.hbm_budget = int(hbm_budget)
+        self.scheme = scheme
+        self.admit_slice = float(admit_slice)
 
         class _NoDG:
             def GridArray(self, fill):
                 return None
         a, b, _ = LSRK(_NoDG(), None, 1.0, 0.0, scheme).coefficients()
         self.a, self.b = a, b
-        self._batches = {}
         self.peak_bytes = 0       # largest footprint held at once
         self.peak_active = 0      # most solves resident at once
         self.steps_run = 0        # batched steps (one per dimension present)
-
-    def _batch(self, dim, members):
-        """The batch of the current mix of one dimension (rebuilt when the
-        mix changes)."""
-        key = tuple(m.key for m in members)
-        cached = self._batches.get(dim)
-        if cached is None or cached[0] != key:
-            self._batches[dim] = (key, Batch([m.dg for m in members]))
-        return self._batches[dim][1]
-
-    def _step(self, dim, members):
-        """One LSRK step of every member: len(a) batched stage launches."""
-        bt = self._batch(dim, members)
-        plan = [m.next_step() for m in members]
-        for s in range(len(self.a)):
-            q_in = [m.bufs[m.cur] for m in members]
-            q_out = [m.bufs[1 - m.cur] for m in members]
-            dq = [m.dq for m in members]
-            a = [m.a[s] for m in members]
-            bdt = [m.bdt(s, st) for m, (st, _) in zip(members, plan)]
-            bt.stage(q_out, q_in, dq, a, bdt)
-            for m in members:
-                m.cur = 1 - m.cur
-        for m, (st, last) in zip(members, plan):
-            m.advance_clock(st, last)
-            m.req.steps += 1
+        # host seconds spent admitting (operator set-up, initial state, batch
+        # rebuilds), issuing steps, waiting for completions, and idle
+        self.host_time = {"admit": 0.0, "step": 0.0, "wait": 0.0, "idle": 0.0}
 
     def run(self, requests, keep_results=False, stream=None):
         """Serve the requests (arrival times relative to the call); returns
-        the requests with admitted / finished times filled in."""
+        the requests with admitted / finished times filled in.  stream: the
+        stream the solves step on (default: a new stream)."""
         import torch
+        compute = stream if stream is not None else torch.cuda.Stream()
+        setup = torch.cuda.current_stream()
+        handle = L.ctypes.c_void_p(compute.cuda_stream)
         pending = sorted(requests, key=lambda r: r.arrival)
-        waiting, active = [], []
+        waiting = []
+        groups = {}               # dim -> _Group of the solves still stepping
+        stepping = []             # admitted solves whose last step is not issued
+        inflight = []             # (solve, event after its last step)
+        graveyard = []            # operators and batches to destroy when idle
         used = 0
         t0 = time.perf_counter()
         k = 0
-        while k < len(pending) or waiting or active:
+        clock = self.host_time
+
+        def bury():
+            # destroying waits for the device: only when it has nothing queued
+            for obj in graveyard:
+                if isinstance(obj, _Group):
+                    obj.batch.close()
+                else:
+                    obj.dg.close()
+            graveyard.clear()
+
+        while k < len(pending) or waiting or stepping or inflight:
             now = time.perf_counter() - t0
             while k < len(pending) and pending[k].arrival <= now:
                 waiting.append(pending[k])
                 k += 1
+            # completions seen so far
+            still = []
+            for m, ev in inflight:
+                if ev.query():
+                    m.req.finished = time.perf_counter() - t0
+                    m.req.result = m.bufs[m.cur] if keep_results else None
+                    used -= m.req.footprint()
+                    m.bufs, m.dq = None, None
+                    graveyard.append(m)
+                else:
+                    still.append((m, ev))
+            inflight = still
             # admission in arrival order while the footprint fits
-            while waiting and (used + waiting[0].footprint() <= self.hbm_budget or not active):
+            ta = time.perf_counter()
+            admitted = []
+            while waiting and (used + waiting[0].footprint() <= self.hbm_budget
+                               or not (stepping or inflight)):
+                if admitted and time.perf_counter() - ta > self.admit_slice:
+                    break
+                if not (stepping or inflight) and graveyard:
+                    bury()
                 r = waiting.pop(0)
                 r.admitted = time.perf_counter() - t0
                 used += r.footprint()
-                active.append(_Active(r, self.a, self.b, stream))
+                with torch.cuda.stream(setup):
+                    m = _Active(r, self.a, self.b, handle)
+                stepping.append(m)
+                admitted.append(m)
                 self.peak_bytes = max(self.peak_bytes, used)
-                self.peak_active = max(self.peak_active, len(active))
-            if not active:
-                if k < len(pending):
-                    time.sleep(max(0.0, min(pending[k].arrival - now, 0.01)))
-                continue
-            for dim in (2, 3):
-                group = [m for m in active if m.req.dim == dim]
-                if group:
-                    self._step(dim, group)
+                self.peak_active = max(self.peak_active, len(stepping) + len(inflight))
+            if admitted:
+                # the first steps wait for the initial states
+                compute.wait_stream(setup)
+                for m in admitted:
+                    for t in m.bufs + [m.dq]:
+                        t.record_stream(compute)
+                for dim in {m.req.dim for m in admitted}:
+                    if dim in groups:
+                        graveyard.append(groups[dim])
+                    groups[dim] = _Group([m for m in stepping if m.req.dim == dim])
+            tb = time.perf_counter()
+            clock["admit"] += tb - ta
+            # one step of every solve still stepping
+            issued = False
+            for dim in sorted(groups):
+                g = groups[dim]
+                if any(not m.done for m in g.members):
+                    for m in g.step(self.scheme, len(self.a)):
+                        ev = torch.cuda.Event()
+                        ev.record(compute)
+                        inflight.append((m, ev))
                     self.steps_run += 1
-            torch.cuda.synchronize()
-            now = time.perf_counter() - t0
-            still = []
-            for m in active:
-                if m.done:
-                    m.req.finished = now
-                    m.req.result = m.bufs[m.cur] if keep_results else None
-                    used -= m.req.footprint()
-                else:
-                    still.append(m)
-            active = still
-        self._batches.clear()
+                    issued = True
+            stepping = [m for m in stepping if not m.done]
+            for dim in [d for d, g in groups.items() if all(m.done for m in g.members)]:
+                graveyard.append(groups.pop(dim))
+            tc = time.perf_counter()
+            clock["step"] += tc - tb
+            if issued:
+                continue
+            if inflight:
+                # nothing to issue: wait for the earliest completion, or
+                # until the next arrival could be admitted
+                inflight[0][1].synchronize()
+                clock["wait"] += time.perf_counter() - tc
+            elif k < len(pending) and not waiting:
+                bury()
+                time.sleep(max(0.0, min(pending[k].arrival - (time.perf_counter() - t0), 0.01)))
+                clock["idle"] += time.perf_counter() - tc
+        compute.synchronize()
+        bury()
         return requests
 
 

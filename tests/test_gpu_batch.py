@@ -59,9 +59,18 @@ MIX2 = [(3, 6, "advection", "f64"), (4, 5, "euler", "f64"), (5, 7, "advection", 
         (2, 9, "euler", "f32"), (5, 8, "euler", "f64")]
 MIX3 = [(4, 3, "euler", "f64"), (5, 2, "advection", "f32"), (6, 2, "euler", "f32"),
         (8, 2, "euler", "f64"), (7, 3, "advection", "f64")]
+# kinds with enough items to launch on their own (two members of one of them)
+# beside small kinds that share the mixed launch
+# (a 2-D wave holds 64 / Nq cells: 96^2 and 92^2 cells at Nq = 8 give 1152 and
+# 1058 items, 130^2 at Nq = 4 gives 1057; 1024 items launch on their own)
+MIX2H = [(8, 96, "euler", "f64"), (3, 6, "advection", "f64"), (8, 92, "euler", "f64"),
+         (4, 130, "advection", "f32"), (5, 7, "advection", "f32")]
+# several members of one 3-D kind
+MIX3H = [(4, 3, "euler", "f64"), (5, 2, "advection", "f32"), (4, 4, "euler", "f64"),
+         (5, 3, "advection", "f32")]
 
 
-@pytest.mark.parametrize("dim,mix", [(2, MIX2), (3, MIX3)])
+@pytest.mark.parametrize("dim,mix", [(2, MIX2), (3, MIX3), (2, MIX2H), (3, MIX3H)])
 def test_batch_stage_equals_members_alone(torch, R, dim, mix):
     say(f"batch stage dim={dim} members={len(mix)}")
     ms = [member(R, torch, dim, nq, n, law, dt, seed) for seed, (nq, n, law, dt) in enumerate(mix)]
@@ -112,7 +121,7 @@ def test_batch_rhs_stage_matches_oracle(torch, R):
         assert err <= 1e-12
 
 
-@pytest.mark.parametrize("dim,mix", [(2, MIX2[:5]), (3, MIX3[:3])])
+@pytest.mark.parametrize("dim,mix", [(2, MIX2[:5]), (3, MIX3[:3]), (2, MIX2H), (3, MIX3H)])
 def test_batch_graph_run_equals_stage_loop(torch, R, dim, mix):
     """rv_batch_lsrk_run (a captured hipGraph per step, replayed) against each
     member stepping alone through its own stages; odd step counts exercise the
@@ -142,6 +151,39 @@ def test_batch_graph_run_equals_stage_loop(torch, R, dim, mix):
         for k in range(len(ms)):
             assert torch.equal(qs2[k], want[k]), f"member {k}, {nsteps} steps"
         batch.close()
+
+
+@pytest.mark.parametrize("dim,mix", [(2, MIX2H), (3, MIX3H)])
+def test_batch_step_skips_resting_members(torch, R, dim, mix):
+    """rv_batch_step with some members at dt = 0: those keep their states
+    bit for bit (their kernels are not launched, or return at once), the
+    others match two steps taken alone."""
+    ms = [member(R, torch, dim, nq, n, law, dt, 20 + s) for s, (nq, n, law, dt) in enumerate(mix)]
+    dgs = [m[0] for m in ms]
+    dts = [1.5e-3 * (1 + 0.5 * k) for k in range(len(ms))]
+    rest = {1, 2}  # one small and one large member (or two of one 3-D kind)
+    want = []
+    for k, (dg, q) in enumerate(zip(dgs, [m[1] for m in ms])):
+        qa = q.clone()
+        if k not in rest:
+            solver = R.LSRK54(dg, qa, dts[k])
+            tmp = torch.empty_like(qa)
+            for _ in range(2):
+                R.dostep_(qa, solver, tmp)
+        want.append(qa)
+    batch = R.Batch(dgs)
+    q0 = [m[1].clone() for m in ms]
+    q1 = [torch.empty_like(q) for q in q0]
+    dq = [torch.zeros_like(q) for q in q0]
+    cur = [0] * len(ms)
+    for _ in range(2):
+        batch.step(q0, q1, dq, cur, [0.0 if k in rest else dts[k] for k in range(len(ms))])
+        cur = [c if k in rest else (c + 5) % 2 for k, c in enumerate(cur)]
+    torch.cuda.synchronize()
+    for k in range(len(ms)):
+        got = (q0, q1)[cur[k]][k]
+        assert torch.equal(got, want[k]), f"member {k} ({'resting' if k in rest else 'stepping'})"
+    batch.close()
 
 
 def test_batch_rejects_unsupported_members(torch, R):

@@ -224,6 +224,30 @@ def test_face_maps_of_brick_handle_bitexact(torch, R):
     assert np.array_equal(vmapP, og.vmapP) and np.array_equal(mapP, og.mapP)
 
 
+def test_gatherscatter_of_brick_handle_bitexact(torch, R):
+    # the brick handle builds its continuous numbering on the first gather
+    cell = R.LobattoCell(4, 4, 4)
+    coords = [PB.julia_range(0, 3, 4), PB.julia_range(0, 2, 3), PB.julia_range(0, 2, 3)]
+    gm = R.GridManager(cell, R.brick(coords, (True, False, True)))
+    dg = R.DGSEM.brick(gm, R.EulerLaw(3))
+    og = O.BrickGrid(4, coords, (True, False, True))
+    nf = 5
+    rng = np.random.default_rng(11)
+    src = rng.standard_normal(og.Np * nf * og.K)
+    want = np.zeros(og.nc * nf)
+    O.lib().orc_gather(og.Np, nf, og.K, og.nc, og.dtoc, src, want)
+    d = torch.zeros(og.nc * nf, dtype=torch.float64, device="cuda")
+    dg.gather(d, torch.from_numpy(src).cuda(), nf)
+    c = rng.standard_normal(og.nc * nf)
+    want_s = np.zeros(og.Np * nf * og.K)
+    O.lib().orc_scatter(og.Np, nf, og.K, og.nc, og.dtoc, c, want_s)
+    out = torch.zeros(og.Np * nf * og.K, dtype=torch.float64, device="cuda")
+    dg.scatter(out, torch.from_numpy(c).cuda(), nf)
+    dg.synchronize()
+    assert np.array_equal(d.cpu().numpy(), want)
+    assert np.array_equal(out.cpu().numpy(), want_s)
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_gatherscatter_bitexact(torch, R, dtype):
     dim, nq = 3, 4

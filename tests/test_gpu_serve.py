@@ -32,7 +32,8 @@ def small_stream(S, arrivals):
 
 def test_results_same_alone_and_in_the_mix(torch, S):
     mixed = small_stream(S, [0.0] * 7)
-    server = S.Server(hbm_budget=8 << 30)
+    # admit the whole burst in one pass: all seven solves step together
+    server = S.Server(hbm_budget=8 << 30, admit_slice=60.0)
     server.run(mixed, keep_results=True)
     assert server.peak_active == 7
     alone = small_stream(S, [0.0] * 7)
