@@ -754,6 +754,8 @@ int create_from_desc(const rv_desc *d, rv_handle *h)
     return finish_setup(h, d->D, d->weights);
 }
 
+constexpr int64_t STREAM_BYTES = (int64_t)256 << 20;
+
 StageParams stage_params(rv_handle *h)
 {
     StageParams P{};
@@ -772,6 +774,11 @@ StageParams stage_params(rv_handle *h)
     P.gamma = h->gamma;
     for (int c = 0; c < 3; ++c)
         P.u[c] = h->u[c];
+    // a stage touches q_in, dq and q_out; beyond the 256 MB Infinity Cache of
+    // MI355X nothing of dq and q_out survives to the next stage, so they are
+    // streamed past L2 (which then keeps the q lines that face reads share)
+    const int64_t state = h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    P.streaming = 3 * state > STREAM_BYTES ? 1 : 0;
     return P;
 }
 

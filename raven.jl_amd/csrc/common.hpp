@@ -53,7 +53,7 @@ struct StageParams {
     int read_dq;          // load dq (a != 0 or increment)
     int write_q;          // stage mode
     int dir_mask;         // active directions (bit d)
-    int reserved;
+    int streaming;        // dq and q_out through L2 as streaming (non-temporal) accesses
     const FaceRec RV_GLOBAL *faces;        // (2dim, K)
     const int64_t RV_GLOBAL *explicit_idx; // explicit face-node offsets
     // affine geometry: per cell, per direction: dRdX[d,d] (FP64)

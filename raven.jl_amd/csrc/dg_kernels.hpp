@@ -252,6 +252,28 @@ struct Law {
 // --------------------------------------------------------------------------
 // The kernel.
 // --------------------------------------------------------------------------
+
+// Loads and stores of a cell's own dq and q_out: plain (ST = 0), or streaming
+// (ST = 1: non-temporal, no reuse expected in L2).  The choice is a kernel
+// variant picked by the launcher from StageParams::streaming: behind a
+// run-time branch the compiler merges the two accesses and drops the hint.
+template <int ST, typename T>
+__device__ __attribute__((always_inline)) inline T load_dq(const T *p)
+{
+    if constexpr (ST)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <int ST, typename T>
+__device__ __attribute__((always_inline)) inline void store_out(T v, T *p)
+{
+    if constexpr (ST)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
 template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>
 struct Stage {
     static constexpr int NS = LawInfo<DIM, LAW>::NS;
@@ -329,8 +351,9 @@ struct Stage {
         }
     }
 
-    // One direction sweep; directions meet in the cell's LDS slab.
-    template <int D>
+    // One direction sweep; directions meet in the cell's LDS slab.  ST
+    // selects streaming accesses for dq and q_out (load_dq / store_out).
+    template <int D, int ST>
     __device__ static inline void sweep(const StageParams &P, const Law<DIM, LAW, R> &law,
                                         bool active, int64_t e, int pen, R *slab)
     {
@@ -470,12 +493,15 @@ struct Stage {
                         tot = slab[s * NP + base + n * STRIDE] + tot;
                     if constexpr (GEO == GEO_GENERAL)
                         tot = R(P.mji[e * NP + base + n * STRIDE]) * tot;
+                    // dq and q_out belong to this cell alone: on operators
+                    // larger than the caches they are streamed, which keeps
+                    // L2 for the shared q lines
                     R v = tot;
                     if (P.read_dq)
-                        v = a * dq[s * NP + n * STRIDE] + tot;
-                    dq[s * NP + n * STRIDE] = v;
+                        v = a * load_dq<ST>(dq + s * NP + n * STRIDE) + tot;
+                    store_out<ST>(v, dq + s * NP + n * STRIDE);
                     if (P.write_q)
-                        qo[s * NP + n * STRIDE] = qe[s * NP + n * STRIDE] + bdt * v;
+                        store_out<ST>(R(qe[s * NP + n * STRIDE] + bdt * v), qo + s * NP + n * STRIDE);
                 }
             }
         }
@@ -485,7 +511,7 @@ struct Stage {
 // The cells [cell0, cell0 + EPW) of one wave (cell0 is the same on every
 // lane): the direction sweeps meet in the wave's own LDS slabs `wlds`
 // (EPW * Ns * Np values).
-template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>
+template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX, int ST = 0>
 __device__ __attribute__((always_inline)) inline void stage_wave(const StageParams &P, int64_t cell0, R *wlds)
 {
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX>;
@@ -498,21 +524,21 @@ __device__ __attribute__((always_inline)) inline void stage_wave(const StagePara
     R *slab = wlds + (slot < S::EPW ? slot : 0) * SLAB;
     const Law<DIM, LAW, R> law(P);
 
-    S::template sweep<0>(P, law, active, e, pen, slab);
+    S::template sweep<0, ST>(P, law, active, e, pen, slab);
     if constexpr (DIM > 1) {
         wave_lds_sync();
-        S::template sweep<1>(P, law, active, e, pen, slab);
+        S::template sweep<1, ST>(P, law, active, e, pen, slab);
     }
     if constexpr (DIM > 2) {
         wave_lds_sync();
-        S::template sweep<2>(P, law, active, e, pen, slab);
+        S::template sweep<2, ST>(P, law, active, e, pen, slab);
     }
 }
 
 // Waves per block (cells of one block share nothing; the knob trades LDS
 // granularity against scheduling) come from blockDim.x; the LDS slabs are
 // dynamic shared memory sized by the launcher.
-template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>
+template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX, int ST>
 __global__ void __launch_bounds__(256) dg_stage_kernel(const StageParams P)
 {
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX>;
@@ -522,7 +548,7 @@ __global__ void __launch_bounds__(256) dg_stage_kernel(const StageParams P)
     const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t cell0 = (blk * wpb + wave) * S::EPW;
     R *wlds = reinterpret_cast<R *>(dg_lds) + wave * S::EPW * S::NS * S::NP;
-    stage_wave<DIM, NQ, LAW, R, GEO, FLUX>(P, cell0, wlds);
+    stage_wave<DIM, NQ, LAW, R, GEO, FLUX, ST>(P, cell0, wlds);
 }
 
 template <int DIM, int NQ, int LAW, typename R, int GEO, int FLUX>
@@ -536,8 +562,12 @@ int launch_stage(const StageParams &P, void *stream, int wpb)
     const int64_t waves = (P.K + S::EPW - 1) / S::EPW;
     const int64_t blocks = (waves + wpb - 1) / wpb;
     const size_t lds = DIM > 1 ? (size_t)wpb * S::EPW * S::NS * S::NP * sizeof(R) : 16;
-    hipLaunchKernelGGL((dg_stage_kernel<DIM, NQ, LAW, R, GEO, FLUX>), dim3((unsigned)blocks),
-                       dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);
+    if (P.streaming)
+        hipLaunchKernelGGL((dg_stage_kernel<DIM, NQ, LAW, R, GEO, FLUX, 1>), dim3((unsigned)blocks),
+                           dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);
+    else
+        hipLaunchKernelGGL((dg_stage_kernel<DIM, NQ, LAW, R, GEO, FLUX, 0>), dim3((unsigned)blocks),
+                           dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
