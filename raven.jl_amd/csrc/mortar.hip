@@ -32,44 +32,73 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
     using S = Stage<DIM, NQ, LAW, R, GEO_GENERAL, FLUX>;
     constexpr int NS = S::NS, NP = S::NP, NFP = S::NP2, NF = 2 * DIM;
     constexpr int NSUB = 1 << (DIM - 1);
+    static_assert(NFP <= 64, "one face node per lane");
+    // tohalves in the state type ([half][a][b]), a face trace, the first pass
+    // of a separable 2-D interpolation, and the coarse side's weighted
+    // sub-face fluxes with the first pass of their projection
+    __shared__ R H[2][NQ][NQ];
     __shared__ R tr[NS][NFP];
+    __shared__ R t1[NS][NFP];
     __shared__ R g[NSUB][NS][NFP];
+    __shared__ R u[NSUB][NS][NFP];
     const StageParams &P = M.stage;
     const Law<DIM, LAW, R> law(P);
     const int64_t e = M.cells[blockIdx.x];
     const int m = threadIdx.x;
     const bool on = m < NFP;
+    const int a = m % NQ, b = m / NQ;  // face node m = a + NQ*b (b = 0 in 2-D)
     const R *qin = static_cast<const R *>(P.q_in);
     R *dq = static_cast<R *>(P.dq);
     R *qo = static_cast<R *>(P.q_out);
-    const double *H = M.halves;  // [half][a][b]
+    for (int t = m; t < 2 * NQ * NQ; t += 64)
+        (&H[0][0][0])[t] = R(M.halves[t]);
+    __syncthreads();
 
-    // interpolation of the trace in tr to node m of sub-face sub
+    // the trace in tr interpolated to node m of sub-face sub:
+    //   sum_jj H[s2][b][jj] (sum_i H[s1][a][i] tr(i, jj)),
+    // the inner sums computed once per (a, jj) by lane a + NQ*jj
     auto interp = [&](int sub, R *out) {
         const int s1 = sub & 1, s2 = (sub >> 1) & 1;
-        const int a = m % NQ, b = m / NQ;
+        if constexpr (DIM == 2) {
+            if (on) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            R v = R(0);
-            if constexpr (DIM == 2) {
-                for (int i = 0; i < NQ; ++i)
-                    v = v + R(H[(s1 * NQ + a) * NQ + i]) * tr[s][i];
-            } else {
-                for (int jj = 0; jj < NQ; ++jj) {
-                    R inner = R(0);
+                for (int s = 0; s < NS; ++s) {
+                    R v = R(0);
+#pragma unroll
                     for (int i = 0; i < NQ; ++i)
-                        inner = inner + R(H[(s1 * NQ + a) * NQ + i]) * tr[s][i + NQ * jj];
-                    v = v + R(H[(s2 * NQ + b) * NQ + jj]) * inner;
+                        v = v + H[s1][a][i] * tr[s][i];
+                    out[s] = v;
                 }
             }
-            out[s] = v;
+        } else {
+            if (on) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    R inner = R(0);
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i)
+                        inner = inner + H[s1][a][i] * tr[s][i + NQ * b];
+                    t1[s][m] = inner;
+                }
+            }
+            __syncthreads();
+            if (on) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    R v = R(0);
+#pragma unroll
+                    for (int jj = 0; jj < NQ; ++jj)
+                        v = v + H[s2][b][jj] * t1[s][a + NQ * jj];
+                    out[s] = v;
+                }
+            }
+            __syncthreads();  // t1 is reused by the next interpolation
         }
     };
     // sub-face flux as the fine cell sees it: q- = fine state, q+ = the
     // interpolated coarse trace, the fine face's normal and weight
-    auto subflux = [&](int64_t fine, int ffine, int sub, R *wnf) {
-        R qi[NS], qf[NS], nf[NS], n[DIM];
-        interp(sub, qi);
+    auto subflux = [&](int64_t fine, int ffine, const R *qi, R *wnf) {
+        R qf[NS], nf[NS], n[DIM];
         const int node = face_node<DIM, NQ>(ffine, m);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -79,7 +108,11 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
         for (int c = 0; c < DIM; ++c)
             n[c] = R(sg[c * NFP]);
         const R wsJ = R(sg[DIM * NFP]);
-        S::nflux_general(law, n, qf, qi, nf);
+        R qp[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            qp[s] = qi[s];
+        S::nflux_general(law, n, qf, qp, nf);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
             wnf[s] = wsJ * nf[s];
@@ -110,9 +143,11 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
                     tr[s][m] = qin[c * NS * NP + s * NP + cn];
             }
             __syncthreads();
+            R qi[NS];
+            interp(sub, qi);
             if (on) {
                 R wnf[NS], delta[NS];
-                subflux(e, f, sub, wnf);
+                subflux(e, f, qi, wnf);
                 const int node = face_node<DIM, NQ>(f, m);
                 const R mji = R(M.stage.mji[e * NP + node]);
 #pragma unroll
@@ -131,18 +166,39 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
                     tr[s][m] = qin[e * NS * NP + s * NP + node];
             }
             __syncthreads();
-            if (on) {
-                for (int sub = 0; sub < NSUB; ++sub) {
+            for (int sub = 0; sub < NSUB; ++sub) {
+                R qi[NS];
+                interp(sub, qi);
+                if (on) {
                     R wnf[NS];
-                    subflux(pt[sub], f ^ 1, sub, wnf);
+                    subflux(pt[sub], f ^ 1, qi, wnf);
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
                         g[sub][s][m] = wnf[s];
                 }
             }
             __syncthreads();
+            // projection: sum_sub sum_b H[s2][b][j] (sum_a H[s1][a][i] g(a, b)),
+            // the inner sums once per (i, b) by lane i + NQ*b
+            if constexpr (DIM == 3) {
+                if (on) {
+                    const int i = a, bb = b;
+                    for (int sub = 0; sub < NSUB; ++sub) {
+                        const int s1 = sub & 1;
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            R inner = R(0);
+#pragma unroll
+                            for (int aa = 0; aa < NQ; ++aa)
+                                inner = inner + H[s1][aa][i] * g[sub][s][aa + NQ * bb];
+                            u[sub][s][m] = inner;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
             if (on) {
-                const int i = m % NQ, j = m / NQ;
+                const int i = a, j = b;
                 R delta[NS];
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
@@ -150,16 +206,13 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
                     for (int sub = 0; sub < NSUB; ++sub) {
                         const int s1 = sub & 1, s2 = (sub >> 1) & 1;
                         if constexpr (DIM == 2) {
-                            for (int a = 0; a < NQ; ++a)
-                                v = v + R(H[(s1 * NQ + a) * NQ + i]) * g[sub][s][a];
+#pragma unroll
+                            for (int aa = 0; aa < NQ; ++aa)
+                                v = v + H[s1][aa][i] * g[sub][s][aa];
                         } else {
-                            for (int b = 0; b < NQ; ++b) {
-                                R inner = R(0);
-                                for (int a = 0; a < NQ; ++a)
-                                    inner = inner + R(H[(s1 * NQ + a) * NQ + i]) *
-                                                        g[sub][s][a + NQ * b];
-                                v = v + R(H[(s2 * NQ + b) * NQ + j]) * inner;
-                            }
+#pragma unroll
+                            for (int bb = 0; bb < NQ; ++bb)
+                                v = v + H[s2][bb][j] * u[sub][s][i + NQ * bb];
                         }
                     }
                     delta[s] = R(M.stage.mji[e * NP + node]) * v;
