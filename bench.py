@@ -26,8 +26,8 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 NQ, NCELL, L = 8, 32, 10.0
 GAMMA = 1.4
-KERNEL_NAME = {"f64": "rv::dg_plane3_kernel<8, Euler, double, affine, Rusanov>",
-               "f32": "rv::dg_plane3_kernel<8, Euler, float, affine, Rusanov>"}
+KERNEL_NAME = {"f64": "rv::dg_plane3_kernel<8, Euler, double, affine, Rusanov, streaming>",
+               "f32": "rv::dg_plane3_kernel<8, Euler, float, affine, Rusanov, streaming>"}
 # rocprofv3 counter summary of this workload's stage kernel (HBM bytes per
 # dispatch); read for roofline.traffic, which stays null without it
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "dg_plane3_kernel_euler3d_32cube_nq8_traffic.md")
@@ -456,6 +456,10 @@ def main():
             "kernel": KERNEL_NAME["f64"],
             "avg_kernel_ms": kavg * 1e3,
             "single_rhs_ms": r64["rhs_ms"],
+            # dg(dq, q, t; increment = false): read q, write dq
+            "single_rhs_GBs": 2 * r64["info"].state_bytes / (r64["rhs_ms"] * 1e-3) / 1e9,
+            "single_rhs_frac": 2 * r64["info"].state_bytes / (r64["rhs_ms"] * 1e-3) / 1e9
+            / HBM_PEAK_GBS,
             "algorithmic_bytes_per_launch": bytes_step / 5,
         },
     }
@@ -473,6 +477,7 @@ def main():
             "value": v32, "unit": "DOF-updates/s", "speedup_vs_fp64": v32 / value,
             "ms_per_step": r32["elapsed"] / args.steps * 1e3,
             "single_rhs_ms": r32["rhs_ms"],
+            "single_rhs_GBs": 2 * i32.state_bytes / (r32["rhs_ms"] * 1e-3) / 1e9,
             "avg_kernel_ms": k32 * 1e3, "achieved_GBs": b32 / k32 / 1e9,
             "frac": b32 / k32 / 1e9 / HBM_PEAK_GBS,
             "kernel": KERNEL_NAME["f32"], "traffic": measured_traffic("f32"),

@@ -75,9 +75,12 @@ class GridManager:
         self._forest = None
 
     def __del__(self):
-        if getattr(self, "_forest", None) is not None:
-            L.lib().rv_forest_destroy(self._forest)
-            self._forest = None
+        try:
+            if getattr(self, "_forest", None) is not None:
+                L.lib().rv_forest_destroy(self._forest)
+                self._forest = None
+        except Exception:
+            pass  # interpreter shutdown: the library may already be unloaded
 
     def _ensure_forest(self):
         if self._forest is None:
