@@ -199,9 +199,10 @@ def cpu_baseline(q0, dt, budget_s):
 def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     """256 independent 2-D advection / Euler solves (degree 3-7, n x n periodic
     cells with n uniform in 8..64, FP64 or FP32, seeded velocities and vortex
-    centres) advanced together: one persistent launch per stage, each LSRK54
-    step a replayed hipGraph.  Compared with the same solves run one after the
-    other through the plain per-operator path (rv_lsrk_run)."""
+    centres) advanced together: the launch layout picked by the batch
+    autotuner, each LSRK54 step a replayed hipGraph.  Compared with the same
+    solves run one after the other through the plain per-operator path
+    (rv_lsrk_run)."""
     import ravenhip.serve as S
     rng = np.random.default_rng(seed)
     reqs = [S.SolveRequest(rid=k, arrival=0.0, dim=2,
@@ -217,6 +218,10 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     batch = R.Batch(dgs)
     qs = [q.clone() for q in q0]
     dqs = [torch.empty_like(q) for q in q0]
+    # launch layout chosen by timing (each kind's waves per block, the mixed
+    # launch's waves per block and the own-launch threshold)
+    tuned = batch.autotune(qs, dts)
+    layout = min(tuned, key=tuned.get)
     batch.lsrk_run(qs, dqs, dts, steps)  # builds and caches the step graphs
     for q, q_init in zip(qs, q0):
         q.copy_(q_init)
@@ -256,7 +261,12 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
         "sequential_steps_match": nsteps == steps * n_inst,
         "finite": finite,
         "work_items": info["items"], "persistent_waves": info["waves"],
-        "launch": "one persistent work-list kernel per stage, one hipGraph per step",
+        "launch": "per stage: one launch per kind large enough to fill the device, one "
+                  "persistent work-list launch for the small kinds; one hipGraph per step",
+        "autotuned_layout": {"mixed_waves_per_block": layout[0],
+                             "own_launch_items": layout[1],
+                             "step_ms_by_layout": {f"{w},{t}": round(v, 4)
+                                                    for (w, t), v in tuned.items()}},
     }
 
 

@@ -219,9 +219,11 @@ int rv_forest_mortars(const rv_forest *f, int64_t *partner);
 
 /* --- batches of independent solves -------------------------------------- */
 
-/* Many operators advanced together: ONE persistent launch per LSRK stage walks
- * a work list of (member, first cell) items, grouped by (Nq, law, precision),
- * so solves of different degree, size and precision share each launch.  Members
+/* Many operators advanced together over a work list of (member, first cell)
+ * items grouped by (Nq, law, precision): kinds with enough items to fill the
+ * device (every 3-D kind) launch once per stage with the single operator's
+ * code, and the small 2-D kinds share ONE persistent launch, so solves of
+ * different degree, size and precision share each stage's launches.  Members
  * are handles from rv_create / rv_create_brick with the same dim and stream,
  * affine geometry, the Rusanov flux and all directions; 2-D members may have
  * Nq 2..8, 3-D members Nq 4..8.  The handles must outlive the batch.  Each
@@ -231,8 +233,15 @@ typedef struct rv_batch rv_batch;
 int rv_batch_create(rv_handle *const *members, int32_t nmembers, rv_batch **out);
 void rv_batch_destroy(rv_batch *b);
 const char *rv_batch_last_error(const rv_batch *b);
-/* work-list length, persistent waves launched, LDS bytes per wave */
+/* work-list length, persistent waves of the mixed launch, its LDS bytes per wave */
 int rv_batch_info(const rv_batch *b, int64_t *nitems, int64_t *nwaves, int32_t *slab_bytes);
+/* Launch layout (the autotuner's knobs): waves per block of the mixed launch
+ * (1, 2 or 4; default 2) and the item count from which a 2-D kind launches on
+ * its own (default 1024).  Kinds launching on their own take their first
+ * member's waves per block (rv_set_waves_per_block) as it is at this call.
+ * Waits for the batch's stream; cached step graphs are rebuilt on next use.
+ * Results do not depend on the layout. */
+int rv_batch_configure(rv_batch *b, int mixed_waves_per_block, int64_t own_launch_items);
 /* One fused stage for every member m (arrays of nmembers entries):
  *   dq[m] = a[m]*dq[m] + L_m(q_in[m]);  q_out[m] = q_in[m] + b_dt[m]*dq[m]. */
 int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, void *const *dq,

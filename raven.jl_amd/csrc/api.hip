@@ -1551,6 +1551,7 @@ struct rv_batch {
     // a kind with at least this many items (waves) fills the device on its
     // own: 256 CUs x 4 waves
     static constexpr int64_t OWN_LAUNCH_ITEMS = 1024;
+    int64_t own_items = OWN_LAUNCH_ITEMS;  // rv_batch_configure may change it
     std::string err;
     int dim = 0, wpb = 2, slab_bytes = 0;
     int64_t nitems = 0, nwaves = 0;
@@ -1585,6 +1586,7 @@ struct rv_batch {
     int tail_mbegin = 0;                   // position in `order` of its first member
     std::vector<int> order;                // members sorted by (small kind, kind)
     std::vector<int64_t> mfirst, mcount;   // each member's items
+    std::vector<int> kind, epw, slab;      // each member's kind, cells per item, LDS slab
 };
 
 namespace {
@@ -1637,6 +1639,70 @@ void free_batch(rv_batch *b)
         if (e_ != hipSuccess)                                                              \
             return bfail(b, RV_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
     } while (0)
+
+// The work list and launch layout of a batch from its members' kinds: members
+// sorted by (small kind, kind), each member's cells in order; kinds with at
+// least own_items items launch on their own, the rest form the mixed tail.
+void batch_layout(rv_batch *b, std::vector<BatchItem> &items)
+{
+    const int n = (int)b->members.size();
+    const std::vector<int> &kind = b->kind, &epw = b->epw, &slab = b->slab;
+    std::map<int, int64_t> kind_items;
+    for (int m = 0; m < n; ++m)
+        kind_items[kind[m]] += (b->members[m]->K + epw[m] - 1) / epw[m];
+    std::vector<char> small(n);
+    for (int m = 0; m < n; ++m)
+        small[m] = b->dim != 3 && kind_items[kind[m]] < b->own_items;
+    std::vector<int> &order = b->order;
+    order.resize(n);
+    for (int m = 0; m < n; ++m)
+        order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return small[x] != small[y] ? small[x] < small[y] : kind[x] < kind[y];
+    });
+    items.clear();
+    b->mfirst.assign(n, 0);
+    b->mcount.assign(n, 0);
+    b->tail_mbegin = n;
+    b->tail_first = -1;
+    b->slab_bytes = 16;
+    b->kind_ranges.clear();
+    for (int p = 0; p < n; ++p) {
+        const int m = order[p];
+        if (small[m] && b->tail_first < 0) {
+            b->tail_first = (int64_t)items.size();
+            b->tail_mbegin = p;
+        }
+        if (small[m])
+            b->slab_bytes = std::max(b->slab_bytes, slab[m]);
+        b->mfirst[m] = (int64_t)items.size();
+        for (int64_t c = 0; c < b->members[m]->K; c += epw[m])
+            items.push_back(BatchItem{m, kind[m], c});
+        b->mcount[m] = (int64_t)items.size() - b->mfirst[m];
+    }
+    b->nitems = (int64_t)items.size();
+    if (b->tail_first < 0)
+        b->tail_first = b->nitems;
+    // a kind launching on its own takes its first member's launch shape (the
+    // single operator's, autotuned or default)
+    for (int p = 0; p < b->tail_mbegin; ++p) {
+        const int m = order[p];
+        if (p == 0 || kind[m] != kind[order[p - 1]])
+            b->kind_ranges.push_back({kind[m], slab[m], b->members[m]->wpb, p, p + 1});
+        else
+            b->kind_ranges.back().mend = p + 1;
+    }
+}
+
+// persistent waves of the mixed launch: what the device holds at once
+int batch_waves(rv_batch *b)
+{
+    int64_t resident = 0;
+    if (batch_resident_waves(b->dim, b->wpb, b->slab_bytes, &resident) != 0 || resident <= 0)
+        return RV_E_HIP;
+    b->nwaves = std::min<int64_t>(resident, std::max<int64_t>(b->nitems - b->tail_first, 1));
+    return RV_OK;
+}
 
 // Stages 0..nst-1 of the batch, stage s's table at table + s * n.  active
 // (optional) marks the members that step.  Kind by kind, all stages of the
@@ -1727,52 +1793,11 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
         }
         slab[m] = (sb + 15) / 16 * 16;
     }
-    // items per kind decide which kinds launch on their own
-    std::map<int, int64_t> kind_items;
-    for (int m = 0; m < n; ++m)
-        kind_items[kind[m]] += (b->members[m]->K + epw[m] - 1) / epw[m];
-    std::vector<char> small(n);
-    for (int m = 0; m < n; ++m)
-        small[m] = b->dim != 3 && kind_items[kind[m]] < rv_batch::OWN_LAUNCH_ITEMS;
-    // work list: members sorted by (small kind, kind), each member's cells in order
-    std::vector<int> &order = b->order;
-    order.resize(n);
-    for (int m = 0; m < n; ++m)
-        order[m] = m;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return small[x] != small[y] ? small[x] < small[y] : kind[x] < kind[y];
-    });
+    b->kind = kind;
+    b->epw = epw;
+    b->slab = slab;
     std::vector<BatchItem> items;
-    b->mfirst.assign(n, 0);
-    b->mcount.assign(n, 0);
-    b->tail_mbegin = n;
-    b->tail_first = -1;
-    b->slab_bytes = 16;
-    for (int p = 0; p < n; ++p) {
-        const int m = order[p];
-        if (small[m] && b->tail_first < 0) {
-            b->tail_first = (int64_t)items.size();
-            b->tail_mbegin = p;
-        }
-        if (small[m])
-            b->slab_bytes = std::max(b->slab_bytes, slab[m]);
-        b->mfirst[m] = (int64_t)items.size();
-        for (int64_t c = 0; c < b->members[m]->K; c += epw[m])
-            items.push_back(BatchItem{m, kind[m], c});
-        b->mcount[m] = (int64_t)items.size() - b->mfirst[m];
-    }
-    b->nitems = (int64_t)items.size();
-    if (b->tail_first < 0)
-        b->tail_first = b->nitems;
-    // a kind launching on its own takes its first member's launch shape (the
-    // single operator's, autotuned or default)
-    for (int p = 0; p < b->tail_mbegin; ++p) {
-        const int m = order[p];
-        if (p == 0 || kind[m] != kind[order[p - 1]])
-            b->kind_ranges.push_back({kind[m], slab[m], b->members[m]->wpb, p, p + 1});
-        else
-            b->kind_ranges.back().mend = p + 1;
-    }
+    batch_layout(b, items);
     auto bail = [&](const char *what, hipError_t e) {
         free_batch(b);
         return bfail(nullptr, RV_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -1792,12 +1817,10 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)))
             return bail("hipEventCreate", e);
     b->device_bytes = (int64_t)(items.size() * sizeof(BatchItem) + sizeof(StageParams) * ring_entries);
-    int64_t resident = 0;
-    if (batch_resident_waves(b->dim, b->wpb, b->slab_bytes, &resident) != 0 || resident <= 0) {
+    if (batch_waves(b) != RV_OK) {
         free_batch(b);
         return bfail(nullptr, RV_E_HIP, "occupancy query of the batch kernel failed");
     }
-    b->nwaves = std::min<int64_t>(resident, std::max<int64_t>(b->nitems - b->tail_first, 1));
     *out = b;
     return RV_OK;
 }
@@ -1819,6 +1842,29 @@ int rv_batch_info(const rv_batch *b, int64_t *nitems, int64_t *nwaves, int32_t *
         *nwaves = b->nwaves;
     if (slab_bytes)
         *slab_bytes = b->slab_bytes;
+    return RV_OK;
+}
+
+int rv_batch_configure(rv_batch *b, int mixed_waves_per_block, int64_t own_launch_items)
+{
+    if (!b)
+        return bfail(nullptr, RV_E_ARG, "null batch");
+    if (mixed_waves_per_block != 1 && mixed_waves_per_block != 2 && mixed_waves_per_block != 4)
+        return bfail(b, RV_E_ARG, "waves per block must be 1, 2 or 4");
+    if (own_launch_items < 1)
+        return bfail(b, RV_E_ARG, "own_launch_items must be positive");
+    // kernels in flight read the work list and the cached graphs hold the old
+    // launches: wait for them, then lay the batch out again
+    BHIP_TRY(b, hipStreamSynchronize(b->stream));
+    drop_graphs(b);
+    b->wpb = mixed_waves_per_block;
+    b->own_items = own_launch_items;
+    std::vector<BatchItem> items;
+    batch_layout(b, items);
+    BHIP_TRY(b, hipMemcpy(b->items, items.data(), items.size() * sizeof(BatchItem),
+                          hipMemcpyHostToDevice));
+    if (batch_waves(b) != RV_OK)
+        return bfail(b, RV_E_HIP, "occupancy query of the batch kernel failed");
     return RV_OK;
 }
 

@@ -1,11 +1,12 @@
-"""Batches of independent solves over one persistent launch per stage
-(rv_batch_* in include/ravenhip.h).
+"""Batches of independent solves (rv_batch_* in include/ravenhip.h).
 
 ``Batch(dgs)`` groups DGSEM operators (same dim; affine bricks, Rusanov flux);
-``stage`` runs one fused LSRK stage of every member in one launch and
-``lsrk_run`` replays whole LSRK steps as captured hipGraphs.  Raven advances
-one solver per ``solve!`` call (odesolvers.jl:9-30); every member's result here
-is the one its own operator gives.
+``stage`` runs one fused LSRK stage of every member (kinds large enough to fill
+the device launch on their own, the small 2-D kinds share one persistent
+launch), ``step`` one whole LSRK step, and ``lsrk_run`` replays whole LSRK steps
+as captured hipGraphs.  ``autotune`` picks the launch layout by timing it.
+Raven advances one solver per ``solve!`` call (odesolvers.jl:9-30); every
+member's result here is the one its own operator gives.
 """
 import ctypes
 
@@ -95,3 +96,50 @@ class Batch:
                 for ts in (q0, q1, dq)]
         L.check_batch(L.lib().rv_batch_step(self._b, *ptrs, L.ptr(cv, ctypes.c_int32),
                                             L.ptr(dv, ctypes.c_double), int(scheme)), self._b)
+
+    def configure(self, mixed_waves_per_block=2, own_launch_items=1024):
+        """Launch layout: waves per block of the mixed launch and the item
+        count from which a 2-D kind launches on its own (rv_batch_configure)."""
+        L.check_batch(L.lib().rv_batch_configure(self._b, int(mixed_waves_per_block),
+                                                 int(own_launch_items)), self._b)
+
+    def autotune(self, q, dt, mixed_waves=(1, 2, 4), own_items=(512, 1024, 4096, 1 << 40),
+                 member_waves=(1, 2), steps=2, scheme=L.RV_LSRK54):
+        """Pick the launch layout by timing what a run does: whole LSRK steps
+        replayed as hipGraphs (lsrk_run), on scratch copies of the states q
+        with step sizes dt (q is not modified).
+
+        First each kind's waves per block (the single operator's launch shape,
+        timed on the kind's largest member with DGSEM.autotune and applied to
+        every member of the kind), then the mixed launch's waves per block and
+        the own-launch threshold: for each layout, one run of `steps` steps
+        builds its graphs and a second is timed.  Returns {(mixed_waves,
+        own_items): ms per step}."""
+        import torch
+        by_kind = {}
+        for k, dg in enumerate(self.dgs):
+            key = (dg.info.dim, dg.info.nq, dg.law.law_id, dg.dtype)
+            if key not in by_kind or dg.K > self.dgs[by_kind[key]].K:
+                by_kind[key] = k
+        for key, k in by_kind.items():
+            self.dgs[k].autotune(q[k], choices=member_waves)
+            w = self.dgs[k].waves_per_block
+            for dg in self.dgs:
+                if (dg.info.dim, dg.info.nq, dg.law.law_id, dg.dtype) == key:
+                    dg.waves_per_block = w
+        qs = [t.clone() for t in q]
+        dqs = [torch.empty_like(t) for t in q]
+        times = {}
+        for w in mixed_waves:
+            for t in own_items:
+                self.configure(w, t)
+                self.lsrk_run(qs, dqs, dt, steps, scheme)  # captures the step graphs
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record()
+                self.lsrk_run(qs, dqs, dt, steps, scheme)
+                ev[1].record()
+                ev[1].synchronize()
+                times[(w, t)] = ev[0].elapsed_time(ev[1]) / steps
+        best = min(times, key=times.get)
+        self.configure(*best)
+        return times

@@ -186,6 +186,42 @@ def test_batch_step_skips_resting_members(torch, R, dim, mix):
     batch.close()
 
 
+def test_batch_layout_does_not_change_results(torch, R):
+    """Every launch layout rv_batch_configure offers (all kinds on their own,
+    all in the mixed launch, each mixed waves-per-block) and the autotuner's
+    choice give bit-identical stages."""
+    ms = [member(R, torch, 2, nq, n, law, dt, 30 + s) for s, (nq, n, law, dt) in enumerate(MIX2H)]
+    dgs = [m[0] for m in ms]
+    batch = R.Batch(dgs)
+    q_in = [m[1] for m in ms]
+    rng = np.random.default_rng(5)
+    dq0 = [torch.from_numpy(1e-3 * rng.standard_normal(q.numel())).to(q.dtype).cuda()
+           .reshape(q.shape) for q in q_in]
+    a = [-0.3 - 0.05 * k for k in range(len(ms))]
+    bdt = [2e-3 * (k + 1) for k in range(len(ms))]
+
+    def run():
+        q_out = [torch.empty_like(q) for q in q_in]
+        dq = [d.clone() for d in dq0]
+        batch.stage(q_out, q_in, dq, a, bdt)
+        torch.cuda.synchronize()
+        return q_out, dq
+
+    want = run()
+    for w, t in [(1, 1), (4, 1), (1, 1 << 40), (2, 1 << 40), (4, 1 << 40), (2, 1024)]:
+        batch.configure(w, t)
+        got = run()
+        for k in range(len(ms)):
+            assert torch.equal(got[0][k], want[0][k]) and torch.equal(got[1][k], want[1][k]), \
+                f"layout ({w}, {t}): member {k} differs"
+    times = batch.autotune(q_in, [1e-3] * len(ms))
+    say(f"   autotune: {times}")
+    got = run()
+    for k in range(len(ms)):
+        assert torch.equal(got[0][k], want[0][k]) and torch.equal(got[1][k], want[1][k])
+    batch.close()
+
+
 def test_batch_rejects_unsupported_members(torch, R):
     cell = R.LobattoCell(5, 5)
     grid = R.generate(R.GridManager(cell, R.brick([PB.julia_range(0, 1, 4)] * 2, (True, True))))
