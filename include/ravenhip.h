@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define RAVENHIP_ABI_VERSION 1
+#define RAVENHIP_ABI_VERSION 2
 
 enum {
     RV_OK = 0,
@@ -50,8 +50,13 @@ enum {
 
 /* law ids (src/balancelaws/balancelaws/{advection,euler}.jl) */
 enum { RV_LAW_ADVECTION = 1, RV_LAW_EULER = 2 };
-/* surface numerical flux (src/balancelaws/numericalfluxes.jl:39-50) */
-enum { RV_FLUX_RUSANOV = 1, RV_FLUX_CENTRAL = 2 };
+/* numerical fluxes (src/balancelaws/numericalfluxes.jl:39-50, euler.jl:150-184):
+ * surface fluxes RUSANOV and CENTRAL; volume (two-point) fluxes of the flux
+ * differencing form CENTRAL and EC (EntropyConservativeFlux, Euler only) */
+enum { RV_FLUX_RUSANOV = 1, RV_FLUX_CENTRAL = 2, RV_FLUX_EC = 3 };
+/* volume form (dgsem.jl:1-17): WeakForm, or FluxDifferencingForm(volume flux)
+ * evaluated direction by direction as esvolumeterm_dir! (dgsem.jl:514-608) */
+enum { RV_FORM_WEAK = 0, RV_FORM_FLUX_DIFFERENCING = 1 };
 /* state precision: FP64, or FP32 state/flux arithmetic with FP64 geometry */
 enum { RV_F64 = 1, RV_F32 = 2 };
 /* geometry handling: AUTO detects affine elements (per-element scale factors)
@@ -86,6 +91,9 @@ typedef struct rv_desc {
     const int64_t *dtoc;   /* discontinuoustocontinuous(grid), may be NULL */
     int64_t ncontinuous;   /* numcnodes(grid) when dtoc is given */
     void *stream;          /* hipStream_t every call runs on; NULL: the default stream */
+    /* from ABI version 2 (version 1 descriptors end above: weak form) */
+    int32_t volume_form;   /* RV_FORM_* */
+    int32_t volume_flux;   /* RV_FLUX_CENTRAL or RV_FLUX_EC with RV_FORM_FLUX_DIFFERENCING */
 } rv_desc;
 
 /* Brick descriptor: the grid a GridManager(LobattoCell(Nq,...), brick(coords,
@@ -101,6 +109,8 @@ typedef struct rv_brick_desc {
     double gamma;
     double velocity[3];
     void *stream;
+    /* from ABI version 2, as in rv_desc */
+    int32_t volume_form, volume_flux;
 } rv_brick_desc;
 
 typedef struct rv_info {

@@ -11,6 +11,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <quadmath.h>
 #include <stdlib.h>
@@ -752,6 +753,7 @@ int orc_lsrk_coefficients(int scheme, double *a, double *b, double *c)
 /* ------------------------------------------------------------------------ */
 struct orc_dg {
     int dim, nq, ns, law, nthreads;
+    int volume_form, volume_flux;  /* 0 weak / 1 flux differencing; 2 central / 3 EC */
     int64_t K, Np, nfp;
     double gamma, u[3];
     const double *D, *metrics, *surf;
@@ -801,6 +803,12 @@ void orc_dg_free(orc_dg *dg)
 }
 
 int orc_dg_ns(const orc_dg *dg) { return dg->ns; }
+
+void orc_dg_set_volume_form(orc_dg *dg, int form, int volume_flux)
+{
+    dg->volume_form = form;
+    dg->volume_flux = volume_flux;
+}
 void orc_dg_set_threads(orc_dg *dg, int nthreads) { dg->nthreads = nthreads; }
 
 /* flux(law, q) as a dim x Ns matrix F[dd + dim*s]
@@ -884,6 +892,136 @@ static void gather_state(const orc_dg *dg, const double *q, int64_t id, double *
         out[s] = q[node + dg->Np * (s + (int64_t)dg->ns * e)];
 }
 
+/* logavg (numericalfluxes.jl:9-22): the logarithmic mean with the series of
+ * log(zeta)/(2f) in u = f^2 below eps */
+static double logavg(double a, double b)
+{
+    double zeta = a / b;
+    double f = (zeta - 1) / (zeta + 1);
+    double u = f * f;
+    double F;
+    if (u < DBL_EPSILON)
+        F = 1.0 + u * (1.0 / 3 + u * (1.0 / 5 + u * (1.0 / 7 + u * (1.0 / 9))));
+    else
+        F = log(zeta) / (2 * f);
+    return (a + b) / (2 * F);
+}
+
+/* twopointflux as a dim x Ns matrix F[d + dim*s]:
+ * CentralFlux (numericalfluxes.jl:39-42): (F(q1) + F(q2)) / 2;
+ * EntropyConservativeFlux for Euler (euler.jl:150-184) */
+static void twopoint(const orc_dg *dg, const double *q1, const double *q2, double *F)
+{
+    const int dim = dg->dim, ns = dg->ns;
+    if (dg->volume_flux == 2 || dg->law == 1) {
+        double F1[15], F2[15];
+        law_flux(dg, q1, F1);
+        law_flux(dg, q2, F2);
+        for (int k = 0; k < dim * ns; ++k)
+            F[k] = (F1[k] + F2[k]) / 2;
+        return;
+    }
+    const double gm1 = dg->gamma - 1;
+    double u1[3], u2[3], uavg[3], fr[3];
+    for (int d = 0; d < dim; ++d) {
+        u1[d] = q1[1 + d] / q1[0];
+        u2[d] = q2[1 + d] / q2[0];
+    }
+    double dot1 = q1[1] * q1[1], dot2 = q2[1] * q2[1];
+    for (int d = 1; d < dim; ++d) {
+        dot1 = dot1 + q1[1 + d] * q1[1 + d];
+        dot2 = dot2 + q2[1 + d] * q2[1 + d];
+    }
+    double p1 = gm1 * (q1[dim + 1] - dot1 / (2 * q1[0]));
+    double p2 = gm1 * (q2[dim + 1] - dot2 / (2 * q2[0]));
+    double b1 = q1[0] / (2 * p1), b2 = q2[0] / (2 * p2);
+    double ravg = (q1[0] + q2[0]) / 2;
+    for (int d = 0; d < dim; ++d)
+        uavg[d] = (u1[d] + u2[d]) / 2;
+    double bavg = (b1 + b2) / 2;
+    double uu1 = u1[0] * u1[0], uu2 = u2[0] * u2[0];
+    for (int d = 1; d < dim; ++d) {
+        uu1 = uu1 + u1[d] * u1[d];
+        uu2 = uu2 + u2[d] * u2[d];
+    }
+    double u2avg = (uu1 + uu2) / 2;
+    double rlog = logavg(q1[0], q2[0]);
+    double blog = logavg(b1, b2);
+    for (int d = 0; d < dim; ++d)
+        fr[d] = uavg[d] * rlog;
+    double pav = ravg / (2 * bavg);
+    for (int d = 0; d < dim; ++d) {
+        F[d + dim * 0] = fr[d];
+        for (int b = 0; b < dim; ++b)
+            F[d + dim * (1 + b)] = uavg[d] * fr[b] + (d == b ? pav : 0.0);
+    }
+    double c = 1 / (2 * gm1 * blog) - u2avg / 2;
+    for (int d = 0; d < dim; ++d) {
+        double v = F[d + dim * 1] * uavg[0];
+        for (int b = 1; b < dim; ++b)
+            v = v + F[d + dim * (1 + b)] * uavg[b];
+        F[d + dim * (dim + 1)] = c * fr[d] + v;
+    }
+}
+
+/* FluxDifferencingForm volume term, one direction at a time as
+ * esvolumeterm_dir! (dgsem.jl:514-608): for every node and line partner n,
+ *   dq -= MJI D[id,n] l_g(ijk) . f  and  dq += f . l_g(n) MJI D[n,id],
+ * l_g = MJ dRdX[dir, :]; the first direction sets (or increments) dq, the
+ * others add to it. */
+static void fd_volume(orc_dg *dg, double *dq, const double *q, int increment, int nth)
+{
+    const int dim = dg->dim, nq = dg->nq, ns = dg->ns;
+    const int64_t Np = dg->Np, K = dg->K;
+    const int L = dim * dim + 2;
+    const double *D = dg->D;
+    (void)nth;
+    for (int dir = 0; dir < dim; ++dir) {
+        const int inc = dir == 0 ? increment : 1;
+#pragma omp parallel for schedule(static) num_threads(nth)
+        for (int64_t e = 0; e < K; ++e) {
+            for (int64_t node = 0; node < Np; ++node) {
+                int i = (int)(node % nq);
+                int j = dim > 1 ? (int)((node / nq) % nq) : 0;
+                int k = dim > 2 ? (int)(node / ((int64_t)nq * nq)) : 0;
+                const int id = dir == 0 ? i : dir == 1 ? j : k;
+                double q1[5], q2[5], F[15], acc[5], lg1[3], lg2[3];
+                for (int s = 0; s < ns; ++s) {
+                    q1[s] = q[node + Np * (s + (int64_t)ns * e)];
+                    acc[s] = -0.0;
+                }
+                const double MJ1 = dg->metrics[node + Np * ((L - 1) + (int64_t)L * e)];
+                for (int d = 0; d < dim; ++d)
+                    lg1[d] = MJ1 * dg->metrics[node + Np * ((dir + dim * d) + (int64_t)L * e)];
+                const double MJI = dg->MJI[node + Np * e];
+                for (int n = 0; n < nq; ++n) {
+                    int64_t ild = dir == 0 ? n + nq * (j + (int64_t)nq * k)
+                                 : dir == 1 ? i + nq * (n + (int64_t)nq * k)
+                                            : i + nq * (j + (int64_t)nq * n);
+                    for (int s = 0; s < ns; ++s)
+                        q2[s] = q[ild + Np * (s + (int64_t)ns * e)];
+                    const double MJ2 = dg->metrics[ild + Np * ((L - 1) + (int64_t)L * e)];
+                    for (int d = 0; d < dim; ++d)
+                        lg2[d] = MJ2 * dg->metrics[ild + Np * ((dir + dim * d) + (int64_t)L * e)];
+                    twopoint(dg, q1, q2, F);
+                    for (int s = 0; s < ns; ++s) {
+                        double Ddn = MJI * D[id + nq * n];
+                        double Dnd = MJI * D[n + nq * id];
+                        for (int d = 0; d < dim; ++d) {
+                            acc[s] = acc[s] - (Ddn * lg1[d]) * F[d + dim * s];
+                            acc[s] = acc[s] + (F[d + dim * s] * lg2[d]) * Dnd;
+                        }
+                    }
+                }
+                for (int s = 0; s < ns; ++s) {
+                    double *dst = &dq[node + Np * (s + (int64_t)ns * e)];
+                    *dst = inc ? (*dst + acc[s]) : acc[s];
+                }
+            }
+        }
+    }
+}
+
 void orc_dg_rhs(orc_dg *dg, double *dq, const double *q, double t, int increment)
 {
     (void)t;
@@ -896,6 +1034,13 @@ void orc_dg_rhs(orc_dg *dg, double *dq, const double *q, double t, int increment
 #ifdef _OPENMP
     int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
 #endif
+#ifndef _OPENMP
+    int nth = 1;
+#endif
+    if (dg->volume_form == 1) {
+        fd_volume(dg, dq, q, increment, nth);
+        goto surface;
+    }
     /* volumeterm! (dgsem.jl:257-348) */
 #pragma omp parallel num_threads(nth)
     {
@@ -947,6 +1092,7 @@ void orc_dg_rhs(orc_dg *dg, double *dq, const double *q, double t, int increment
         }
         free(lF);
     }
+surface:;
     /* surfaceterm! (dgsem.jl:733-784): each element updates only its own
      * nodes, faces in the order d = 1..dim, f = 1, 2 */
 #pragma omp parallel for schedule(static) num_threads(nth)

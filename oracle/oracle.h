@@ -112,6 +112,11 @@ void orc_dg_mortar_add(const orc_dg *dg, const int32_t *kind, const int64_t *par
                        const double *lower, const double *upper, const double *surf_full,
                        const double *q, double *dq);
 
+/* Volume form (dgsem.jl:1-17): 0 = WeakForm, 1 = FluxDifferencingForm with
+ * volume_flux 2 = CentralFlux or 3 = EntropyConservativeFlux (Euler),
+ * evaluated direction by direction as esvolumeterm_dir! (dgsem.jl:514-608). */
+void orc_dg_set_volume_form(orc_dg *dg, int form, int volume_flux);
+
 /* weightednorm(dg, q) component norms (measures.jl:6-14), p = 2. */
 void orc_dg_weightednorm(const orc_dg *dg, const double *q, double *comp);
 

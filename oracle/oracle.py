@@ -54,6 +54,7 @@ def lib():
         L.orc_dg_ns.argtypes = [ctypes.c_void_p]
         L.orc_dg_ns.restype = ctypes.c_int
         L.orc_dg_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_dg_set_volume_form.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.orc_dg_mortar_add.argtypes = [ctypes.c_void_p, i32p, i64p, f64p, f64p, f64p, f64p,
                                          f64p]
         L.orc_dg_rhs.argtypes = [ctypes.c_void_p, f64p, f64p, ctypes.c_double, ctypes.c_int]
@@ -139,7 +140,9 @@ class BrickGrid:
 class DG:
     """Oracle DGSEM operator for law 'advection' or 'euler'."""
 
-    def __init__(self, grid, law, gamma=1.4, u=None):
+    def __init__(self, grid, law, gamma=1.4, u=None, volume_flux=None):
+        """volume_flux None: WeakForm; "central" or "ec": FluxDifferencingForm
+        with CentralFlux or EntropyConservativeFlux (dgsem.jl:1-17)."""
         self.grid = grid
         self.law = {"advection": 1, "euler": 2}[law]
         u = np.zeros(3) if u is None else np.asarray(u, dtype=np.float64)
@@ -149,6 +152,8 @@ class DG:
                                   grid.D, grid.metrics, grid.surf, grid.vmapM, grid.vmapP,
                                   grid.bc)
         self.ns = lib().orc_dg_ns(self.h)
+        if volume_flux is not None:
+            lib().orc_dg_set_volume_form(self.h, 1, {"central": 2, "ec": 3}[volume_flux])
 
     def __del__(self):
         if getattr(self, "h", None):

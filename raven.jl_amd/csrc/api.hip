@@ -27,9 +27,25 @@ StageLauncher find_d3_f64_euler(const KernelKey &);
 StageLauncher find_d3_f32_adv(const KernelKey &);
 StageLauncher find_d3_f32_euler(const KernelKey &);
 
+StageLauncher find_fd_d1_f64(const KernelKey &);
+StageLauncher find_fd_d1_f32(const KernelKey &);
+StageLauncher find_fd_d2_f64(const KernelKey &);
+StageLauncher find_fd_d2_f32(const KernelKey &);
+StageLauncher find_fd_d3_f64(const KernelKey &);
+StageLauncher find_fd_d3_f32(const KernelKey &);
+
 StageLauncher find_stage_launcher(const KernelKey &k)
 {
     using Finder = StageLauncher (*)(const KernelKey &);
+    if (k.vflux != 0) {
+        // flux differencing form (fluxdiff.hip)
+        static const Finder fd[] = {find_fd_d1_f64, find_fd_d1_f32, find_fd_d2_f64,
+                                    find_fd_d2_f32, find_fd_d3_f64, find_fd_d3_f32};
+        for (Finder f : fd)
+            if (StageLauncher l = f(k))
+                return l;
+        return nullptr;
+    }
     static const Finder finders[] = {
         find_d1_f64_adv, find_d1_f64_euler, find_d1_f32_adv, find_d1_f32_euler,
         find_d2_f64_adv, find_d2_f64_euler, find_d2_f32_adv, find_d2_f32_euler,
@@ -46,6 +62,7 @@ using namespace rv;
 struct rv_handle {
     std::string err;
     int dim = 0, nq = 0, ns = 0, law = 0, flux = 0, dtype = 0, geo = 0, dir_mask = 0;
+    int vflux = 0;  // two-point flux of the flux-differencing volume form, 0: weak form
     int64_t K = 0, Np = 0, nfp = 0;
     double gamma = 1.4, u[3] = {0, 0, 0};
     int device = 0;
@@ -530,7 +547,8 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
         (rc = upload(h, &h->winv, winv.data(), winv.size())) ||
         (rc = dalloc(h, &h->partials, (size_t)2 * RED_BLOCKS * h->ns)))
         return rc;
-    const KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
+    KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
+    key.vflux = h->vflux;
     h->launcher = find_stage_launcher(key);
     // waves per workgroup measured on MI355X for the 3-D column kernel at
     // Nq = 8: two for FP64 (256 VGPRs, 2 waves/SIMD), four for FP32
@@ -538,7 +556,9 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
     if (!h->launcher)
         return fail(h, RV_E_UNSUPPORTED,
                     "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +
-                        " law=" + std::to_string(h->law) + " dtype=" + std::to_string(h->dtype));
+                        " law=" + std::to_string(h->law) + " dtype=" + std::to_string(h->dtype) +
+                        (h->vflux ? " (flux differencing: Nq^dim <= 512, Rusanov surface flux)"
+                                  : ""));
     return RV_OK;
 }
 
@@ -846,6 +866,22 @@ int reduce(rv_handle *h, const void *q, int power, double *out)
 // ===========================================================================
 extern "C" {
 
+// The volume form of a descriptor as the kernel key's two-point flux (0: weak
+// form); the entropy-conservative flux is defined for Euler only
+static int volume_flux_of(int form, int flux, int law, int *vflux)
+{
+    *vflux = 0;
+    if (form == RV_FORM_WEAK)
+        return RV_OK;
+    if (form != RV_FORM_FLUX_DIFFERENCING)
+        return RV_E_ARG;
+    if (flux == RV_FLUX_CENTRAL || (flux == RV_FLUX_EC && law == RV_LAW_EULER)) {
+        *vflux = flux;
+        return RV_OK;
+    }
+    return RV_E_ARG;
+}
+
 int rv_create(const rv_desc *desc, rv_handle **out)
 {
     if (!out)
@@ -853,14 +889,19 @@ int rv_create(const rv_desc *desc, rv_handle **out)
     *out = nullptr;
     if (!desc)
         return fail(nullptr, RV_E_ARG, "descriptor is null");
-    if (desc->abi_version != RAVENHIP_ABI_VERSION)
+    if (desc->abi_version != 1 && desc->abi_version != RAVENHIP_ABI_VERSION)
         return fail(nullptr, RV_E_ARG, "ABI version mismatch");
+    int vflux = 0;
+    if (desc->abi_version >= 2 &&
+        volume_flux_of(desc->volume_form, desc->volume_flux, desc->law, &vflux) != RV_OK)
+        return fail(nullptr, RV_E_ARG, "invalid volume form or volume flux");
     if (desc->dim < 1 || desc->dim > 3 || desc->nq < 2 || desc->nq > 16 || desc->ncells < 0 ||
         (desc->law != RV_LAW_ADVECTION && desc->law != RV_LAW_EULER) ||
         (desc->dtype != RV_F64 && desc->dtype != RV_F32) ||
         (desc->flux != 0 && desc->flux != RV_FLUX_RUSANOV && desc->flux != RV_FLUX_CENTRAL))
         return fail(nullptr, RV_E_ARG, "invalid descriptor field (dim, nq, ncells, law, dtype or flux)");
     rv_handle *h = new rv_handle();
+    h->vflux = vflux;
     int rc = create_from_desc(desc, h);
     if (rc) {
         g_last_error = h->err;
@@ -873,7 +914,7 @@ int rv_create(const rv_desc *desc, rv_handle **out)
 
 static int brick_spec(const rv_brick_desc *bd, BrickSpec *b)
 {
-    if (!bd || bd->abi_version != RAVENHIP_ABI_VERSION)
+    if (!bd || (bd->abi_version != 1 && bd->abi_version != RAVENHIP_ABI_VERSION))
         return fail(nullptr, RV_E_ARG, "null descriptor or ABI version mismatch");
     if (bd->dim < 1 || bd->dim > 3 || bd->nq < 2 || bd->nq > 16 || bd->level < 0 ||
         bd->level > 10 || (bd->law != RV_LAW_ADVECTION && bd->law != RV_LAW_EULER) ||
@@ -921,6 +962,11 @@ int rv_create_brick(const rv_brick_desc *bd, rv_handle **out)
     h->nq = nq;
     h->law = bd->law;
     h->flux = bd->flux ? bd->flux : RV_FLUX_RUSANOV;
+    if (bd->abi_version >= 2 &&
+        volume_flux_of(bd->volume_form, bd->volume_flux, bd->law, &h->vflux) != RV_OK) {
+        free_handle(h);
+        return fail(nullptr, RV_E_ARG, "invalid volume form or volume flux");
+    }
     h->dtype = bd->dtype;
     h->K = brick_numcells(b);
     h->ns = bd->law == RV_LAW_ADVECTION ? 1 : dim + 2;
@@ -1781,10 +1827,12 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
             free_batch(b);
             return bfail(nullptr, RV_E_ARG, "batch members must share dim and stream");
         }
-        if (h->geo != GEO_AFFINE || h->flux != RV_FLUX_RUSANOV || h->dir_mask != (1 << h->dim) - 1) {
+        if (h->geo != GEO_AFFINE || h->flux != RV_FLUX_RUSANOV || h->vflux != 0 ||
+            h->dir_mask != (1 << h->dim) - 1) {
             free_batch(b);
             return bfail(nullptr, RV_E_UNSUPPORTED,
-                         "batch members need affine geometry, the Rusanov flux and all directions");
+                         "batch members need affine geometry, the weak form, the Rusanov flux "
+                         "and all directions");
         }
         int sb = 0;
         if (!batch_kind(h->dim, h->nq, h->law, h->dtype == RV_F32 ? 4 : 8, &kind[m], &epw[m], &sb)) {

@@ -8,7 +8,7 @@ namespace rv {
 
 enum LawId : int { LAW_ADVECTION = 1, LAW_EULER = 2 };
 enum GeoId : int { GEO_AFFINE = 1, GEO_GENERAL = 2 };
-enum FluxId : int { FLUX_RUSANOV = 1, FLUX_CENTRAL = 2 };
+enum FluxId : int { FLUX_RUSANOV = 1, FLUX_CENTRAL = 2, FLUX_EC = 3 };
 
 // Per-face neighbour record, one per (face, cell):
 //   x >= 0 : neighbour cell; y = neighbour face | orientation << 3
@@ -77,6 +77,7 @@ using StageLauncher = int (*)(const StageParams &, void *stream, int waves_per_b
 
 struct KernelKey {
     int dim, nq, law, real_bytes, geo, flux;
+    int vflux = 0;  // 0: weak form; else the flux-differencing form's two-point flux
 };
 
 // Mortar coupling of hanging faces (mortar.hip), launched after a stage.

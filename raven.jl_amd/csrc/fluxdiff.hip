@@ -1,0 +1,339 @@
+// FluxDifferencingForm stage kernel (dgsem.jl:1-17, 163-255, 514-608):
+// the volume term from a symmetric two-point flux instead of the weak form,
+//   dq(ijk) += sum_dir sum_n [ -MJI D[id,n] l_g(ijk) . f#(q_ijk, q_n)
+//                              + f#(q_ijk, q_n) . l_g(n) MJI D[n,id] ],
+// l_g = MJ dRdX[dir, :], n running over the line through ijk along dir, with
+// f# = CentralFlux (numericalfluxes.jl:39-42) or, for Euler,
+// EntropyConservativeFlux (euler.jl:150-184, logavg numericalfluxes.jl:9-22),
+// fused with the Rusanov surface term (dgsem.jl:733-784) and the LSRK update
+// dq = a dq + L(q), q_out = q + b dt dq (odesolvers.jl:55-57).
+//
+// The form is compute-bound (Nq two-point fluxes per node and direction, two
+// logarithms each for the entropy-conservative flux), so the layout is the
+// plain one: a thread per node, whole cells per workgroup, the cells' states
+// staged in LDS for the line partners.  Affine cells evaluate only the row
+// dir of the two-point flux (their l_g has no other component).
+//
+// Compiled once per (dim, precision) with -DFD_DIM and -DFD_REAL; api.hip asks
+// every unit for a launcher.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "dg_kernels.hpp"
+
+#ifndef FD_DIM
+#error "FD_DIM must be defined"
+#endif
+
+namespace rv {
+
+namespace {
+
+template <int DIM, int NS, int LAW, int VF, typename R>
+struct TwoPoint {
+    // row `row` of the dim x Ns two-point flux matrix between q1 and q2
+    __device__ static inline void rows(const Law<DIM, LAW, R> &law, const R *q1, const R *q2,
+                                       int row0, int nrows, R (*F)[NS])
+    {
+        if constexpr (VF == FLUX_CENTRAL || LAW == LAW_ADVECTION) {
+            for (int r = 0; r < nrows; ++r) {
+                R f1[NS], f2[NS];
+                law.flux_row(row0 + r, q1, f1);
+                law.flux_row(row0 + r, q2, f2);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    F[r][s] = (f1[s] + f2[s]) / R(2);
+            }
+        } else {
+            // EntropyConservativeFlux (euler.jl:150-184)
+            const R gm1 = law.gm1;
+            R u1[DIM], u2[DIM], uavg[DIM], fr[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                u1[d] = q1[1 + d] / q1[0];
+                u2[d] = q2[1 + d] / q2[0];
+            }
+            R dot1 = q1[1] * q1[1], dot2 = q2[1] * q2[1];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) {
+                dot1 = dot1 + q1[1 + d] * q1[1 + d];
+                dot2 = dot2 + q2[1 + d] * q2[1 + d];
+            }
+            const R p1 = gm1 * (q1[DIM + 1] - dot1 / (R(2) * q1[0]));
+            const R p2 = gm1 * (q2[DIM + 1] - dot2 / (R(2) * q2[0]));
+            const R b1 = q1[0] / (R(2) * p1), b2 = q2[0] / (R(2) * p2);
+            const R ravg = (q1[0] + q2[0]) / R(2);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                uavg[d] = (u1[d] + u2[d]) / R(2);
+            const R bavg = (b1 + b2) / R(2);
+            R uu1 = u1[0] * u1[0], uu2 = u2[0] * u2[0];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) {
+                uu1 = uu1 + u1[d] * u1[d];
+                uu2 = uu2 + u2[d] * u2[d];
+            }
+            const R u2avg = (uu1 + uu2) / R(2);
+            const R rlog = logavg(q1[0], q2[0]);
+            const R blog = logavg(b1, b2);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                fr[d] = uavg[d] * rlog;
+            const R pav = ravg / (R(2) * bavg);
+            const R c = R(1) / (R(2) * gm1 * blog) - u2avg / R(2);
+            for (int r = 0; r < nrows; ++r) {
+                const int d = row0 + r;
+                F[r][0] = fr[d];
+#pragma unroll
+                for (int b = 0; b < DIM; ++b)
+                    F[r][1 + b] = uavg[d] * fr[b] + (d == b ? pav : R(0));
+                R v = F[r][1] * uavg[0];
+#pragma unroll
+                for (int b = 1; b < DIM; ++b)
+                    v = v + F[r][1 + b] * uavg[b];
+                F[r][DIM + 1] = c * fr[d] + v;
+            }
+        }
+    }
+
+    // logavg (numericalfluxes.jl:9-22): the logarithmic mean, with the series
+    // of log(zeta) / (2f) in u = f^2 when u is below the type's epsilon
+    __device__ static inline R logavg(R a, R b)
+    {
+        const R zeta = a / b;
+        const R f = (zeta - R(1)) / (zeta + R(1));
+        const R u = f * f;
+        const R eps = sizeof(R) == 8 ? R(2.220446049250313e-16) : R(1.1920929e-07);
+        R F;
+        if (u < eps)
+            F = R(1) + u * (R(1) / R(3) + u * (R(1) / R(5) + u * (R(1) / R(7) + u * (R(1) / R(9)))));
+        else
+            F = log(zeta) / (R(2) * f);
+        return (a + b) / (R(2) * F);
+    }
+};
+
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF>
+struct FD {
+    static constexpr int NS = LawInfo<DIM, LAW>::NS;
+    static constexpr int NP = IPow<NQ, DIM>::v;
+    static constexpr int NFP = IPow<NQ, DIM - 1>::v;
+    static constexpr int NF = 2 * DIM;
+    static constexpr int CPB = NP >= 256 ? 1 : 256 / NP;  // cells per workgroup
+    static constexpr int THREADS = CPB * NP;
+    using S = Stage<DIM, NQ, LAW, R, GEO, FLUX_RUSANOV>;
+};
+
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF>
+__global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
+{
+    using T = FD<DIM, NQ, LAW, R, GEO, VF>;
+    using S = typename T::S;
+    constexpr int NS = T::NS, NP = T::NP, NFP = T::NFP, NF = T::NF;
+    __shared__ R qs[T::CPB][NS][NP];
+    const int t = threadIdx.x;
+    const int c = t / NP, node = t - c * NP;
+    const int64_t e = (int64_t)blockIdx.x * T::CPB + c;
+    const bool active = e < P.K;
+    const Law<DIM, LAW, R> law(P);
+    const R *qin = static_cast<const R *>(P.q_in);
+    R q1[NS], acc[NS];
+    if (active) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            q1[s] = qin[(e * NS + s) * NP + node];
+            qs[c][s][node] = q1[s];
+        }
+    }
+    __syncthreads();
+    if (!active)
+        return;
+    const int ijk[3] = {node % NQ, DIM > 1 ? (node / NQ) % NQ : 0,
+                        DIM > 2 ? node / (NQ * NQ) : 0};
+    const double MJ1 = P.mj[e * NP + node];
+    const double MJI = GEO == GEO_GENERAL ? P.mji[e * NP + node] : 1.0 / MJ1;
+    const R a = R(P.a);
+    R *dq = static_cast<R *>(P.dq);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        acc[s] = P.read_dq ? a * dq[(e * NS + s) * NP + node] : R(0);
+
+    // ---- volume: one direction after another (esvolumeterm_dir!)
+#pragma unroll
+    for (int dir = 0; dir < DIM; ++dir) {
+        if (!((P.dir_mask >> dir) & 1))
+            continue;
+        const int id = ijk[dir];
+        int stride = 1;
+#pragma unroll
+        for (int d = 0; d < dir; ++d)
+            stride *= NQ;
+        const int line0 = node - id * stride;
+        R lg1[DIM];
+        if constexpr (GEO == GEO_GENERAL) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                lg1[d] = R(P.vgeo[((e * DIM + dir) * DIM + d) * NP + node]);
+        } else {
+            lg1[0] = R(MJ1 * P.affine[e * DIM + dir]);
+        }
+        R part[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            part[s] = R(-0.0);
+        for (int n = 0; n < NQ; ++n) {
+            const int ild = line0 + n * stride;
+            R q2[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                q2[s] = qs[c][s][ild];
+            const R Ddn = R(MJI * P.Dt[n * NQ + id]);   // MJI D[id, n]
+            const R Dnd = R(MJI * P.Dt[id * NQ + n]);   // MJI D[n, id]
+            if constexpr (GEO == GEO_GENERAL) {
+                R F[DIM][NS], lg2[DIM];
+                TwoPoint<DIM, NS, LAW, VF, R>::rows(law, q1, q2, 0, DIM, F);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                    lg2[d] = R(P.vgeo[((e * DIM + dir) * DIM + d) * NP + ild]);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        part[s] = part[s] - (Ddn * lg1[d]) * F[d][s];
+                        part[s] = part[s] + (F[d][s] * lg2[d]) * Dnd;
+                    }
+            } else {
+                R F[1][NS];
+                TwoPoint<DIM, NS, LAW, VF, R>::rows(law, q1, q2, dir, 1, F);
+                const R lg2 = R(P.mj[e * NP + ild] * P.affine[e * DIM + dir]);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    part[s] = part[s] - (Ddn * lg1[0]) * F[0][s];
+                    part[s] = part[s] + (F[0][s] * lg2) * Dnd;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            acc[s] = acc[s] + part[s];
+    }
+
+    // ---- surface: the faces this node lies on, in order d = 1..dim, f = 1, 2
+    // (surfaceterm!, Rusanov flux)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int d = f >> 1;
+        if (!((P.dir_mask >> d) & 1))
+            continue;
+        if (ijk[d] != ((f & 1) ? NQ - 1 : 0))
+            continue;
+        int m = 0;
+        if constexpr (DIM == 2)
+            m = d == 0 ? ijk[1] : ijk[0];
+        if constexpr (DIM == 3)
+            m = d == 0 ? ijk[1] + NQ * ijk[2] : d == 1 ? ijk[0] + NQ * ijk[2] : ijk[0] + NQ * ijk[1];
+        const FaceRec rec = P.faces[e * NF + f];
+        R qp[NS], nf[NS];
+        if (rec.x >= 0) {
+            const int fn = rec.y & 7, o = (rec.y >> 3) & 7;
+            const int nn = face_node<DIM, NQ>(fn, orient_node<DIM, NQ>(o, m));
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                qp[s] = qin[((int64_t)rec.x * NS + s) * NP + nn];
+        } else if (rec.x == FACE_EXPLICIT) {
+            const int64_t off = P.explicit_idx[(int64_t)rec.y + m];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                qp[s] = qin[off + s * NP];
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                qp[s] = q1[s];
+        }
+        R lift;
+        if constexpr (GEO == GEO_AFFINE) {
+            R fm[NS];
+            law.flux_row(d, q1, fm);
+            S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, q1, qp, nf);
+            lift = R(P.affine[e * DIM + d] * P.winv[(f & 1) ? NQ - 1 : 0]);
+        } else {
+            const double *sg = P.sgeo + ((e * NF + f) * (DIM + 1)) * NFP + m;
+            R nv[DIM];
+#pragma unroll
+            for (int cc = 0; cc < DIM; ++cc)
+                nv[cc] = R(sg[cc * NFP]);
+            S::nflux_general(law, nv, q1, qp, nf);
+            lift = R(sg[DIM * NFP] * MJI);
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            acc[s] = acc[s] - lift * nf[s];
+    }
+
+    // ---- update
+    const R bdt = R(P.b_dt);
+    R *qo = static_cast<R *>(P.q_out);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        dq[(e * NS + s) * NP + node] = acc[s];
+        if (P.write_q)
+            qo[(e * NS + s) * NP + node] = q1[s] + bdt * acc[s];
+    }
+}
+
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF>
+int launch_fd(const StageParams &P, void *stream, int)
+{
+    using T = FD<DIM, NQ, LAW, R, GEO, VF>;
+    if (P.K <= 0)
+        return 0;
+    const int64_t blocks = (P.K + T::CPB - 1) / T::CPB;
+    hipLaunchKernelGGL((dg_fd_kernel<DIM, NQ, LAW, R, GEO, VF>), dim3((unsigned)blocks),
+                       dim3(T::THREADS), 0, static_cast<hipStream_t>(stream), P);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NQ>
+StageLauncher pick_fd(const KernelKey &k)
+{
+    using R = FD_REAL;
+    constexpr int DIM = FD_DIM;
+    if constexpr (IPow<NQ, DIM>::v > 512) {
+        return nullptr;
+    } else {
+        const bool gen = k.geo == GEO_GENERAL;
+        if (k.law == LAW_ADVECTION) {
+            if (k.vflux != FLUX_CENTRAL)
+                return nullptr;
+            return gen ? &launch_fd<DIM, NQ, LAW_ADVECTION, R, GEO_GENERAL, FLUX_CENTRAL>
+                       : &launch_fd<DIM, NQ, LAW_ADVECTION, R, GEO_AFFINE, FLUX_CENTRAL>;
+        }
+        if (k.vflux == FLUX_CENTRAL)
+            return gen ? &launch_fd<DIM, NQ, LAW_EULER, R, GEO_GENERAL, FLUX_CENTRAL>
+                       : &launch_fd<DIM, NQ, LAW_EULER, R, GEO_AFFINE, FLUX_CENTRAL>;
+        if (k.vflux == FLUX_EC)
+            return gen ? &launch_fd<DIM, NQ, LAW_EULER, R, GEO_GENERAL, FLUX_EC>
+                       : &launch_fd<DIM, NQ, LAW_EULER, R, GEO_AFFINE, FLUX_EC>;
+        return nullptr;
+    }
+}
+
+}  // namespace
+
+StageLauncher FD_FN(const KernelKey &k)
+{
+    if (k.dim != FD_DIM || k.real_bytes != (int)sizeof(FD_REAL) || k.flux != FLUX_RUSANOV)
+        return nullptr;
+    switch (k.nq) {
+    case 2: return pick_fd<2>(k);
+    case 3: return pick_fd<3>(k);
+    case 4: return pick_fd<4>(k);
+    case 5: return pick_fd<5>(k);
+    case 6: return pick_fd<6>(k);
+    case 7: return pick_fd<7>(k);
+    case 8: return pick_fd<8>(k);
+    default: return nullptr;
+    }
+}
+
+}  // namespace rv

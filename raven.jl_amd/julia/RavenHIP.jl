@@ -21,6 +21,9 @@ const RV_LAW_ADVECTION = Int32(1)
 const RV_LAW_EULER = Int32(2)
 const RV_FLUX_RUSANOV = Int32(1)
 const RV_FLUX_CENTRAL = Int32(2)
+const RV_FLUX_EC = Int32(3)
+const RV_FORM_WEAK = Int32(0)
+const RV_FORM_FLUX_DIFFERENCING = Int32(1)
 const RV_F64 = Int32(1)
 const RV_F32 = Int32(2)
 
@@ -47,6 +50,8 @@ struct RvDesc
     dtoc::Ptr{Int64}
     ncontinuous::Int64
     stream::Ptr{Cvoid}
+    volume_form::Int32
+    volume_flux::Int32
 end
 
 mutable struct HIPDGSEM{L}
@@ -66,15 +71,20 @@ lawid(::EulerLaw) = RV_LAW_EULER
 lawid(::AdvectionLaw) = RV_LAW_ADVECTION
 fluxid(::RusanovFlux) = RV_FLUX_RUSANOV
 fluxid(::CentralFlux) = RV_FLUX_CENTRAL
+fluxid(::EntropyConservativeFlux) = RV_FLUX_EC
+formid(::WeakForm) = (RV_FORM_WEAK, Int32(0))
+formid(f::FluxDifferencingForm) = (RV_FORM_FLUX_DIFFERENCING, fluxid(f.volume_numericalflux))
 
 """
-    HIPDGSEM(; law, grid, surface_numericalflux, dtype = Float64, stream = C_NULL)
+    HIPDGSEM(; law, grid, surface_numericalflux, volume_form = WeakForm(),
+             dtype = Float64, stream = C_NULL)
 
 Same keywords as `DGSEM` (dgsem.jl:46-81).  Host copies of the grid arrays
 are read once (`Array(...)` of device arrays is fine); the library keeps
 device copies and per-face neighbour records.
 """
-function HIPDGSEM(; law, grid, surface_numericalflux, dtype = Float64, stream = C_NULL)
+function HIPDGSEM(; law, grid, surface_numericalflux, volume_form = WeakForm(),
+                  dtype = Float64, stream = C_NULL)
     cell = referencecell(grid)
     dim = ndims(cell)
     Nq = size(cell, 1)
@@ -92,11 +102,12 @@ function HIPDGSEM(; law, grid, surface_numericalflux, dtype = Float64, stream = 
     γ = law isa EulerLaw ? Float64(constants(law).γ) : 1.4
     handle = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve D w metrics surf vmapM vmapP bc dtoc begin
-        desc = RvDesc(1, dim, Nq, lawid(law), fluxid(surface_numericalflux),
+        form, vflux = formid(volume_form)
+        desc = RvDesc(2, dim, Nq, lawid(law), fluxid(surface_numericalflux),
                       dtype == Float64 ? RV_F64 : RV_F32, 0, 0, length(grid), γ, vel,
                       pointer(D), pointer(w), pointer(metrics), pointer(surf),
                       pointer(vmapM), pointer(vmapP), pointer(bc), pointer(dtoc),
-                      numcnodes(grid), stream)
+                      numcnodes(grid), stream, form, vflux)
         check(ccall((:rv_create, libravenhip), Cint, (Ref{RvDesc}, Ref{Ptr{Cvoid}}),
                     desc, handle), C_NULL)
     end

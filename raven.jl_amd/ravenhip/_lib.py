@@ -10,10 +10,11 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libravenhip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 RV_OK = 0
 RV_LAW_ADVECTION, RV_LAW_EULER = 1, 2
-RV_FLUX_RUSANOV, RV_FLUX_CENTRAL = 1, 2
+RV_FLUX_RUSANOV, RV_FLUX_CENTRAL, RV_FLUX_EC = 1, 2, 3
+RV_FORM_WEAK, RV_FORM_FLUX_DIFFERENCING = 0, 1
 RV_F64, RV_F32 = 1, 2
 RV_GEOM_AUTO, RV_GEOM_AFFINE, RV_GEOM_GENERAL = 0, 1, 2
 RV_LSRK54, RV_LSRK144 = 54, 144
@@ -37,6 +38,7 @@ class RvDesc(ctypes.Structure):
         ("surface_metrics", c_f64p), ("vmapM", c_i64p), ("vmapP", c_i64p),
         ("boundary_codes", c_i64p), ("dtoc", c_i64p), ("ncontinuous", ctypes.c_int64),
         ("stream", ctypes.c_void_p),
+        ("volume_form", ctypes.c_int32), ("volume_flux", ctypes.c_int32),
     ]
 
 
@@ -47,6 +49,7 @@ class RvBrickDesc(ctypes.Structure):
         ("n", ctypes.c_int32 * 3), ("periodic", ctypes.c_int32 * 3), ("level", ctypes.c_int32),
         ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3), ("gamma", ctypes.c_double),
         ("velocity", ctypes.c_double * 3), ("stream", ctypes.c_void_p),
+        ("volume_form", ctypes.c_int32), ("volume_flux", ctypes.c_int32),
     ]
 
 

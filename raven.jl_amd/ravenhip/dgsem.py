@@ -47,6 +47,29 @@ class CentralFlux:
     flux_id = L.RV_FLUX_CENTRAL
 
 
+class EntropyConservativeFlux:
+    """EntropyConservativeFlux (numericalfluxes.jl:56; Euler: euler.jl:150-184);
+    a volume (two-point) flux of FluxDifferencingForm."""
+    flux_id = L.RV_FLUX_EC
+
+
+class WeakForm:
+    """WeakForm() (dgsem.jl:7): the flux-divergence volume term."""
+    form_id = L.RV_FORM_WEAK
+    flux_id = 0
+
+
+class FluxDifferencingForm:
+    """FluxDifferencingForm(volume_numericalflux) (dgsem.jl:8-17): the volume
+    term from a two-point flux (CentralFlux, or EntropyConservativeFlux for
+    Euler), direction by direction as the :per_dir kernel (dgsem.jl:514-608)."""
+    form_id = L.RV_FORM_FLUX_DIFFERENCING
+
+    def __init__(self, volume_numericalflux):
+        self.volume_numericalflux = volume_numericalflux
+        self.flux_id = volume_numericalflux.flux_id
+
+
 _DTYPES = {"f64": L.RV_F64, "f32": L.RV_F32}
 _GEOMS = {"auto": L.RV_GEOM_AUTO, "affine": L.RV_GEOM_AFFINE, "general": L.RV_GEOM_GENERAL}
 
@@ -71,10 +94,11 @@ class DGSEM:
     """
 
     def __init__(self, *, law, grid, surface_numericalflux=None, dtype="f64", geometry="auto",
-                 directions=None, stream=None, _handle=None):
+                 directions=None, stream=None, volume_form=None, _handle=None):
         self.law = law
         self.grid = grid
         self.dtype = dtype
+        self.volume_form = volume_form or WeakForm()
         flux = surface_numericalflux or RusanovFlux()
         if _handle is not None:
             self._h = _handle
@@ -101,6 +125,7 @@ class DGSEM:
             d.vmapM, d.vmapP, d.boundary_codes, d.dtoc = (L.ptr(a, ctypes.c_int64) for a in ikeep)
             d.ncontinuous = grid.ncontinuous
             d.stream = stream if stream is not None else _current_stream()
+            d.volume_form, d.volume_flux = self.volume_form.form_id, self.volume_form.flux_id
             h = L.H()
             L.check(L.lib().rv_create(ctypes.byref(d), ctypes.byref(h)))
             self._h = h
@@ -112,15 +137,19 @@ class DGSEM:
         self.Np = info.nq ** info.dim
 
     @classmethod
-    def brick(cls, gm, law, surface_numericalflux=None, dtype="f64", stream=None):
+    def brick(cls, gm, law, surface_numericalflux=None, dtype="f64", stream=None,
+              volume_form=None):
         """DGSEM on a library-generated uniform brick (no per-node host arrays)."""
         flux = surface_numericalflux or RusanovFlux()
+        vf = volume_form or WeakForm()
         desc = gm.brick_desc(law=law.law_id, dtype=_DTYPES[dtype], flux=flux.flux_id,
                              gamma=law.gamma, velocity=law.u,
-                             stream=stream if stream is not None else _current_stream())
+                             stream=stream if stream is not None else _current_stream(),
+                             volume_form=vf.form_id, volume_flux=vf.flux_id)
         h = L.H()
         L.check(L.lib().rv_create_brick(ctypes.byref(desc), ctypes.byref(h)))
-        return cls(law=law, grid=None, surface_numericalflux=flux, dtype=dtype, _handle=h)
+        return cls(law=law, grid=None, surface_numericalflux=flux, dtype=dtype, volume_form=vf,
+                   _handle=h)
 
     # -- lifetime ----------------------------------------------------------
     def close(self):

@@ -142,7 +142,8 @@ class GridManager:
         return out
 
     def brick_desc(self, law=L.RV_LAW_ADVECTION, dtype=L.RV_F64, flux=L.RV_FLUX_RUSANOV,
-                   gamma=1.4, velocity=(0.0, 0.0, 0.0), stream=None):
+                   gamma=1.4, velocity=(0.0, 0.0, 0.0), stream=None,
+                   volume_form=L.RV_FORM_WEAK, volume_flux=0):
         cg = self.coarsegrid
         d = L.RvBrickDesc()
         d.abi_version = L.ABI_VERSION
@@ -156,6 +157,7 @@ class GridManager:
         d.level = self.min_level
         d.gamma = gamma
         d.stream = stream
+        d.volume_form, d.volume_flux = volume_form, volume_flux
         return d
 
 
