@@ -270,6 +270,39 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     }
 
 
+def run_fluxdiff(R, torch, gm, q0, dt, steps=4):
+    """The C2 problem (32^3 Nq=8 Euler) with FluxDifferencingForm and the
+    entropy-conservative two-point flux, FP64 and FP32: LSRK54 steps timed
+    with events.  The form is compute-bound (3 Nq two-point fluxes per node,
+    two logarithms each), so it reports two-point flux evaluations per second
+    next to DOF-updates/s."""
+    out = {"workload": "32^3 Nq=8 3-D Euler, FluxDifferencingForm(EntropyConservativeFlux()), "
+                       "Rusanov surface flux, LSRK54", "steps": steps}
+    for dtype in ("f64", "f32"):
+        dg = R.DGSEM.brick(gm, R.EulerLaw(3, GAMMA), dtype=dtype,
+                           volume_form=R.FluxDifferencingForm(R.EntropyConservativeFlux()))
+        q = torch.from_numpy(q0).to(dg.torch_dtype).cuda().reshape(dg.K, dg.ns, dg.Np)
+        q = q.contiguous()
+        solver = R.LSRK54(dg, q, dt)
+        R.solve_(q, dt * 0.75, solver)  # warm-up: one step
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        n = R.solve_(q, solver.time + (steps - 0.25) * dt, solver)
+        ev[1].record()
+        torch.cuda.synchronize()
+        sec = ev[0].elapsed_time(ev[1]) / 1e3
+        nodes = dg.K * dg.Np
+        stages = 5 * n
+        out[dtype] = {"ms_per_stage": sec / stages * 1e3,
+                      "dof_updates_per_s": nodes * dg.ns * stages / sec,
+                      "two_point_fluxes_per_s": nodes * 3 * NQ * stages / sec,
+                      "finite": bool(torch.isfinite(q).all())}
+        del dg, q, solver
+    out["fp32_speedup"] = out["f32"]["dof_updates_per_s"] / out["f64"]["dof_updates_per_s"]
+    return out
+
+
 def run_adaptive(R, torch, steps=200, nq=6, seed=99):
     """3-D Euler on a 16^3 brick over [0,10]^3 with a seeded smooth warp,
     refined 2:1-balanced to level 2 inside the sphere of radius 2 at the
@@ -396,6 +429,8 @@ def main():
                     help="skip the 256-solve batched workload")
     ap.add_argument("--no-served", action="store_true",
                     help="skip the served request stream")
+    ap.add_argument("--no-fluxdiff", action="store_true",
+                    help="skip the flux-differencing (entropy-conservative) workload")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="skip the adaptive warped degree-5 FP32/FP64 workload")
     ap.add_argument("--serve-rates", default="8,32,burst",
@@ -501,6 +536,11 @@ def main():
         del r32
     del r64
     torch.cuda.empty_cache()
+    if not args.no_fluxdiff:
+        log("bench: flux differencing (entropy-conservative) on the 32^3 Nq=8 brick")
+        out["fluxdiff"] = run_fluxdiff(R, torch, gm, q0, dt)
+        out["fluxdiff"]["weak_form_ms_per_stage"] = kavg * 1e3
+        torch.cuda.empty_cache()
     if not args.no_adaptive:
         log("bench: adaptive warped 16^3 degree-5 workload")
         out["adaptive"] = run_adaptive(R, torch)

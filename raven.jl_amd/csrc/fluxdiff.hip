@@ -45,54 +45,66 @@ struct TwoPoint {
                     F[r][s] = (f1[s] + f2[s]) / R(2);
             }
         } else {
-            // EntropyConservativeFlux (euler.jl:150-184)
-            const R gm1 = law.gm1;
-            R u1[DIM], u2[DIM], uavg[DIM], fr[DIM];
+            ec_rows(law, q1, prim(law, q1), q2, prim(law, q2), row0, nrows, F);
+        }
+    }
+
+    // Per-node quantities of the entropy-conservative flux: velocity u = rho u / rho,
+    // b = rho / 2p with p = (gamma - 1)(rho e - |rho u|^2 / 2 rho), |u|^2; computed
+    // with the operations of euler.jl:163-175, so staging them per node gives the
+    // values a per-pair evaluation would
+    struct Prim {
+        R u[DIM], b, uu;
+    };
+    __device__ static inline Prim prim(const Law<DIM, LAW, R> &law, const R *q)
+    {
+        Prim P;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                u1[d] = q1[1 + d] / q1[0];
-                u2[d] = q2[1 + d] / q2[0];
-            }
-            R dot1 = q1[1] * q1[1], dot2 = q2[1] * q2[1];
+        for (int d = 0; d < DIM; ++d)
+            P.u[d] = q[1 + d] / q[0];
+        R dot = q[1] * q[1];
 #pragma unroll
-            for (int d = 1; d < DIM; ++d) {
-                dot1 = dot1 + q1[1 + d] * q1[1 + d];
-                dot2 = dot2 + q2[1 + d] * q2[1 + d];
-            }
-            const R p1 = gm1 * (q1[DIM + 1] - dot1 / (R(2) * q1[0]));
-            const R p2 = gm1 * (q2[DIM + 1] - dot2 / (R(2) * q2[0]));
-            const R b1 = q1[0] / (R(2) * p1), b2 = q2[0] / (R(2) * p2);
-            const R ravg = (q1[0] + q2[0]) / R(2);
+        for (int d = 1; d < DIM; ++d)
+            dot = dot + q[1 + d] * q[1 + d];
+        const R p = law.gm1 * (q[DIM + 1] - dot / (R(2) * q[0]));
+        P.b = q[0] / (R(2) * p);
+        P.uu = P.u[0] * P.u[0];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d)
-                uavg[d] = (u1[d] + u2[d]) / R(2);
-            const R bavg = (b1 + b2) / R(2);
-            R uu1 = u1[0] * u1[0], uu2 = u2[0] * u2[0];
+        for (int d = 1; d < DIM; ++d)
+            P.uu = P.uu + P.u[d] * P.u[d];
+        return P;
+    }
+
+    // EntropyConservativeFlux (euler.jl:150-184), rows row0 .. row0 + nrows - 1
+    __device__ static inline void ec_rows(const Law<DIM, LAW, R> &law, const R *q1,
+                                          const Prim &P1, const R *q2, const Prim &P2, int row0,
+                                          int nrows, R (*F)[NS])
+    {
+        R uavg[DIM], fr[DIM];
+        const R ravg = (q1[0] + q2[0]) / R(2);
 #pragma unroll
-            for (int d = 1; d < DIM; ++d) {
-                uu1 = uu1 + u1[d] * u1[d];
-                uu2 = uu2 + u2[d] * u2[d];
-            }
-            const R u2avg = (uu1 + uu2) / R(2);
-            const R rlog = logavg(q1[0], q2[0]);
-            const R blog = logavg(b1, b2);
+        for (int d = 0; d < DIM; ++d)
+            uavg[d] = (P1.u[d] + P2.u[d]) / R(2);
+        const R bavg = (P1.b + P2.b) / R(2);
+        const R u2avg = (P1.uu + P2.uu) / R(2);
+        const R rlog = logavg(q1[0], q2[0]);
+        const R blog = logavg(P1.b, P2.b);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d)
-                fr[d] = uavg[d] * rlog;
-            const R pav = ravg / (R(2) * bavg);
-            const R c = R(1) / (R(2) * gm1 * blog) - u2avg / R(2);
-            for (int r = 0; r < nrows; ++r) {
-                const int d = row0 + r;
-                F[r][0] = fr[d];
+        for (int d = 0; d < DIM; ++d)
+            fr[d] = uavg[d] * rlog;
+        const R pav = ravg / (R(2) * bavg);
+        const R c = R(1) / (R(2) * law.gm1 * blog) - u2avg / R(2);
+        for (int r = 0; r < nrows; ++r) {
+            const int d = row0 + r;
+            F[r][0] = fr[d];
 #pragma unroll
-                for (int b = 0; b < DIM; ++b)
-                    F[r][1 + b] = uavg[d] * fr[b] + (d == b ? pav : R(0));
-                R v = F[r][1] * uavg[0];
+            for (int b = 0; b < DIM; ++b)
+                F[r][1 + b] = uavg[d] * fr[b] + (d == b ? pav : R(0));
+            R v = F[r][1] * uavg[0];
 #pragma unroll
-                for (int b = 1; b < DIM; ++b)
-                    v = v + F[r][1 + b] * uavg[b];
-                F[r][DIM + 1] = c * fr[d] + v;
-            }
+            for (int b = 1; b < DIM; ++b)
+                v = v + F[r][1 + b] * uavg[b];
+            F[r][DIM + 1] = c * fr[d] + v;
         }
     }
 
@@ -131,6 +143,11 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     using S = typename T::S;
     constexpr int NS = T::NS, NP = T::NP, NFP = T::NFP, NF = T::NF;
     __shared__ R qs[T::CPB][NS][NP];
+    // entropy-conservative flux: per-node primitives of the line partners
+    constexpr bool EC = VF == FLUX_EC && LAW == LAW_EULER;
+    using TP = TwoPoint<DIM, NS, LAW, VF, R>;
+    using PrimT = typename TP::Prim;
+    __shared__ PrimT ps[EC ? T::CPB : 1][EC ? NP : 1];
     const int t = threadIdx.x;
     const int c = t / NP, node = t - c * NP;
     const int64_t e = (int64_t)blockIdx.x * T::CPB + c;
@@ -138,11 +155,16 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     const Law<DIM, LAW, R> law(P);
     const R *qin = static_cast<const R *>(P.q_in);
     R q1[NS], acc[NS];
+    PrimT p1{};
     if (active) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             q1[s] = qin[(e * NS + s) * NP + node];
             qs[c][s][node] = q1[s];
+        }
+        if constexpr (EC) {
+            p1 = TP::prim(law, q1);
+            ps[c][node] = p1;
         }
     }
     __syncthreads();
@@ -191,7 +213,10 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
             const R Dnd = R(MJI * P.Dt[id * NQ + n]);   // MJI D[n, id]
             if constexpr (GEO == GEO_GENERAL) {
                 R F[DIM][NS], lg2[DIM];
-                TwoPoint<DIM, NS, LAW, VF, R>::rows(law, q1, q2, 0, DIM, F);
+                if constexpr (EC)
+                    TP::ec_rows(law, q1, p1, q2, ps[c][ild], 0, DIM, F);
+                else
+                    TP::rows(law, q1, q2, 0, DIM, F);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)
                     lg2[d] = R(P.vgeo[((e * DIM + dir) * DIM + d) * NP + ild]);
@@ -204,7 +229,10 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
                     }
             } else {
                 R F[1][NS];
-                TwoPoint<DIM, NS, LAW, VF, R>::rows(law, q1, q2, dir, 1, F);
+                if constexpr (EC)
+                    TP::ec_rows(law, q1, p1, q2, ps[c][ild], dir, 1, F);
+                else
+                    TP::rows(law, q1, q2, dir, 1, F);
                 const R lg2 = R(P.mj[e * NP + ild] * P.affine[e * DIM + dir]);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
