@@ -303,6 +303,51 @@ def run_fluxdiff(R, torch, gm, q0, dt, steps=4):
     return out
 
 
+def run_rlsrk(R, torch, gm, q0, dt, steps=4):
+    """RLSRK54 (relaxation LSRK, odesolvers.jl:72-162) on the C2 problem with
+    the entropy-conservative pairing (EC two-point flux in the volume and on
+    the faces), FP64 and FP32: wall time per step (each step reads back one
+    pair of sums per Newton iteration), the same handle's plain LSRK54 step
+    for comparison, Newton iterations per step and the entropy change over
+    the run relative to eta(0)."""
+    out = {"workload": "32^3 Nq=8 3-D Euler, FluxDifferencingForm(EntropyConservativeFlux()), "
+                       "EntropyConservativeFlux surface flux, RLSRK54", "steps": steps}
+    for dtype in ("f64", "f32"):
+        ec = R.EntropyConservativeFlux()
+        dg = R.DGSEM.brick(gm, R.EulerLaw(3, GAMMA), surface_numericalflux=ec, dtype=dtype,
+                           volume_form=R.FluxDifferencingForm(ec))
+        q = torch.from_numpy(q0).to(dg.torch_dtype).cuda().reshape(dg.K, dg.ns, dg.Np)
+        q = q.contiguous()
+        eta0 = R.entropyintegral(dg, q)
+        solver = R.RLSRK54(dg, q, dt)
+        R.dostep_(q, solver)  # warm-up: one step
+        torch.cuda.synchronize()
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            R.dostep_(q, solver)
+            iters.append(solver.newton_iterations)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        d_eta = (R.entropyintegral(dg, q) - eta0) / abs(eta0)
+        plain = R.LSRK54(dg, q, dt)
+        R.solve_(q, dt * 0.75, plain)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        n = R.solve_(q, plain.time + (steps - 0.25) * dt, plain)
+        torch.cuda.synchronize()
+        sec_plain = time.perf_counter() - t1
+        out[dtype] = {"ms_per_step": sec / steps * 1e3,
+                      "ms_per_lsrk54_step": sec_plain / n * 1e3,
+                      "relaxation_overhead": (sec / steps) / (sec_plain / n) - 1.0,
+                      "newton_iterations_per_step": float(np.mean(iters)),
+                      "gamma": solver.gamma,
+                      "delta_eta_rel": d_eta,
+                      "finite": bool(torch.isfinite(q).all())}
+        del dg, q, solver, plain
+    return out
+
+
 def run_adaptive(R, torch, steps=200, nq=6, seed=99):
     """3-D Euler on a 16^3 brick over [0,10]^3 with a seeded smooth warp,
     refined 2:1-balanced to level 2 inside the sphere of radius 2 at the
@@ -431,6 +476,8 @@ def main():
                     help="skip the served request stream")
     ap.add_argument("--no-fluxdiff", action="store_true",
                     help="skip the flux-differencing (entropy-conservative) workload")
+    ap.add_argument("--no-rlsrk", action="store_true",
+                    help="skip the relaxation RK (RLSRK54) workload")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="skip the adaptive warped degree-5 FP32/FP64 workload")
     ap.add_argument("--serve-rates", default="8,32,burst",
@@ -540,6 +587,10 @@ def main():
         log("bench: flux differencing (entropy-conservative) on the 32^3 Nq=8 brick")
         out["fluxdiff"] = run_fluxdiff(R, torch, gm, q0, dt)
         out["fluxdiff"]["weak_form_ms_per_stage"] = kavg * 1e3
+        torch.cuda.empty_cache()
+    if not args.no_rlsrk:
+        log("bench: relaxation RK (entropy-conservative pairing) on the 32^3 Nq=8 brick")
+        out["rlsrk"] = run_rlsrk(R, torch, gm, q0, dt)
         torch.cuda.empty_cache()
     if not args.no_adaptive:
         log("bench: adaptive warped 16^3 degree-5 workload")

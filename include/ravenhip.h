@@ -51,8 +51,11 @@ enum {
 /* law ids (src/balancelaws/balancelaws/{advection,euler}.jl) */
 enum { RV_LAW_ADVECTION = 1, RV_LAW_EULER = 2 };
 /* numerical fluxes (src/balancelaws/numericalfluxes.jl:39-50, euler.jl:150-184):
- * surface fluxes RUSANOV and CENTRAL; volume (two-point) fluxes of the flux
- * differencing form CENTRAL and EC (EntropyConservativeFlux, Euler only) */
+ * surface fluxes RUSANOV and CENTRAL with either volume form, and with the
+ * flux differencing form also its own volume flux as twopointflux(q-, q+)' n
+ * (numericalfluxes.jl:27-37; EC on faces is Raven's Euler entropy-conservation
+ * pairing); volume (two-point) fluxes of the flux differencing form CENTRAL
+ * and EC (EntropyConservativeFlux, Euler only) */
 enum { RV_FLUX_RUSANOV = 1, RV_FLUX_CENTRAL = 2, RV_FLUX_EC = 3 };
 /* volume form (dgsem.jl:1-17): WeakForm, or FluxDifferencingForm(volume flux)
  * evaluated direction by direction as esvolumeterm_dir! (dgsem.jl:514-608) */
@@ -193,6 +196,24 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
  * may be NULL; *nstages receives 5 or 14.  No handle, no device work. */
 int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstages);
 
+/* One relaxation LSRK step, dostep!(q, ::RLSRK) (odesolvers.jl:117-151), the
+ * solver of RLSRK54 / RLSRK144 (odesolvers.jl:154-162):
+ *   q0 = q;  eta0 = entropyintegral(q0);  dEta = -0
+ *   per stage s: k = L(q);  dq = rka_s dq + k;  dEta += bfull_s dt v(q).k;
+ *                q += rkb_s dt dq
+ *   k = q - q0;  gamma: eta(q0 + gamma k) - eta0 - gamma dEta = 0;  q = q0 + gamma k
+ * bfull are the Butcher weights of the scheme (odesolvers.jl:96-101).  The
+ * root is found by Newton's method from *gamma (the previous step's root, 1 at
+ * the start), stopping at |residual| <= 10 eps of the state type (NLsolve's
+ * ftol there), or when the update falls below 2 eps |gamma|, after at most 50
+ * iterations; *gamma receives the root and *iterations (may be NULL) the
+ * Newton iterations taken.  dq carries between steps like rv_lsrk_stage's
+ * (not read at a stage with rka = 0); k and q0 are work arrays.  All four
+ * arrays have the state's shape and must be distinct.  Blocks on the handle's
+ * stream (each Newton iteration reads back its two sums). */
+int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme, double time,
+                  double dt, double *gamma, int32_t *iterations);
+
 /* --- adaptive forests ---------------------------------------------------- */
 
 /* The leaves of GridManager(cell, brick(...); min_level) after adapt!(gm,
@@ -280,6 +301,12 @@ int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, 
 int rv_weightednorm(rv_handle *h, const void *q, double *out);
 /* Compensated conservation integrals sum(MJ .* q_s): out[s], s < Ns. */
 int rv_integrals(rv_handle *h, const void *q, double *out);
+/* entropyintegral(dg, q) = sum(MJ .* entropy.(q)) (measures.jl:23-27) and
+ * entropyproduct(dg, p, q) = sum(MJ .* entropyvariables.(p) . q)
+ * (measures.jl:28-33), entropy and entropy variables of euler.jl:70-91 /
+ * advection.jl:20-21, in FP64 with compensated sums in a fixed order. */
+int rv_entropy_integral(rv_handle *h, const void *q, double *out);
+int rv_entropy_product(rv_handle *h, const void *p, const void *q, double *out);
 
 /* --- GatherScatter (gatherscatter.jl:1-103) ----------------------------- */
 

@@ -754,6 +754,7 @@ int orc_lsrk_coefficients(int scheme, double *a, double *b, double *c)
 struct orc_dg {
     int dim, nq, ns, law, nthreads;
     int volume_form, volume_flux;  /* 0 weak / 1 flux differencing; 2 central / 3 EC */
+    int surface_flux;              /* 1 Rusanov (default), 2 central, 3 EC */
     int64_t K, Np, nfp;
     double gamma, u[3];
     const double *D, *metrics, *surf;
@@ -785,6 +786,7 @@ orc_dg *orc_dg_new(int dim, int nq, int64_t K, int law, double gamma, const doub
         dg->Np *= nq;
     dg->nfp = face_nodes(dim, nq);
     dg->nthreads = 0;
+    dg->surface_flux = 1;
     /* MJI = 1 ./ MJ with MJ = wJ (dgsem.jl:55-56) */
     const int L = dim * dim + 2;
     dg->MJI = malloc(sizeof(double) * dg->Np * K);
@@ -809,6 +811,8 @@ void orc_dg_set_volume_form(orc_dg *dg, int form, int volume_flux)
     dg->volume_form = form;
     dg->volume_flux = volume_flux;
 }
+
+void orc_dg_set_surface_flux(orc_dg *dg, int flux) { dg->surface_flux = flux; }
 void orc_dg_set_threads(orc_dg *dg, int nthreads) { dg->nthreads = nthreads; }
 
 /* flux(law, q) as a dim x Ns matrix F[dd + dim*s]
@@ -910,10 +914,10 @@ static double logavg(double a, double b)
 /* twopointflux as a dim x Ns matrix F[d + dim*s]:
  * CentralFlux (numericalfluxes.jl:39-42): (F(q1) + F(q2)) / 2;
  * EntropyConservativeFlux for Euler (euler.jl:150-184) */
-static void twopoint(const orc_dg *dg, const double *q1, const double *q2, double *F)
+static void twopoint(const orc_dg *dg, int flux, const double *q1, const double *q2, double *F)
 {
     const int dim = dg->dim, ns = dg->ns;
-    if (dg->volume_flux == 2 || dg->law == 1) {
+    if (flux == 2 || dg->law == 1) {
         double F1[15], F2[15];
         law_flux(dg, q1, F1);
         law_flux(dg, q2, F2);
@@ -1003,7 +1007,7 @@ static void fd_volume(orc_dg *dg, double *dq, const double *q, int increment, in
                     const double MJ2 = dg->metrics[ild + Np * ((L - 1) + (int64_t)L * e)];
                     for (int d = 0; d < dim; ++d)
                         lg2[d] = MJ2 * dg->metrics[ild + Np * ((dir + dim * d) + (int64_t)L * e)];
-                    twopoint(dg, q1, q2, F);
+                    twopoint(dg, dg->volume_flux, q1, q2, F);
                     for (int s = 0; s < ns; ++s) {
                         double Ddn = MJI * D[id + nq * n];
                         double Dnd = MJI * D[n + nq * id];
@@ -1112,7 +1116,19 @@ surface:;
                     for (int s = 0; s < ns; ++s)
                         qp[s] = qm[s];
                 }
-                rusanov(dg, nv, qm, qp, nf);
+                if (dg->surface_flux == 1) {
+                    rusanov(dg, nv, qm, qp, nf);
+                } else {
+                    /* surfaceflux default: twopointflux(q-, q+)' n (numericalfluxes.jl:27-37) */
+                    double F[15];
+                    twopoint(dg, dg->surface_flux, qm, qp, F);
+                    for (int s = 0; s < ns; ++s) {
+                        double v = F[0 + dim * s] * nv[0];
+                        for (int d = 1; d < dim; ++d)
+                            v = v + F[d + dim * s] * nv[d];
+                        nf[s] = v;
+                    }
+                }
                 int64_t ee = idm / Np, nn = idm % Np;
                 double MJI = dg->MJI[idm];
                 for (int s = 0; s < ns; ++s) {
@@ -1270,6 +1286,151 @@ int64_t orc_dg_solve(orc_dg *dg, int scheme, double *q, double dt, double t0,
     if (t_out)
         *t_out = time;
     return step;
+}
+
+/* entropy and entropy variables (euler.jl:70-91, advection.jl:20-21) */
+static double law_entropy(const orc_dg *dg, const double *q)
+{
+    const int dim = dg->dim;
+    if (dg->law == 1)
+        return q[0] * q[0] / 2;
+    const double g = dg->gamma;
+    double dot = q[1] * q[1];
+    for (int d = 1; d < dim; ++d)
+        dot = dot + q[1 + d] * q[1 + d];
+    double p = (g - 1) * (q[dim + 1] - dot / (2 * q[0]));
+    double s = log(p / pow(q[0], g));
+    return -q[0] * s / (g - 1);
+}
+
+static void law_entropy_variables(const orc_dg *dg, const double *q, double *v)
+{
+    const int dim = dg->dim;
+    if (dg->law == 1) {
+        v[0] = q[0];
+        return;
+    }
+    const double g = dg->gamma;
+    double dot = q[1] * q[1];
+    for (int d = 1; d < dim; ++d)
+        dot = dot + q[1 + d] * q[1 + d];
+    double p = (g - 1) * (q[dim + 1] - dot / (2 * q[0]));
+    double s = log(p / pow(q[0], g));
+    double b = q[0] / (2 * p);
+    double u[3] = {0, 0, 0}, uu;
+    for (int d = 0; d < dim; ++d)
+        u[d] = q[1 + d] / q[0];
+    uu = u[0] * u[0];
+    for (int d = 1; d < dim; ++d)
+        uu = uu + u[d] * u[d];
+    v[0] = (g - s) / (g - 1) - uu * b;
+    for (int d = 0; d < dim; ++d)
+        v[1 + d] = 2 * b * u[d];
+    v[dim + 1] = -2 * b;
+}
+
+/* entropyintegral(dg, q) = sum MJ eta(q) (measures.jl:23-27) */
+double orc_dg_entropy_integral(const orc_dg *dg, const double *q)
+{
+    const int dim = dg->dim, ns = dg->ns, L = dim * dim + 2;
+    const int64_t Np = dg->Np;
+    double sum = 0;
+    for (int64_t e = 0; e < dg->K; ++e)
+        for (int64_t n = 0; n < Np; ++n) {
+            double qv[5];
+            for (int s = 0; s < ns; ++s)
+                qv[s] = q[n + Np * (s + (int64_t)ns * e)];
+            sum += dg->metrics[n + Np * ((L - 1) + (int64_t)L * e)] * law_entropy(dg, qv);
+        }
+    return sum;
+}
+
+/* entropyproduct(dg, p, q) = sum MJ v(p) . q (measures.jl:28-33) */
+double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q)
+{
+    const int dim = dg->dim, ns = dg->ns, L = dim * dim + 2;
+    const int64_t Np = dg->Np;
+    double sum = 0;
+    for (int64_t e = 0; e < dg->K; ++e)
+        for (int64_t n = 0; n < Np; ++n) {
+            double pv[5], v[5], dot;
+            for (int s = 0; s < ns; ++s)
+                pv[s] = p[n + Np * (s + (int64_t)ns * e)];
+            law_entropy_variables(dg, pv, v);
+            dot = v[0] * q[n + Np * (0 + (int64_t)ns * e)];
+            for (int s = 1; s < ns; ++s)
+                dot = dot + v[s] * q[n + Np * (s + (int64_t)ns * e)];
+            sum += dg->metrics[n + Np * ((L - 1) + (int64_t)L * e)] * dot;
+        }
+    return sum;
+}
+
+int orc_rlsrk_weights(int scheme, double *bfull);
+
+/* One relaxation LSRK step, dostep!(q, ::RLSRK) (odesolvers.jl:117-151): the
+ * LSRK stages with the entropy production dEta = sum_s bfull_s dt v(q).k, then
+ * gamma solving eta(q0 + gamma k) - eta0 - gamma dEta = 0 (k = q - q0) from the
+ * previous gamma.  Raven calls NLsolve (trust region, ftol = 10 eps); this is
+ * Newton's method on the same residual and derivative r'(gamma) = v(q).k - dEta,
+ * stopped at the same |residual| <= 10 eps, or when the Newton update is
+ * below 2 eps |gamma| (the residual of a large integral can sit above the
+ * absolute 10 eps in rounding alone), at most 50 iterations.  bfull_s are the
+ * standard RK weights of the low-storage scheme (odesolvers.jl:96-101).
+ * Returns the number of Newton iterations, -1 when the update is not finite. */
+int orc_dg_rlsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double *k, double *q0,
+                      double time, double dt, double *gamma)
+{
+    double A[14], B[14], C[14];
+    const int nst = orc_lsrk_coefficients(scheme, A, B, C);
+    const int64_t n = dg->Np * dg->ns * dg->K;
+    double bfull[14];
+    orc_rlsrk_weights(scheme, bfull);
+    memcpy(q0, q, sizeof(double) * n);
+    const double eta0 = orc_dg_entropy_integral(dg, q0);
+    double deta = -0.0;
+    for (int s = 0; s < nst; ++s) {
+        for (int64_t i = 0; i < n; ++i)
+            dq[i] *= A[s];
+        orc_dg_rhs(dg, k, q, time + C[s] * dt, 0);
+        for (int64_t i = 0; i < n; ++i)
+            dq[i] += k[i];
+        deta += bfull[s] * dt * orc_dg_entropy_product(dg, q, k);
+        for (int64_t i = 0; i < n; ++i)
+            q[i] += B[s] * dt * dq[i];
+    }
+    for (int64_t i = 0; i < n; ++i)
+        k[i] = q[i] - q0[i];
+    double g = *gamma;
+    int it = 0;
+    for (; it < 50; ++it) {
+        for (int64_t i = 0; i < n; ++i)
+            q[i] = q0[i] + g * k[i];
+        const double F = orc_dg_entropy_integral(dg, q) - eta0 - g * deta;
+        if (fabs(F) <= 10 * DBL_EPSILON)
+            break;
+        const double step = F / (orc_dg_entropy_product(dg, q, k) - deta);
+        if (!isfinite(step))
+            return -1;
+        if (!(fabs(step) > 2 * DBL_EPSILON * fabs(g)))
+            break;
+        g = g - step;
+    }
+    *gamma = g;
+    for (int64_t i = 0; i < n; ++i)
+        q[i] = q0[i] + g * k[i];
+    return it;
+}
+
+/* bfull of dostep!(q, ::RLSRK) (odesolvers.jl:96-101): the Butcher weights
+ * b_i = rka_{i+1} b_{i+1} + rkb_i, b_last = rkb_last */
+int orc_rlsrk_weights(int scheme, double *bfull)
+{
+    double A[14], B[14], C[14];
+    const int nst = orc_lsrk_coefficients(scheme, A, B, C);
+    bfull[nst - 1] = B[nst - 1];
+    for (int i = nst - 2; i >= 0; --i)
+        bfull[i] = A[i + 1] * bfull[i + 1] + B[i];
+    return nst;
 }
 
 void orc_dg_weightednorm(const orc_dg *dg, const double *q, double *comp)

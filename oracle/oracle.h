@@ -116,6 +116,18 @@ void orc_dg_mortar_add(const orc_dg *dg, const int32_t *kind, const int64_t *par
  * volume_flux 2 = CentralFlux or 3 = EntropyConservativeFlux (Euler),
  * evaluated direction by direction as esvolumeterm_dir! (dgsem.jl:514-608). */
 void orc_dg_set_volume_form(orc_dg *dg, int form, int volume_flux);
+/* surface numerical flux: 1 Rusanov, 2 central, 3 entropy conservative
+ * (the last two as twopointflux(q-, q+)' n, numericalfluxes.jl:27-37) */
+void orc_dg_set_surface_flux(orc_dg *dg, int flux);
+/* sum MJ eta(q) and sum MJ v(p).q (measures.jl:23-33) */
+double orc_dg_entropy_integral(const orc_dg *dg, const double *q);
+double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q);
+/* one RLSRK step (odesolvers.jl:117-151); gamma in: previous root, out: this one;
+ * dq carries between steps, k and q0 are work arrays of the state's size */
+int orc_dg_rlsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double *k, double *q0,
+                      double time, double dt, double *gamma);
+/* Butcher weights of the low-storage scheme (odesolvers.jl:96-101); returns #stages */
+int orc_rlsrk_weights(int scheme, double *bfull);
 
 /* weightednorm(dg, q) component norms (measures.jl:6-14), p = 2. */
 void orc_dg_weightednorm(const orc_dg *dg, const double *q, double *comp);

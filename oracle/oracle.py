@@ -65,8 +65,25 @@ def lib():
                                    ctypes.POINTER(ctypes.c_double)]
         L.orc_dg_solve.restype = ctypes.c_int64
         L.orc_dg_weightednorm.argtypes = [ctypes.c_void_p, f64p, f64p]
+        L.orc_dg_set_surface_flux.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_dg_entropy_integral.argtypes = [ctypes.c_void_p, f64p]
+        L.orc_dg_entropy_integral.restype = ctypes.c_double
+        L.orc_dg_entropy_product.argtypes = [ctypes.c_void_p, f64p, f64p]
+        L.orc_dg_entropy_product.restype = ctypes.c_double
+        L.orc_dg_rlsrk_step.argtypes = [ctypes.c_void_p, ctypes.c_int, f64p, f64p, f64p, f64p,
+                                        ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_double)]
+        L.orc_dg_rlsrk_step.restype = ctypes.c_int
+        L.orc_rlsrk_weights.argtypes = [ctypes.c_int, f64p]
+        L.orc_rlsrk_weights.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def rlsrk_weights(scheme=54):
+    b = np.zeros(14)
+    n = lib().orc_rlsrk_weights(scheme, b)
+    return b[:n].copy()
 
 
 def gll(nq):
@@ -137,12 +154,16 @@ class BrickGrid:
         return self.metrics.reshape(self.K, L, self.Np)[:, L - 1, :]
 
 
+FLUXES = {"rusanov": 1, "central": 2, "ec": 3}
+
+
 class DG:
     """Oracle DGSEM operator for law 'advection' or 'euler'."""
 
-    def __init__(self, grid, law, gamma=1.4, u=None, volume_flux=None):
+    def __init__(self, grid, law, gamma=1.4, u=None, volume_flux=None, surface_flux="rusanov"):
         """volume_flux None: WeakForm; "central" or "ec": FluxDifferencingForm
-        with CentralFlux or EntropyConservativeFlux (dgsem.jl:1-17)."""
+        with CentralFlux or EntropyConservativeFlux (dgsem.jl:1-17).
+        surface_flux "rusanov", "central" or "ec" (numericalfluxes.jl:27-37)."""
         self.grid = grid
         self.law = {"advection": 1, "euler": 2}[law]
         u = np.zeros(3) if u is None else np.asarray(u, dtype=np.float64)
@@ -154,6 +175,7 @@ class DG:
         self.ns = lib().orc_dg_ns(self.h)
         if volume_flux is not None:
             lib().orc_dg_set_volume_form(self.h, 1, {"central": 2, "ec": 3}[volume_flux])
+        lib().orc_dg_set_surface_flux(self.h, FLUXES[surface_flux])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -185,6 +207,35 @@ class DG:
                              np.ascontiguousarray(upper, dtype=np.float64).reshape(-1),
                              np.ascontiguousarray(surf_full, dtype=np.float64).reshape(-1)]
         lib().orc_dg_mortar_add(self.h, *self._mortar_keep, q, dq)
+
+    def entropy_integral(self, q):
+        return lib().orc_dg_entropy_integral(self.h, q)
+
+    def entropy_product(self, p, q):
+        return lib().orc_dg_entropy_product(self.h, p, q)
+
+    def rlsrk_solve(self, q, dt, timeend, t0=0.0, scheme=54, after_step=None):
+        """solve!(q, timeend, RLSRK) (odesolvers.jl:9-31, 117-151): the final
+        step is shortened to end at timeend; after_step(step, time, q, gamma)
+        after every step.  Returns (steps, end time, last gamma)."""
+        n = q.size
+        dq, k, q0 = np.zeros(n), np.zeros(n), np.zeros(n)
+        gamma = ctypes.c_double(1.0)
+        time, step = t0, 0
+        while True:
+            step += 1
+            final = time + dt >= timeend
+            if final:
+                dt = timeend - time
+            it = lib().orc_dg_rlsrk_step(self.h, scheme, q, dq, k, q0, time, dt,
+                                         ctypes.byref(gamma))
+            if it < 0:
+                raise FloatingPointError("relaxation Newton iteration broke down")
+            if after_step is not None:
+                after_step(step, time, q, gamma.value)
+            time += dt
+            if final:
+                return step, time, gamma.value
 
     def weightednorm(self, q):
         comp = np.zeros(self.ns)

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -78,6 +79,7 @@ struct rv_handle {
     double *MJ = nullptr;
     double *cellJ = nullptr;     // brick handles: per-cell J that MJ is filled from
     double *partials = nullptr;  // reduction scratch
+    double *relax_partials = nullptr;  // entropy passes: RELAX_SLOTS slots of 4*RELAX_BLOCKS
     void *qtmp = nullptr;        // ping-pong state of rv_lsrk_run
     int64_t *dtoc = nullptr;     // 0-based
     int64_t *ctod_ptr = nullptr;
@@ -192,7 +194,7 @@ void free_handle(rv_handle *h)
                     h->sgeo,   h->Dt,           h->Dw,       h->winv,     h->MJ,
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
                     h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
-                    h->cellJ};
+                    h->cellJ,  h->relax_partials};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -557,8 +559,9 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
         return fail(h, RV_E_UNSUPPORTED,
                     "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +
                         " law=" + std::to_string(h->law) + " dtype=" + std::to_string(h->dtype) +
-                        (h->vflux ? " (flux differencing: Nq^dim <= 512, Rusanov surface flux)"
-                                  : ""));
+                        (h->vflux ? " (flux differencing: Nq^dim <= 512, Rusanov surface flux or"
+                                    " the volume flux)"
+                                  : " (weak form: Rusanov or central surface flux)"));
     return RV_OK;
 }
 
@@ -859,6 +862,68 @@ int reduce(rv_handle *h, const void *q, int power, double *out)
     return RV_OK;
 }
 
+// Entropy passes (entropy.hip): slot 0 holds Newton evaluations and the
+// diagnostics, slot 1 the entropy integral of q0, slots 2.. the stages
+constexpr int RELAX_SLOTS = 16;
+// Newton iterations of the relaxation root per step at most
+constexpr int RELAX_MAX_NEWTON = 50;
+
+int ensure_relax(rv_handle *h)
+{
+    if (h->law != RV_LAW_ADVECTION && h->law != RV_LAW_EULER)
+        return fail(h, RV_E_UNSUPPORTED, "no entropy for this law");
+    if (h->relax_partials)
+        return RV_OK;
+    return dalloc(h, &h->relax_partials, (size_t)4 * RELAX_BLOCKS * RELAX_SLOTS);
+}
+
+RelaxParams relax_params(rv_handle *h, int mode, int slot)
+{
+    RelaxParams P{};
+    P.mode = mode;
+    P.law = h->law == RV_LAW_ADVECTION ? LAW_ADVECTION : LAW_EULER;
+    P.dim = h->dim;
+    P.real_bytes = h->dtype == RV_F64 ? 8 : 4;
+    P.Np = h->Np;
+    P.K = h->K;
+    P.gas_gamma = h->gamma;
+    P.MJ = h->MJ;
+    P.partial = h->relax_partials + (size_t)4 * RELAX_BLOCKS * slot;
+    return P;
+}
+
+int relax(rv_handle *h, const RelaxParams &P)
+{
+    if (launch_relax(P, h->stream) != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(h, RV_E_HIP, std::string("entropy pass launch failed: ") + hipGetErrorString(e));
+    }
+    return RV_OK;
+}
+
+// Neumaier sums of slots [0, nslots) in workgroup order: out[2*slot] the
+// entropy integral, out[2*slot + 1] the entropy product
+int relax_read(rv_handle *h, int nslots, double *out)
+{
+    std::vector<double> part((size_t)4 * RELAX_BLOCKS * nslots);
+    HIP_TRY(h, hipMemcpyAsync(part.data(), h->relax_partials, part.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int slot = 0; slot < nslots; ++slot)
+        for (int which = 0; which < 2; ++which) {
+            double sum = 0, comp = 0;
+            for (int b = 0; b < RELAX_BLOCKS; ++b)
+                for (int k = 0; k < 2; ++k) {
+                    const double x = part[(size_t)4 * (RELAX_BLOCKS * slot + b) + 2 * which + k];
+                    const double t = sum + x;
+                    comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
+                    sum = t;
+                }
+            out[2 * slot + which] = sum + comp;
+        }
+    return RV_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -898,7 +963,8 @@ int rv_create(const rv_desc *desc, rv_handle **out)
     if (desc->dim < 1 || desc->dim > 3 || desc->nq < 2 || desc->nq > 16 || desc->ncells < 0 ||
         (desc->law != RV_LAW_ADVECTION && desc->law != RV_LAW_EULER) ||
         (desc->dtype != RV_F64 && desc->dtype != RV_F32) ||
-        (desc->flux != 0 && desc->flux != RV_FLUX_RUSANOV && desc->flux != RV_FLUX_CENTRAL))
+        (desc->flux != 0 && desc->flux != RV_FLUX_RUSANOV && desc->flux != RV_FLUX_CENTRAL &&
+         !(desc->flux == RV_FLUX_EC && desc->law == RV_LAW_EULER)))
         return fail(nullptr, RV_E_ARG, "invalid descriptor field (dim, nq, ncells, law, dtype or flux)");
     rv_handle *h = new rv_handle();
     h->vflux = vflux;
@@ -1121,7 +1187,9 @@ int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *par
     MortarLauncher ml = find_mortar_launcher(h->dim, h->nq, h->law, h->dtype == RV_F32 ? 4 : 8,
                                              h->flux);
     if (!ml)
-        return fail(h, RV_E_UNSUPPORTED, "no mortar kernel for this (dim, Nq)");
+        return fail(h, RV_E_UNSUPPORTED,
+                    "no mortar kernel for this (dim, Nq, surface flux): 2-D or 3-D, Nq <= 8,"
+                    " Rusanov or central");
     std::vector<double> halves(2 * h->nq * h->nq);
     lobatto_halves(h->nq, halves.data(), halves.data() + h->nq * h->nq);
     const size_t nsg = (size_t)h->K * nf * (h->dim + 1) * h->nfp;
@@ -1311,6 +1379,145 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
         *t_out = time;
     if (nsteps_out)
         *nsteps_out = steps;
+    return RV_OK;
+}
+
+int rv_entropy_integral(rv_handle *h, const void *q, double *out)
+{
+    if (!h || !q || !out)
+        return fail(h, RV_E_ARG, "null argument");
+    if (int rc = ensure_relax(h))
+        return rc;
+    RelaxParams P = relax_params(h, RELAX_ETA, 0);
+    P.q = const_cast<void *>(q);
+    if (int rc = relax(h, P))
+        return rc;
+    double v[2];
+    if (int rc = relax_read(h, 1, v))
+        return rc;
+    *out = v[0];
+    return RV_OK;
+}
+
+int rv_entropy_product(rv_handle *h, const void *p, const void *q, double *out)
+{
+    if (!h || !p || !q || !out)
+        return fail(h, RV_E_ARG, "null argument");
+    if (int rc = ensure_relax(h))
+        return rc;
+    RelaxParams P = relax_params(h, RELAX_PRODUCT, 0);
+    P.q = const_cast<void *>(p);
+    P.k = const_cast<void *>(q);
+    if (int rc = relax(h, P))
+        return rc;
+    double v[2];
+    if (int rc = relax_read(h, 1, v))
+        return rc;
+    *out = v[1];
+    return RV_OK;
+}
+
+int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme, double time,
+                  double dt, double *gamma, int32_t *iterations)
+{
+    (void)time;  // the laws here do not depend on the stage time
+    if (!h || !q || !dq || !k || !q0 || !gamma)
+        return fail(h, RV_E_ARG, "null argument");
+    if (q == dq || q == k || q == q0 || dq == k || dq == q0 || k == q0)
+        return fail(h, RV_E_ARG, "q, dq, k and q0 must be distinct arrays");
+    const double *A, *B, *C;
+    const int nst = lsrk_tables(scheme, &A, &B, &C);
+    if (nst == 0)
+        return fail(h, RV_E_ARG, "unknown LSRK scheme");
+    (void)C;
+    if (int rc = ensure_relax(h))
+        return rc;
+    const bool f32 = h->dtype == RV_F32;
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (f32 ? 4 : 8);
+    auto rnd = [f32](double v) { return f32 ? (double)(float)v : v; };
+    // standard RK weights of the low-storage scheme (odesolvers.jl:96-101)
+    double bfull[14];
+    bfull[nst - 1] = rnd(B[nst - 1]);
+    for (int i = nst - 2; i >= 0; --i)
+        bfull[i] = rnd(rnd(A[i + 1]) * bfull[i + 1] + rnd(B[i]));
+    const double step_dt = rnd(dt);
+
+    // q0 .= q; eta0 = entropyintegral(q0)
+    HIP_TRY(h, hipMemcpyAsync(q0, q, bytes, hipMemcpyDeviceToDevice, h->stream));
+    RelaxParams E = relax_params(h, RELAX_ETA, 1);
+    E.q = q0;
+    if (int rc = relax(h, E))
+        return rc;
+    // stages: k = L(q); dq = a dq + k; product v(q).k into slot 2 + s; q += b dt dq
+    StageParams S = stage_params(h);
+    S.q_in = as_global(q);
+    S.q_out = nullptr;
+    S.dq = as_global(k);
+    S.a = 1.0;
+    S.b_dt = 0.0;
+    S.read_dq = 0;
+    S.write_q = 0;
+    for (int st = 0; st < nst; ++st) {
+        if (int rc = launch(h, S))
+            return rc;
+        RelaxParams U = relax_params(h, RELAX_STAGE, 2 + st);
+        U.q = q;
+        U.k = k;
+        U.dq = dq;
+        U.a = rnd(A[st]);
+        U.read_dq = U.a != 0.0 ? 1 : 0;
+        U.b_dt = f32 ? (double)((float)rnd(B[st]) * (float)step_dt) : B[st] * step_dt;
+        if (int rc = relax(h, U))
+            return rc;
+    }
+    double sums[2 * RELAX_SLOTS];
+    if (int rc = relax_read(h, 2 + nst, sums))
+        return rc;
+    const double eta0 = sums[2];
+    double deta = -0.0;
+    for (int st = 0; st < nst; ++st)
+        deta += bfull[st] * step_dt * sums[2 * (2 + st) + 1];
+
+    // gamma: root of eta(q0 + gamma k) - eta0 - gamma deta near the previous one
+    const double eps = f32 ? (double)FLT_EPSILON : DBL_EPSILON;
+    double g = rnd(*gamma);
+    int it = 0, evaluated = 0;
+    for (; it < RELAX_MAX_NEWTON; ++it) {
+        RelaxParams N = relax_params(h, RELAX_NEWTON, 0);
+        N.q = q;
+        N.k = k;
+        N.q0 = q0;
+        N.first = it == 0;
+        N.relax = g;
+        if (int rc = relax(h, N))
+            return rc;
+        double v[2];
+        if (int rc = relax_read(h, 1, v))
+            return rc;
+        evaluated = 1;
+        const double F = v[0] - eta0 - g * deta;
+        if (std::fabs(F) <= 10 * eps)
+            break;
+        const double step = F / (v[1] - deta);
+        if (!std::isfinite(step))
+            return fail(h, RV_E_STATE, "relaxation Newton iteration broke down");
+        if (!(std::fabs(step) > 2 * eps * std::fabs(g)))
+            break;
+        g = rnd(g - step);
+        evaluated = 0;
+    }
+    if (!evaluated) {
+        RelaxParams X = relax_params(h, RELAX_AXPY, 0);
+        X.q = q;
+        X.k = k;
+        X.q0 = q0;
+        X.relax = g;
+        if (int rc = relax(h, X))
+            return rc;
+    }
+    *gamma = g;
+    if (iterations)
+        *iterations = it;
     return RV_OK;
 }
 

@@ -93,6 +93,34 @@ struct MortarParams {
 using MortarLauncher = int (*)(const MortarParams &, void *stream);
 MortarLauncher find_mortar_launcher(int dim, int nq, int law, int real_bytes, int flux);
 
+// Entropy passes of the relaxation LSRK step (entropy.hip).  Each pass runs
+// RELAX_BLOCKS workgroups over contiguous node ranges and leaves per workgroup
+// two Neumaier sums (value, compensation) in partial[4*block ..]: the entropy
+// integral sum MJ eta(x) at +0 and the entropy product sum MJ v(x).k at +2.
+enum RelaxMode : int {
+    RELAX_ETA = 0,      // x = q; eta sum
+    RELAX_PRODUCT = 1,  // x = q (p of entropyproduct); product sum with k
+    RELAX_STAGE = 2,    // x = q; product with k; dq = a dq + k; q += b dt dq
+    RELAX_NEWTON = 3,   // (first: k = q - q0); q = x = q0 + gamma k; both sums
+    RELAX_AXPY = 4      // q = q0 + gamma k; no sums
+};
+constexpr int RELAX_BLOCKS = 512;
+constexpr int RELAX_THREADS = 256;
+struct RelaxParams {
+    int mode, law, dim, real_bytes;
+    int64_t Np, K;
+    double gas_gamma;     // the Euler law's ratio of specific heats
+    const double *MJ;     // (K, Np)
+    void *q, *k, *dq;     // (K, Ns, Np) states of the handle's precision
+    const void *q0;
+    double a, b_dt;       // RELAX_STAGE
+    int read_dq;          // RELAX_STAGE: a != 0
+    int first;            // RELAX_NEWTON: form k = q - q0 first
+    double relax;         // RELAX_NEWTON / RELAX_AXPY: gamma
+    double *partial;      // 4 * RELAX_BLOCKS doubles
+};
+int launch_relax(const RelaxParams &P, void *stream);
+
 // Batched stages (batch.hip): one work-list item per wave's worth of cells of
 // one member; `kind` selects the member's (Nq, law, precision) routine.
 struct BatchItem {

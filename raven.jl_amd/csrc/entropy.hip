@@ -1,0 +1,238 @@
+// Entropy passes of the relaxation low-storage Runge-Kutta step
+// (RLSRK, odesolvers.jl:72-151) and the entropy diagnostics
+// entropyintegral / entropyproduct (measures.jl:23-33).
+//
+// Every pass is one sweep over the state: HBM-bound elementwise work with a
+// Neumaier-compensated sum per workgroup, in a fixed order (contiguous node
+// range per workgroup, fixed thread stride, fixed tree), so the host's sum of
+// the workgroup partials is reproducible run to run.  The per-node entropy
+// arithmetic runs in FP64 for both state precisions.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace rv {
+
+namespace {
+
+struct CSum {
+    double s, c;
+};
+
+__device__ inline CSum cadd(CSum a, double x)
+{
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x))
+        a.c += (a.s - t) + x;
+    else
+        a.c += (x - t) + a.s;
+    a.s = t;
+    return a;
+}
+
+__device__ inline CSum cmerge(CSum a, CSum b)
+{
+    CSum r = cadd(a, b.s);
+    r.c += b.c;
+    return r;
+}
+
+// entropy(law, q): advection q^2 / 2 (advection.jl:20); Euler
+// -rho log(p / rho^gamma) / (gamma - 1) (euler.jl:70-77)
+template <int LAW, int DIM>
+__device__ inline double entropy_of(const double *q, double g)
+{
+    if constexpr (LAW == LAW_ADVECTION) {
+        return q[0] * q[0] / 2;
+    } else {
+        double dot = q[1] * q[1];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            dot = dot + q[1 + d] * q[1 + d];
+        const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
+        const double s = log(p / pow(q[0], g));
+        return -q[0] * s / (g - 1);
+    }
+}
+
+// entropyvariables(law, q) . k: advection q k (advection.jl:21); Euler
+// ((gamma - s)/(gamma - 1) - |u|^2 b, 2 b u, -2 b), b = rho / 2p (euler.jl:79-91)
+template <int LAW, int DIM>
+__device__ inline double vdot(const double *q, const double *k, double g)
+{
+    if constexpr (LAW == LAW_ADVECTION) {
+        return q[0] * k[0];
+    } else {
+        double dot = q[1] * q[1];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            dot = dot + q[1 + d] * q[1 + d];
+        const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
+        const double s = log(p / pow(q[0], g));
+        const double b = q[0] / (2 * p);
+        double u[DIM], uu;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            u[d] = q[1 + d] / q[0];
+        uu = u[0] * u[0];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            uu = uu + u[d] * u[d];
+        double r = ((g - s) / (g - 1) - uu * b) * k[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            r = r + (2 * b * u[d]) * k[1 + d];
+        return r + (-2 * b) * k[DIM + 1];
+    }
+}
+
+template <typename R, int LAW, int DIM, int MODE>
+__global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams P)
+{
+    constexpr int NS = LAW == LAW_ADVECTION ? 1 : DIM + 2;
+    const int64_t Np = P.Np, n = P.Np * P.K;
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    R *q = static_cast<R *>(P.q);
+    R *k = static_cast<R *>(P.k);
+    R *dq = static_cast<R *>(P.dq);
+    const R *q0 = static_cast<const R *>(P.q0);
+    const R a = R(P.a), bdt = R(P.b_dt), gam = R(P.relax);
+    const double g = P.gas_gamma;
+    CSum eta{0.0, 0.0}, prod{0.0, 0.0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += RELAX_THREADS) {
+        const int64_t e = i / Np, node = i - e * Np;
+        const int64_t base = node + Np * NS * e;
+        double x[NS], kk[NS];
+        if constexpr (MODE == RELAX_ETA) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                x[s] = (double)q[base + Np * s];
+            eta = cadd(eta, P.MJ[i] * entropy_of<LAW, DIM>(x, g));
+        } else if constexpr (MODE == RELAX_PRODUCT) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                x[s] = (double)q[base + Np * s];
+                kk[s] = (double)k[base + Np * s];
+            }
+            prod = cadd(prod, P.MJ[i] * vdot<LAW, DIM>(x, kk, g));
+        } else if constexpr (MODE == RELAX_STAGE) {
+            // dq .*= rka; rhs!(k, q); dq .+= k; deta term at q; q += rkb dt dq
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int64_t o = base + Np * s;
+                const R qs = q[o], ks = k[o];
+                const R d = P.read_dq ? a * dq[o] + ks : ks;
+                dq[o] = d;
+                q[o] = qs + bdt * d;
+                x[s] = (double)qs;
+                kk[s] = (double)ks;
+            }
+            prod = cadd(prod, P.MJ[i] * vdot<LAW, DIM>(x, kk, g));
+        } else if constexpr (MODE == RELAX_NEWTON) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int64_t o = base + Np * s;
+                const R z = q0[o];
+                R ks;
+                if (P.first) {
+                    ks = q[o] - z;
+                    k[o] = ks;
+                } else {
+                    ks = k[o];
+                }
+                const R xs = z + gam * ks;
+                q[o] = xs;
+                x[s] = (double)xs;
+                kk[s] = (double)ks;
+            }
+            const double mj = P.MJ[i];
+            eta = cadd(eta, mj * entropy_of<LAW, DIM>(x, g));
+            prod = cadd(prod, mj * vdot<LAW, DIM>(x, kk, g));
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int64_t o = base + Np * s;
+                q[o] = q0[o] + gam * k[o];
+            }
+        }
+    }
+    if constexpr (MODE != RELAX_AXPY) {
+        __shared__ CSum sh[2][RELAX_THREADS];
+        sh[0][threadIdx.x] = eta;
+        sh[1][threadIdx.x] = prod;
+        __syncthreads();
+        for (int w = RELAX_THREADS / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                sh[0][threadIdx.x] = cmerge(sh[0][threadIdx.x], sh[0][threadIdx.x + w]);
+                sh[1][threadIdx.x] = cmerge(sh[1][threadIdx.x], sh[1][threadIdx.x + w]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double *out = P.partial + 4 * (int64_t)blockIdx.x;
+            out[0] = sh[0][0].s;
+            out[1] = sh[0][0].c;
+            out[2] = sh[1][0].s;
+            out[3] = sh[1][0].c;
+        }
+    }
+}
+
+template <typename R, int LAW, int DIM>
+int launch_mode(const RelaxParams &P, hipStream_t st)
+{
+    const dim3 grid(RELAX_BLOCKS), block(RELAX_THREADS);
+    switch (P.mode) {
+    case RELAX_ETA:
+        hipLaunchKernelGGL((relax_kernel<R, LAW, DIM, RELAX_ETA>), grid, block, 0, st, P);
+        break;
+    case RELAX_PRODUCT:
+        hipLaunchKernelGGL((relax_kernel<R, LAW, DIM, RELAX_PRODUCT>), grid, block, 0, st, P);
+        break;
+    case RELAX_STAGE:
+        hipLaunchKernelGGL((relax_kernel<R, LAW, DIM, RELAX_STAGE>), grid, block, 0, st, P);
+        break;
+    case RELAX_NEWTON:
+        hipLaunchKernelGGL((relax_kernel<R, LAW, DIM, RELAX_NEWTON>), grid, block, 0, st, P);
+        break;
+    case RELAX_AXPY:
+        hipLaunchKernelGGL((relax_kernel<R, LAW, DIM, RELAX_AXPY>), grid, block, 0, st, P);
+        break;
+    default:
+        return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename R>
+int launch_real(const RelaxParams &P, hipStream_t st)
+{
+    if (P.law == LAW_ADVECTION) {
+        switch (P.dim) {
+        case 1: return launch_mode<R, LAW_ADVECTION, 1>(P, st);
+        case 2: return launch_mode<R, LAW_ADVECTION, 2>(P, st);
+        case 3: return launch_mode<R, LAW_ADVECTION, 3>(P, st);
+        }
+    } else if (P.law == LAW_EULER) {
+        switch (P.dim) {
+        case 1: return launch_mode<R, LAW_EULER, 1>(P, st);
+        case 2: return launch_mode<R, LAW_EULER, 2>(P, st);
+        case 3: return launch_mode<R, LAW_EULER, 3>(P, st);
+        }
+    }
+    return -1;
+}
+
+}  // namespace
+
+int launch_relax(const RelaxParams &P, void *stream)
+{
+    if (P.K <= 0 && P.mode == RELAX_AXPY)
+        return 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return P.real_bytes == 8 ? launch_real<double>(P, st) : launch_real<float>(P, st);
+}
+
+}  // namespace rv

@@ -5,8 +5,11 @@
 // l_g = MJ dRdX[dir, :], n running over the line through ijk along dir, with
 // f# = CentralFlux (numericalfluxes.jl:39-42) or, for Euler,
 // EntropyConservativeFlux (euler.jl:150-184, logavg numericalfluxes.jl:9-22),
-// fused with the Rusanov surface term (dgsem.jl:733-784) and the LSRK update
-// dq = a dq + L(q), q_out = q + b dt dq (odesolvers.jl:55-57).
+// fused with the surface term (dgsem.jl:733-784) and the LSRK update
+// dq = a dq + L(q), q_out = q + b dt dq (odesolvers.jl:55-57).  The surface
+// flux is Rusanov or the volume form's own two-point flux f#(q-, q+)' n
+// (surfaceflux, numericalfluxes.jl:27-37), the pairing of Raven's
+// entropy-conservation tests.
 //
 // The form is compute-bound (Nq two-point fluxes per node and direction, two
 // logarithms each for the entropy-conservative flux), so the layout is the
@@ -125,7 +128,7 @@ struct TwoPoint {
     }
 };
 
-template <int DIM, int NQ, int LAW, typename R, int GEO, int VF>
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
 struct FD {
     static constexpr int NS = LawInfo<DIM, LAW>::NS;
     static constexpr int NP = IPow<NQ, DIM>::v;
@@ -136,10 +139,10 @@ struct FD {
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX_RUSANOV>;
 };
 
-template <int DIM, int NQ, int LAW, typename R, int GEO, int VF>
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
 __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
 {
-    using T = FD<DIM, NQ, LAW, R, GEO, VF>;
+    using T = FD<DIM, NQ, LAW, R, GEO, VF, SF>;
     using S = typename T::S;
     constexpr int NS = T::NS, NP = T::NP, NFP = T::NFP, NF = T::NF;
     __shared__ R qs[T::CPB][NS][NP];
@@ -247,7 +250,7 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     }
 
     // ---- surface: the faces this node lies on, in order d = 1..dim, f = 1, 2
-    // (surfaceterm!, Rusanov flux)
+    // (surfaceterm!)
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const int d = f >> 1;
@@ -279,7 +282,38 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
                 qp[s] = q1[s];
         }
         R lift;
-        if constexpr (GEO == GEO_AFFINE) {
+        if constexpr (SF != FLUX_RUSANOV) {
+            // twopointflux(q-, q+)' n: on an affine face n = +-e_d, so the
+            // product is row d of the flux with the face's sign
+            using TS = TwoPoint<DIM, NS, LAW, SF, R>;
+            if constexpr (GEO == GEO_AFFINE) {
+                R F[1][NS];
+                if constexpr (SF == FLUX_EC && LAW == LAW_EULER)
+                    TS::ec_rows(law, q1, p1, qp, TS::prim(law, qp), d, 1, F);
+                else
+                    TS::rows(law, q1, qp, d, 1, F);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    nf[s] = (f & 1) ? F[0][s] : -F[0][s];
+                lift = R(P.affine[e * DIM + d] * P.winv[(f & 1) ? NQ - 1 : 0]);
+            } else {
+                const double *sg = P.sgeo + ((e * NF + f) * (DIM + 1)) * NFP + m;
+                R F[DIM][NS];
+                if constexpr (SF == FLUX_EC && LAW == LAW_EULER)
+                    TS::ec_rows(law, q1, p1, qp, TS::prim(law, qp), 0, DIM, F);
+                else
+                    TS::rows(law, q1, qp, 0, DIM, F);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    R v = F[0][s] * R(sg[0]);
+#pragma unroll
+                    for (int cc = 1; cc < DIM; ++cc)
+                        v = v + F[cc][s] * R(sg[cc * NFP]);
+                    nf[s] = v;
+                }
+                lift = R(sg[DIM * NFP] * MJI);
+            }
+        } else if constexpr (GEO == GEO_AFFINE) {
             R fm[NS];
             law.flux_row(d, q1, fm);
             S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, q1, qp, nf);
@@ -309,39 +343,50 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     }
 }
 
-template <int DIM, int NQ, int LAW, typename R, int GEO, int VF>
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
 int launch_fd(const StageParams &P, void *stream, int)
 {
-    using T = FD<DIM, NQ, LAW, R, GEO, VF>;
+    using T = FD<DIM, NQ, LAW, R, GEO, VF, SF>;
     if (P.K <= 0)
         return 0;
     const int64_t blocks = (P.K + T::CPB - 1) / T::CPB;
-    hipLaunchKernelGGL((dg_fd_kernel<DIM, NQ, LAW, R, GEO, VF>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((dg_fd_kernel<DIM, NQ, LAW, R, GEO, VF, SF>), dim3((unsigned)blocks),
                        dim3(T::THREADS), 0, static_cast<hipStream_t>(stream), P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NQ, int LAW, int VF, int SF>
+StageLauncher pick_geo(const KernelKey &k)
+{
+    using R = FD_REAL;
+    return k.geo == GEO_GENERAL ? &launch_fd<FD_DIM, NQ, LAW, R, GEO_GENERAL, VF, SF>
+                                : &launch_fd<FD_DIM, NQ, LAW, R, GEO_AFFINE, VF, SF>;
+}
+
+// surface flux: Rusanov, or the volume form's own two-point flux
+template <int NQ, int LAW, int VF>
+StageLauncher pick_surface(const KernelKey &k)
+{
+    if (k.flux == FLUX_RUSANOV)
+        return pick_geo<NQ, LAW, VF, FLUX_RUSANOV>(k);
+    if (k.flux == VF)
+        return pick_geo<NQ, LAW, VF, VF>(k);
+    return nullptr;
 }
 
 template <int NQ>
 StageLauncher pick_fd(const KernelKey &k)
 {
-    using R = FD_REAL;
-    constexpr int DIM = FD_DIM;
-    if constexpr (IPow<NQ, DIM>::v > 512) {
+    if constexpr (IPow<NQ, FD_DIM>::v > 512) {
         return nullptr;
     } else {
-        const bool gen = k.geo == GEO_GENERAL;
-        if (k.law == LAW_ADVECTION) {
-            if (k.vflux != FLUX_CENTRAL)
-                return nullptr;
-            return gen ? &launch_fd<DIM, NQ, LAW_ADVECTION, R, GEO_GENERAL, FLUX_CENTRAL>
-                       : &launch_fd<DIM, NQ, LAW_ADVECTION, R, GEO_AFFINE, FLUX_CENTRAL>;
-        }
+        if (k.law == LAW_ADVECTION)
+            return k.vflux == FLUX_CENTRAL ? pick_surface<NQ, LAW_ADVECTION, FLUX_CENTRAL>(k)
+                                           : nullptr;
         if (k.vflux == FLUX_CENTRAL)
-            return gen ? &launch_fd<DIM, NQ, LAW_EULER, R, GEO_GENERAL, FLUX_CENTRAL>
-                       : &launch_fd<DIM, NQ, LAW_EULER, R, GEO_AFFINE, FLUX_CENTRAL>;
+            return pick_surface<NQ, LAW_EULER, FLUX_CENTRAL>(k);
         if (k.vflux == FLUX_EC)
-            return gen ? &launch_fd<DIM, NQ, LAW_EULER, R, GEO_GENERAL, FLUX_EC>
-                       : &launch_fd<DIM, NQ, LAW_EULER, R, GEO_AFFINE, FLUX_EC>;
+            return pick_surface<NQ, LAW_EULER, FLUX_EC>(k);
         return nullptr;
     }
 }
@@ -350,7 +395,7 @@ StageLauncher pick_fd(const KernelKey &k)
 
 StageLauncher FD_FN(const KernelKey &k)
 {
-    if (k.dim != FD_DIM || k.real_bytes != (int)sizeof(FD_REAL) || k.flux != FLUX_RUSANOV)
+    if (k.dim != FD_DIM || k.real_bytes != (int)sizeof(FD_REAL))
         return nullptr;
     switch (k.nq) {
     case 2: return pick_fd<2>(k);

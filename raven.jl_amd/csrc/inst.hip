@@ -30,6 +30,8 @@ StageLauncher pick(const KernelKey &k)
     if constexpr (IPow<NQ, INST_DIM - 1>::v > 64) {
         return nullptr;
     } else {
+        if (k.flux != FLUX_RUSANOV && k.flux != FLUX_CENTRAL)
+            return nullptr;
         if (k.geo == GEO_AFFINE)
             return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_AFFINE, FLUX_RUSANOV>()
                                           : pick_one<NQ, GEO_AFFINE, FLUX_CENTRAL>();

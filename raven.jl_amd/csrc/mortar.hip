@@ -261,6 +261,8 @@ MortarLauncher pick_nq(int nq, int law, int real_bytes, int flux)
 
 MortarLauncher find_mortar_launcher(int dim, int nq, int law, int real_bytes, int flux)
 {
+    if (flux != FLUX_RUSANOV && flux != FLUX_CENTRAL)
+        return nullptr;
     if (dim == 2)
         return pick_nq<2, 2>(nq, law, real_bytes, flux);
     if (dim == 3)

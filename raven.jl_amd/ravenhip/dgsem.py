@@ -49,7 +49,8 @@ class CentralFlux:
 
 class EntropyConservativeFlux:
     """EntropyConservativeFlux (numericalfluxes.jl:56; Euler: euler.jl:150-184);
-    a volume (two-point) flux of FluxDifferencingForm."""
+    a volume (two-point) flux of FluxDifferencingForm, and with that form also
+    a surface flux, twopointflux(q-, q+)' n (numericalfluxes.jl:27-37)."""
     flux_id = L.RV_FLUX_EC
 
 
@@ -248,6 +249,18 @@ class DGSEM:
                                         L.ptr(out, ctypes.c_double)), self._h)
         return out
 
+    def entropy_integral(self, q):
+        out = np.zeros(1)
+        L.check(L.lib().rv_entropy_integral(self._h, self._check_state(q),
+                                            L.ptr(out, ctypes.c_double)), self._h)
+        return float(out[0])
+
+    def entropy_product(self, p, q):
+        out = np.zeros(1)
+        L.check(L.lib().rv_entropy_product(self._h, self._check_state(p), self._check_state(q),
+                                           L.ptr(out, ctypes.c_double)), self._h)
+        return float(out[0])
+
     def integrals(self, q):
         out = np.zeros(self.ns)
         L.check(L.lib().rv_integrals(self._h, self._check_state(q), L.ptr(out, ctypes.c_double)),
@@ -268,6 +281,16 @@ class DGSEM:
         out = [np.zeros(n, dtype=np.int64) for _ in range(4)]
         L.check(L.lib().rv_get_maps(self._h, *(L.ptr(a, ctypes.c_int64) for a in out)), self._h)
         return out
+
+
+def entropyintegral(dg, q):
+    """entropyintegral(dg, q) = sum(MJ .* entropy.(q)) (measures.jl:23-27)."""
+    return dg.entropy_integral(q)
+
+
+def entropyproduct(dg, p, q):
+    """entropyproduct(dg, p, q) = sum(MJ .* v(p) . q) (measures.jl:28-33)."""
+    return dg.entropy_product(p, q)
 
 
 def weightednorm(dg, q, componentwise=False):

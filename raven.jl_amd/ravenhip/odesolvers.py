@@ -1,9 +1,11 @@
-"""Low-storage Runge-Kutta over the fused HIP stage (odesolvers.jl:9-71).
+"""Low-storage Runge-Kutta over the fused HIP stage (odesolvers.jl:9-162).
 
 ``LSRK54(dg, q, dt; t0)`` / ``LSRK144`` build a solver; ``solve_(q, timeend,
 solver)`` is Raven's ``solve!`` (final step shortened to land on timeend) and
 runs entirely inside libravenhip (rv_lsrk_run); ``dostep_`` runs one step
-stage by stage through rv_lsrk_stage.
+stage by stage through rv_lsrk_stage.  ``RLSRK54`` / ``RLSRK144`` are the
+relaxation variants (entropy-conserving or -stable steps, odesolvers.jl:72-162):
+``solve_`` steps them one rv_rlsrk_step at a time.
 """
 import ctypes
 
@@ -38,8 +40,59 @@ def LSRK144(dg, q, dt, t0=0.0):
     return LSRK(dg, q, dt, t0, L.RV_LSRK144)
 
 
-def solve_(q, timeend, solver):
-    """solve!(q, timeend, solver) with adjust_final = true; returns #steps."""
+class RLSRK(LSRK):
+    """RLSRK (odesolvers.jl:72-115): the LSRK scheme plus the relaxation
+    parameter gamma (1 at the start) and the work arrays q0, k."""
+
+    def __init__(self, dg, q, dt, t0, scheme):
+        super().__init__(dg, q, dt, t0, scheme)
+        self.gamma = 1.0
+        self.k = dg.GridArray(0.0)
+        self.q0 = dg.GridArray(0.0)
+        self.newton_iterations = 0
+
+
+def RLSRK54(dg, q, dt, t0=0.0):
+    return RLSRK(dg, q, dt, t0, L.RV_LSRK54)
+
+
+def RLSRK144(dg, q, dt, t0=0.0):
+    return RLSRK(dg, q, dt, t0, L.RV_LSRK144)
+
+
+def _rlsrk_dostep(q, solver):
+    """dostep!(q, rlsrk) (odesolvers.jl:117-151) through rv_rlsrk_step."""
+    dg = solver.dg
+    g = np.array([solver.gamma])
+    it = ctypes.c_int32(0)
+    L.check(L.lib().rv_rlsrk_step(dg.handle, dg._check_state(q), dg._check_state(solver.dq),
+                                  dg._check_state(solver.k), dg._check_state(solver.q0),
+                                  solver.scheme, solver.time, solver.dt,
+                                  L.ptr(g, ctypes.c_double), ctypes.byref(it)), dg.handle)
+    solver.gamma = float(g[0])
+    solver.newton_iterations = it.value
+    solver.time += solver.dt
+
+
+def solve_(q, timeend, solver, after_step=None):
+    """solve!(q, timeend, solver) with adjust_final = true; returns #steps.
+    after_step(step, time, q) is called after every step of an RLSRK solver
+    (LSRK solvers run all their steps inside the library and take none)."""
+    if isinstance(solver, RLSRK):
+        step = 0
+        while True:
+            step += 1
+            time = solver.time
+            final = time + solver.dt >= timeend
+            if final:
+                solver.dt = timeend - time
+            _rlsrk_dostep(q, solver)
+            if after_step is not None:
+                after_step(step, time, q)
+            if final:
+                return step
+    if after_step is not None:
+        raise ValueError("after_step needs an RLSRK solver")
     t = ctypes.c_double(0.0)
     n = ctypes.c_int64(0)
     dg = solver.dg
@@ -50,10 +103,16 @@ def solve_(q, timeend, solver):
     return n.value
 
 
-def dostep_(q, solver, qtmp):
+def dostep_(q, solver, qtmp=None):
     """dostep!(q, lsrk) (odesolvers.jl:50-61) through the per-stage entry
     point, stage times time + rkc*dt; q updated in place (qtmp is the
-    ping-pong buffer; an odd stage count ends with a copy)."""
+    ping-pong buffer; an odd stage count ends with a copy).  An RLSRK solver
+    takes its step through rv_rlsrk_step and needs no qtmp."""
+    if isinstance(solver, RLSRK):
+        _rlsrk_dostep(q, solver)
+        return
+    if qtmp is None:
+        raise ValueError("dostep_ of an LSRK solver needs the ping-pong buffer qtmp")
     a, b, c = solver.coefficients()
     if solver.dg.dtype == "f32":
         # LSRK{Float32}: coefficients and dt in the state type, rkb*dt in FP32

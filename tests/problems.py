@@ -79,6 +79,32 @@ def isentropic_vortex_3d(P, t, gamma=1.4, centre=(5.0, 5.0), noise=None):
 
 
 # Golden cases of Raven's tests: (name, dim, law, domain, cfl, dt rule, T)
+def entropy_square_setup(law, N=4, K=5, gamma=1.4):
+    """test/balancelaws/{euler,advection}/entropy_conservation_1d.jl: a square
+    density pulse (ρ = 2 for |x| < 1/2, else 1) on the periodic brick [-1, 1]
+    with K cells of degree N, run to T = 5 by RLSRK54.  Euler: u = 1, p = 1,
+    EntropyConservativeFlux in volume and on faces, cfl 1/2 over the sound
+    speed; advection: u = 1, CentralFlux in both, cfl 1/4."""
+    nq = N + 1
+    v = julia_range(-1, 1, K + 1)
+
+    def rho_of(P):
+        return np.where(np.abs(P[:, 0, :]) < 0.5, 2.0, 1.0)
+
+    if law == "euler":
+        dt = 0.5 * julia_step(-1, 1, K + 1) / nq / np.sqrt(gamma * 1.0 / 1.0)
+
+        def ic(P):
+            rho = rho_of(P)
+            mom = [rho * 1.0]
+            rhoe = euler_energy(rho, mom, 1.0, gamma)
+            return np.ascontiguousarray(np.stack([rho] + mom + [rhoe], axis=1)).reshape(-1)
+        return dict(nq=nq, coords=[v], law="euler", u=None, flux="ec", ic=ic, dt=dt, T=5.0)
+    dt = 0.25 * julia_step(-1, 1, K + 1) / nq / 1.0
+    return dict(nq=nq, coords=[v], law="advection", u=[1.0], flux="central",
+                ic=lambda P: rho_of(P).reshape(-1).copy(), dt=dt, T=5.0)
+
+
 def golden_setup(case, K):
     """Grid extents, law, initial condition, dt and end time of a golden case."""
     N = 4

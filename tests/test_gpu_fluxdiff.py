@@ -184,9 +184,11 @@ def test_fd_rejects_unsupported_combinations(torch, R):
     with pytest.raises(R.RavenHIPError):   # EC is defined for Euler only
         R.DGSEM(law=R.AdvectionLaw(2), grid=grid, surface_numericalflux=R.RusanovFlux(),
                 volume_form=volume_form(R, "ec"))
-    with pytest.raises(R.RavenHIPError):   # flux differencing kernels carry Rusanov faces
+    with pytest.raises(R.RavenHIPError):   # faces: Rusanov or the volume flux itself
         R.DGSEM(law=R.EulerLaw(2), grid=grid, surface_numericalflux=R.CentralFlux(),
                 volume_form=volume_form(R, "ec"))
+    with pytest.raises(R.RavenHIPError):   # the weak form has no EC surface flux
+        R.DGSEM(law=R.EulerLaw(2), grid=grid, surface_numericalflux=R.EntropyConservativeFlux())
     dg = R.DGSEM(law=R.EulerLaw(2), grid=grid, surface_numericalflux=R.RusanovFlux(),
                  volume_form=volume_form(R, "ec"))
     with pytest.raises(R.RavenHIPError):   # batches run the weak form only

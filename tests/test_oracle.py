@@ -416,3 +416,113 @@ def test_oracle_reproduces_raven_golden_errors(case, K, want, tolscale):
     _, t = dg.solve(q, s["dt"], s["T"])
     err, _ = dg.weightednorm(q - s["ic"](P, t))
     assert err == pytest.approx(want, rel=tolscale * np.sqrt(EPS))
+
+
+# --------------------------------------------------------------------------
+# Entropy, two-point surface fluxes and relaxation RK (odesolvers.jl:72-162,
+# measures.jl:23-33, numericalfluxes.jl:27-37)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("scheme", [54, 144])
+def test_rlsrk_weights_are_the_butcher_weights(scheme):
+    """bfull (odesolvers.jl:96-101) are the Butcher b of the 2N scheme: with the
+    stage times c they satisfy the quadrature conditions sum b c^(k-1) = 1/k of
+    a fourth-order method, and agree with the weights read off the scheme's
+    action on y' = f(t) (dq enters y with weight sum_{j>=s} B_j prod A)."""
+    a, b, c = (np.asarray(v) for v in O.lsrk_coefficients(scheme))
+    bf = O.rlsrk_weights(scheme)
+    for k in range(1, 5):
+        assert abs(np.sum(bf * c ** (k - 1)) - 1.0 / k) < 1e-13
+    # brute force: the weight of stage s's right-hand side in y after all stages
+    n = len(a)
+    for s in range(n):
+        w, carry = 0.0, 1.0
+        for j in range(s, n):
+            if j > s:
+                carry *= a[j]
+            w += b[j] * carry
+        assert abs(w - bf[s]) < 1e-14
+
+
+def _entropy_grid(law, dim, vflux, sflux="rusanov", K=3, nq=5):
+    coords = [PB.julia_range(-1, 1, K + 1)] * dim
+    g = O.BrickGrid(nq, coords, [True] * dim)
+    u = [1.0, 0.5, 0.25][:dim] if law == "advection" else None
+    return g, O.DG(g, law, u=u, volume_flux=vflux, surface_flux=sflux)
+
+
+def _smooth_state(g, law, dim, seed=0):
+    P = g.point_fields()
+    rng = np.random.default_rng(seed)
+    if law == "advection":
+        return PB.advection_wave(P, dim, 0.0) + 0.05 * rng.standard_normal(g.K * g.Np)
+    return PB.euler_wave(P, dim, 0.0) + 1e-2 * rng.standard_normal(g.K * g.Np * (dim + 2))
+
+
+def test_entropy_integral_closed_form():
+    """A uniform Euler state: eta = -rho log(p / rho^gamma) / (gamma - 1) times
+    the domain's area; advection: q^2 / 2 times the area."""
+    g, dg = _entropy_grid("euler", 2, None)
+    rho, m, n, p, gam = 1.3, 0.4, -0.2, 0.9, 1.4
+    rhoe = p / (gam - 1) + (m * m + n * n) / (2 * rho)
+    q = np.tile(np.repeat([rho, m, n, rhoe], g.Np), g.K).copy()
+    want = -rho * np.log(p / rho ** gam) / (gam - 1) * 4.0
+    assert dg.entropy_integral(q) == pytest.approx(want, rel=1e-14)
+    g, dg = _entropy_grid("advection", 3, None)
+    assert dg.entropy_integral(np.full(g.K * g.Np, 1.5)) == pytest.approx(1.5 ** 2 / 2 * 8.0,
+                                                                          rel=1e-14)
+
+
+@pytest.mark.parametrize("law,dim", [("euler", 1), ("euler", 2), ("euler", 3), ("advection", 2)])
+def test_entropy_product_is_the_derivative_of_the_integral(law, dim):
+    """v(q) = d eta / dq: entropyproduct(q, k) is the directional derivative of
+    entropyintegral at q along k (central differences, step 1e-5)."""
+    g, dg = _entropy_grid(law, dim, None, K=2, nq=4)
+    q = _smooth_state(g, law, dim)
+    k = np.random.default_rng(7).standard_normal(q.size)
+    h = 1e-5
+    fd = (dg.entropy_integral(q + h * k) - dg.entropy_integral(q - h * k)) / (2 * h)
+    assert dg.entropy_product(q, k) == pytest.approx(fd, rel=1e-8)
+
+
+@pytest.mark.parametrize("law,dim,flux", [("euler", 1, "ec"), ("euler", 2, "ec"), ("euler", 3, "ec"),
+                                          ("advection", 1, "central"),
+                                          ("advection", 2, "central")])
+def test_two_point_surface_flux_conserves_entropy_semidiscretely(law, dim, flux):
+    """With the entropy-conservative (Euler) or central (advection) two-point
+    flux in the volume and on the faces, the periodic semi-discretisation
+    conserves entropy: sum MJ v(q) . L(q) vanishes to rounding, while the
+    Rusanov faces dissipate it (a strictly negative sum)."""
+    g, dg = _entropy_grid(law, dim, flux, sflux=flux)
+    q = _smooth_state(g, law, dim, seed=3)
+    L = np.zeros(q.size)
+    dg.rhs(L, q.copy(), increment=False)
+    scale = dg.entropy_product(q, np.abs(L)) if law == "advection" else np.linalg.norm(L)
+    prod = dg.entropy_product(q, L)
+    assert abs(prod) <= 1e-12 * max(abs(scale), 1.0)
+    _, dgr = _entropy_grid(law, dim, flux, sflux="rusanov")
+    Lr = np.zeros(q.size)
+    dgr.rhs(Lr, q.copy(), increment=False)
+    assert dgr.entropy_product(q, Lr) < -1e-6
+
+
+@pytest.mark.parametrize("law", ["euler", "advection"])
+def test_oracle_rlsrk_conserves_entropy_as_ravens_test(law):
+    """test/balancelaws/{euler,advection}/entropy_conservation_1d.jl: N = 4,
+    K = 5, square pulse, flux differencing with the matching two-point surface
+    flux, RLSRK54 to T = 5: |eta(T) - eta(0)| / |eta(0)| <= 40 eps.  The plain
+    LSRK54 run of the same set-up misses it by orders of magnitude, so the
+    relaxation is what the bound checks."""
+    s = PB.entropy_square_setup(law)
+    g = O.BrickGrid(s["nq"], s["coords"], [True])
+    dg = O.DG(g, law, u=s["u"], volume_flux=s["flux"], surface_flux=s["flux"])
+    q = s["ic"](g.point_fields())
+    eta0 = dg.entropy_integral(q)
+    gammas = []
+    n, t, _ = dg.rlsrk_solve(q, s["dt"], s["T"], after_step=lambda *a: gammas.append(a[3]))
+    assert t == s["T"] and n == len(gammas)
+    d_eta = (dg.entropy_integral(q) - eta0) / abs(eta0)
+    assert abs(d_eta) <= 40 * EPS
+    assert all(abs(gm - 1) < 1e-2 for gm in gammas)   # relaxation is a small correction
+    q = s["ic"](g.point_fields())
+    dg.solve(q, s["dt"], s["T"])
+    assert abs((dg.entropy_integral(q) - eta0) / abs(eta0)) > 1e-6
