@@ -205,8 +205,9 @@ int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstag
  * bfull are the Butcher weights of the scheme (odesolvers.jl:96-101).  The
  * root is found by Newton's method from *gamma (the previous step's root, 1 at
  * the start), stopping at |residual| <= 10 eps of the state type (NLsolve's
- * ftol there), or when the update falls below 2 eps |gamma|, after at most 50
- * iterations; *gamma receives the root and *iterations (may be NULL) the
+ * ftol there), when the residual no longer halves (its rounding floor; the
+ * better of the last two roots is kept), or when the update falls below
+ * 2 eps |gamma|, after at most 50 iterations; *gamma receives the root and *iterations (may be NULL) the
  * Newton iterations taken.  dq carries between steps like rv_lsrk_stage's
  * (not read at a stage with rka = 0); k and q0 are work arrays.  All four
  * arrays have the state's shape and must be distinct.  Blocks on the handle's

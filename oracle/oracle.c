@@ -1372,9 +1372,10 @@ int orc_rlsrk_weights(int scheme, double *bfull);
  * gamma solving eta(q0 + gamma k) - eta0 - gamma dEta = 0 (k = q - q0) from the
  * previous gamma.  Raven calls NLsolve (trust region, ftol = 10 eps); this is
  * Newton's method on the same residual and derivative r'(gamma) = v(q).k - dEta,
- * stopped at the same |residual| <= 10 eps, or when the Newton update is
- * below 2 eps |gamma| (the residual of a large integral can sit above the
- * absolute 10 eps in rounding alone), at most 50 iterations.  bfull_s are the
+ * stopped at the same |residual| <= 10 eps, or when the residual no longer
+ * halves (its rounding floor, which for a large integral sits above the
+ * absolute 10 eps; the better of the last two roots is kept), or when the
+ * Newton update is below 2 eps |gamma|, at most 50 iterations.  bfull_s are the
  * standard RK weights of the low-storage scheme (odesolvers.jl:96-101).
  * Returns the number of Newton iterations, -1 when the update is not finite. */
 int orc_dg_rlsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double *k, double *q0,
@@ -1400,7 +1401,7 @@ int orc_dg_rlsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double *k, 
     }
     for (int64_t i = 0; i < n; ++i)
         k[i] = q[i] - q0[i];
-    double g = *gamma;
+    double g = *gamma, gprev = g, Fprev = INFINITY;
     int it = 0;
     for (; it < 50; ++it) {
         for (int64_t i = 0; i < n; ++i)
@@ -1408,11 +1409,20 @@ int orc_dg_rlsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double *k, 
         const double F = orc_dg_entropy_integral(dg, q) - eta0 - g * deta;
         if (fabs(F) <= 10 * DBL_EPSILON)
             break;
+        if (fabs(F) > 0.5 * fabs(Fprev)) {
+            /* no longer contracting: the residual is at its rounding floor;
+             * keep the better of the last two roots */
+            if (fabs(F) > fabs(Fprev))
+                g = gprev;
+            break;
+        }
         const double step = F / (orc_dg_entropy_product(dg, q, k) - deta);
         if (!isfinite(step))
             return -1;
         if (!(fabs(step) > 2 * DBL_EPSILON * fabs(g)))
             break;
+        gprev = g;
+        Fprev = F;
         g = g - step;
     }
     *gamma = g;

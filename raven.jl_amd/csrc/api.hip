@@ -38,7 +38,7 @@ StageLauncher find_fd_d3_f32(const KernelKey &);
 StageLauncher find_stage_launcher(const KernelKey &k)
 {
     using Finder = StageLauncher (*)(const KernelKey &);
-    if (k.vflux != 0) {
+    if (k.vflux != 0 || k.relax) {
         // flux differencing form (fluxdiff.hip)
         static const Finder fd[] = {find_fd_d1_f64, find_fd_d1_f32, find_fd_d2_f64,
                                     find_fd_d2_f32, find_fd_d3_f64, find_fd_d3_f32};
@@ -71,6 +71,7 @@ struct rv_handle {
     bool own_stream = false;
     int wpb = 1;
     StageLauncher launcher = nullptr;
+    StageLauncher relax_launcher = nullptr;  // fused relaxation stage, when built
     // device buffers
     FaceRec *faces = nullptr;
     int64_t *explicit_idx = nullptr;
@@ -80,6 +81,7 @@ struct rv_handle {
     double *cellJ = nullptr;     // brick handles: per-cell J that MJ is filled from
     double *partials = nullptr;  // reduction scratch
     double *relax_partials = nullptr;  // entropy passes: RELAX_SLOTS slots of 4*RELAX_BLOCKS
+    double *eprod = nullptr;           // fused relaxation stages: 14 slots of fd_workgroups(h)
     void *qtmp = nullptr;        // ping-pong state of rv_lsrk_run
     int64_t *dtoc = nullptr;     // 0-based
     int64_t *ctod_ptr = nullptr;
@@ -194,7 +196,7 @@ void free_handle(rv_handle *h)
                     h->sgeo,   h->Dt,           h->Dw,       h->winv,     h->MJ,
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
                     h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
-                    h->cellJ,  h->relax_partials};
+                    h->cellJ,  h->relax_partials, h->eprod};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -552,6 +554,11 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
     KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
     key.vflux = h->vflux;
     h->launcher = find_stage_launcher(key);
+    if (h->vflux != 0) {
+        KernelKey rk = key;
+        rk.relax = 1;
+        h->relax_launcher = find_stage_launcher(rk);
+    }
     // waves per workgroup measured on MI355X for the 3-D column kernel at
     // Nq = 8: two for FP64 (256 VGPRs, 2 waves/SIMD), four for FP32
     h->wpb = h->dim == 3 && h->dtype == RV_F32 ? 2 : 1;
@@ -805,9 +812,9 @@ StageParams stage_params(rv_handle *h)
     return P;
 }
 
-int launch(rv_handle *h, const StageParams &P)
+int launch(rv_handle *h, const StageParams &P, StageLauncher fn = nullptr)
 {
-    if (h->launcher(P, h->stream, h->wpb) != 0) {
+    if ((fn ? fn : h->launcher)(P, h->stream, h->wpb) != 0) {
         hipError_t e = hipGetLastError();
         return fail(h, RV_E_HIP, std::string("stage launch failed: ") + hipGetErrorString(e));
     }
@@ -875,6 +882,13 @@ int ensure_relax(rv_handle *h)
     if (h->relax_partials)
         return RV_OK;
     return dalloc(h, &h->relax_partials, (size_t)4 * RELAX_BLOCKS * RELAX_SLOTS);
+}
+
+// workgroups of one flux-differencing stage launch
+int64_t fd_workgroups(const rv_handle *h)
+{
+    const int cpb = fd_cells_per_workgroup((int)h->Np);
+    return (h->K + cpb - 1) / cpb;
 }
 
 RelaxParams relax_params(rv_handle *h, int mode, int slot)
@@ -1448,65 +1462,123 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
     E.q = q0;
     if (int rc = relax(h, E))
         return rc;
-    // stages: k = L(q); dq = a dq + k; product v(q).k into slot 2 + s; q += b dt dq
-    StageParams S = stage_params(h);
-    S.q_in = as_global(q);
-    S.q_out = nullptr;
-    S.dq = as_global(k);
-    S.a = 1.0;
-    S.b_dt = 0.0;
-    S.read_dq = 0;
-    S.write_q = 0;
-    for (int st = 0; st < nst; ++st) {
-        if (int rc = launch(h, S))
+    double stage_prod[14];
+    double eta0;
+    const void *qfinal = q;
+    if (h->relax_launcher && !h->mortar) {
+        // fused: each flux-differencing stage forms k = L(q) in registers,
+        // dq = a dq + k and q_out = q + b dt dq, and leaves its workgroups'
+        // sums of MJ v(q) . k; the stages ping-pong between q and k (k is
+        // needed as an array only after the stages)
+        const int64_t nwg = fd_workgroups(h);
+        if (!h->eprod)
+            if (int rc = dalloc(h, &h->eprod, (size_t)14 * nwg))
+                return rc;
+        void *cur = q, *alt = k;
+        for (int st = 0; st < nst; ++st) {
+            StageParams S = stage_params(h);
+            S.q_in = as_global(cur);
+            S.q_out = as_global(alt);
+            S.dq = as_global(dq);
+            S.a = rnd(A[st]);
+            S.read_dq = S.a != 0.0 ? 1 : 0;
+            S.b_dt = f32 ? (double)((float)rnd(B[st]) * (float)step_dt) : B[st] * step_dt;
+            S.write_q = 1;
+            S.eprod = as_global(h->eprod + (size_t)st * nwg);
+            if (int rc = launch(h, S, h->relax_launcher))
+                return rc;
+            std::swap(cur, alt);
+        }
+        qfinal = cur;
+        std::vector<double> part((size_t)nst * nwg);
+        HIP_TRY(h, hipMemcpyAsync(part.data(), h->eprod, part.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream));
+        double e2[4];
+        if (int rc = relax_read(h, 2, e2))  // synchronises the stream
             return rc;
-        RelaxParams U = relax_params(h, RELAX_STAGE, 2 + st);
-        U.q = q;
-        U.k = k;
-        U.dq = dq;
-        U.a = rnd(A[st]);
-        U.read_dq = U.a != 0.0 ? 1 : 0;
-        U.b_dt = f32 ? (double)((float)rnd(B[st]) * (float)step_dt) : B[st] * step_dt;
-        if (int rc = relax(h, U))
+        eta0 = e2[2];
+        for (int st = 0; st < nst; ++st) {
+            double sum = 0, comp = 0;
+            for (int64_t b = 0; b < nwg; ++b) {
+                const double x = part[(size_t)st * nwg + b];
+                const double t = sum + x;
+                comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
+                sum = t;
+            }
+            stage_prod[st] = sum + comp;
+        }
+    } else {
+        // stages: k = L(q); dq = a dq + k; product v(q).k into slot 2 + s; q += b dt dq
+        StageParams S = stage_params(h);
+        S.q_in = as_global(q);
+        S.q_out = nullptr;
+        S.dq = as_global(k);
+        S.a = 1.0;
+        S.b_dt = 0.0;
+        S.read_dq = 0;
+        S.write_q = 0;
+        for (int st = 0; st < nst; ++st) {
+            if (int rc = launch(h, S))
+                return rc;
+            RelaxParams U = relax_params(h, RELAX_STAGE, 2 + st);
+            U.q = q;
+            U.k = k;
+            U.dq = dq;
+            U.a = rnd(A[st]);
+            U.read_dq = U.a != 0.0 ? 1 : 0;
+            U.b_dt = f32 ? (double)((float)rnd(B[st]) * (float)step_dt) : B[st] * step_dt;
+            if (int rc = relax(h, U))
+                return rc;
+        }
+        double sums[2 * RELAX_SLOTS];
+        if (int rc = relax_read(h, 2 + nst, sums))
             return rc;
+        eta0 = sums[2];
+        for (int st = 0; st < nst; ++st)
+            stage_prod[st] = sums[2 * (2 + st) + 1];
     }
-    double sums[2 * RELAX_SLOTS];
-    if (int rc = relax_read(h, 2 + nst, sums))
-        return rc;
-    const double eta0 = sums[2];
     double deta = -0.0;
     for (int st = 0; st < nst; ++st)
-        deta += bfull[st] * step_dt * sums[2 * (2 + st) + 1];
+        deta += bfull[st] * step_dt * stage_prod[st];
 
     // gamma: root of eta(q0 + gamma k) - eta0 - gamma deta near the previous one
     const double eps = f32 ? (double)FLT_EPSILON : DBL_EPSILON;
-    double g = rnd(*gamma);
-    int it = 0, evaluated = 0;
+    // (stops as the oracle's: |residual| <= 10 eps; the residual no longer
+    // halves, keeping the better of the last two roots; a negligible update)
+    double g = rnd(*gamma), gprev = g, Fprev = INFINITY, g_in_q = NAN;
+    int it = 0;
     for (; it < RELAX_MAX_NEWTON; ++it) {
         RelaxParams N = relax_params(h, RELAX_NEWTON, 0);
         N.q = q;
         N.k = k;
         N.q0 = q0;
         N.first = it == 0;
+        N.qf = qfinal;
         N.relax = g;
         if (int rc = relax(h, N))
             return rc;
         double v[2];
         if (int rc = relax_read(h, 1, v))
             return rc;
-        evaluated = 1;
+        g_in_q = g;
         const double F = v[0] - eta0 - g * deta;
         if (std::fabs(F) <= 10 * eps)
             break;
+        if (std::fabs(F) > 0.5 * std::fabs(Fprev)) {
+            if (std::fabs(F) > std::fabs(Fprev))
+                g = gprev;
+            break;
+        }
         const double step = F / (v[1] - deta);
         if (!std::isfinite(step))
             return fail(h, RV_E_STATE, "relaxation Newton iteration broke down");
         if (!(std::fabs(step) > 2 * eps * std::fabs(g)))
             break;
+        gprev = g;
+        Fprev = F;
         g = rnd(g - step);
-        evaluated = 0;
     }
-    if (!evaluated) {
+    if (!(g == g_in_q)) {
         RelaxParams X = relax_params(h, RELAX_AXPY, 0);
         X.q = q;
         X.k = k;

@@ -69,6 +69,10 @@ struct StageParams {
     const double RV_GLOBAL *winv;  // [i] = 1/w_i
     double gamma;
     double u[3];
+    // relaxation stage (the flux-differencing kernels built for it): dq = a dq + k
+    // with k = L(q_in) accumulated alone, and each workgroup stores
+    // sum MJ v(q_in) . k over its nodes at eprod[workgroup]
+    double RV_GLOBAL *eprod;
 };
 
 // Launch the fused stage for one kernel configuration; returns false when no
@@ -78,6 +82,7 @@ using StageLauncher = int (*)(const StageParams &, void *stream, int waves_per_b
 struct KernelKey {
     int dim, nq, law, real_bytes, geo, flux;
     int vflux = 0;  // 0: weak form; else the flux-differencing form's two-point flux
+    int relax = 0;  // the fused relaxation stage (StageParams::eprod) of a flux-differencing kernel
 };
 
 // Mortar coupling of hanging faces (mortar.hip), launched after a stage.
@@ -93,6 +98,10 @@ struct MortarParams {
 using MortarLauncher = int (*)(const MortarParams &, void *stream);
 MortarLauncher find_mortar_launcher(int dim, int nq, int law, int real_bytes, int flux);
 
+// Flux-differencing kernels (fluxdiff.hip): whole cells per workgroup, a
+// thread per node, at least 256 threads where a cell has fewer nodes
+constexpr int fd_cells_per_workgroup(int nodes) { return nodes >= 256 ? 1 : 256 / nodes; }
+
 // Entropy passes of the relaxation LSRK step (entropy.hip).  Each pass runs
 // RELAX_BLOCKS workgroups over contiguous node ranges and leaves per workgroup
 // two Neumaier sums (value, compensation) in partial[4*block ..]: the entropy
@@ -101,7 +110,7 @@ enum RelaxMode : int {
     RELAX_ETA = 0,      // x = q; eta sum
     RELAX_PRODUCT = 1,  // x = q (p of entropyproduct); product sum with k
     RELAX_STAGE = 2,    // x = q; product with k; dq = a dq + k; q += b dt dq
-    RELAX_NEWTON = 3,   // (first: k = q - q0); q = x = q0 + gamma k; both sums
+    RELAX_NEWTON = 3,   // (first: k = qf - q0); q = x = q0 + gamma k; both sums
     RELAX_AXPY = 4      // q = q0 + gamma k; no sums
 };
 constexpr int RELAX_BLOCKS = 512;
@@ -115,7 +124,8 @@ struct RelaxParams {
     const void *q0;
     double a, b_dt;       // RELAX_STAGE
     int read_dq;          // RELAX_STAGE: a != 0
-    int first;            // RELAX_NEWTON: form k = q - q0 first
+    int first;            // RELAX_NEWTON: form k = qf - q0 first
+    const void *qf;       // RELAX_NEWTON first: the state after the stages (may be q or k)
     double relax;         // RELAX_NEWTON / RELAX_AXPY: gamma
     double *partial;      // 4 * RELAX_BLOCKS doubles
 };

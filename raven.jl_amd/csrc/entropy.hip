@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "entropy.hpp"
 
 namespace rv {
 
@@ -37,55 +38,6 @@ __device__ inline CSum cmerge(CSum a, CSum b)
     return r;
 }
 
-// entropy(law, q): advection q^2 / 2 (advection.jl:20); Euler
-// -rho log(p / rho^gamma) / (gamma - 1) (euler.jl:70-77)
-template <int LAW, int DIM>
-__device__ inline double entropy_of(const double *q, double g)
-{
-    if constexpr (LAW == LAW_ADVECTION) {
-        return q[0] * q[0] / 2;
-    } else {
-        double dot = q[1] * q[1];
-#pragma unroll
-        for (int d = 1; d < DIM; ++d)
-            dot = dot + q[1 + d] * q[1 + d];
-        const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
-        const double s = log(p / pow(q[0], g));
-        return -q[0] * s / (g - 1);
-    }
-}
-
-// entropyvariables(law, q) . k: advection q k (advection.jl:21); Euler
-// ((gamma - s)/(gamma - 1) - |u|^2 b, 2 b u, -2 b), b = rho / 2p (euler.jl:79-91)
-template <int LAW, int DIM>
-__device__ inline double vdot(const double *q, const double *k, double g)
-{
-    if constexpr (LAW == LAW_ADVECTION) {
-        return q[0] * k[0];
-    } else {
-        double dot = q[1] * q[1];
-#pragma unroll
-        for (int d = 1; d < DIM; ++d)
-            dot = dot + q[1 + d] * q[1 + d];
-        const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
-        const double s = log(p / pow(q[0], g));
-        const double b = q[0] / (2 * p);
-        double u[DIM], uu;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-            u[d] = q[1 + d] / q[0];
-        uu = u[0] * u[0];
-#pragma unroll
-        for (int d = 1; d < DIM; ++d)
-            uu = uu + u[d] * u[d];
-        double r = ((g - s) / (g - 1) - uu * b) * k[0];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-            r = r + (2 * b * u[d]) * k[1 + d];
-        return r + (-2 * b) * k[DIM + 1];
-    }
-}
-
 template <typename R, int LAW, int DIM, int MODE>
 __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams P)
 {
@@ -98,6 +50,7 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
     R *k = static_cast<R *>(P.k);
     R *dq = static_cast<R *>(P.dq);
     const R *q0 = static_cast<const R *>(P.q0);
+    const R *qf = static_cast<const R *>(P.qf);
     const R a = R(P.a), bdt = R(P.b_dt), gam = R(P.relax);
     const double g = P.gas_gamma;
     CSum eta{0.0, 0.0}, prod{0.0, 0.0};
@@ -109,14 +62,14 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
 #pragma unroll
             for (int s = 0; s < NS; ++s)
                 x[s] = (double)q[base + Np * s];
-            eta = cadd(eta, P.MJ[i] * entropy_of<LAW, DIM>(x, g));
+            eta = cadd(eta, P.MJ[i] * law_entropy<LAW, DIM>(x, g));
         } else if constexpr (MODE == RELAX_PRODUCT) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 x[s] = (double)q[base + Np * s];
                 kk[s] = (double)k[base + Np * s];
             }
-            prod = cadd(prod, P.MJ[i] * vdot<LAW, DIM>(x, kk, g));
+            prod = cadd(prod, P.MJ[i] * law_entropy_dot<LAW, DIM>(x, kk, g));
         } else if constexpr (MODE == RELAX_STAGE) {
             // dq .*= rka; rhs!(k, q); dq .+= k; deta term at q; q += rkb dt dq
 #pragma unroll
@@ -129,7 +82,7 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
                 x[s] = (double)qs;
                 kk[s] = (double)ks;
             }
-            prod = cadd(prod, P.MJ[i] * vdot<LAW, DIM>(x, kk, g));
+            prod = cadd(prod, P.MJ[i] * law_entropy_dot<LAW, DIM>(x, kk, g));
         } else if constexpr (MODE == RELAX_NEWTON) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
@@ -137,7 +90,7 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
                 const R z = q0[o];
                 R ks;
                 if (P.first) {
-                    ks = q[o] - z;
+                    ks = qf[o] - z;
                     k[o] = ks;
                 } else {
                     ks = k[o];
@@ -148,8 +101,8 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
                 kk[s] = (double)ks;
             }
             const double mj = P.MJ[i];
-            eta = cadd(eta, mj * entropy_of<LAW, DIM>(x, g));
-            prod = cadd(prod, mj * vdot<LAW, DIM>(x, kk, g));
+            eta = cadd(eta, mj * law_entropy<LAW, DIM>(x, g));
+            prod = cadd(prod, mj * law_entropy_dot<LAW, DIM>(x, kk, g));
         } else {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {

@@ -1,0 +1,60 @@
+// Entropy and entropy variables of the laws (euler.jl:70-91,
+// advection.jl:20-21), evaluated in FP64 from state values of either
+// precision; shared by the entropy passes and the fused relaxation stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace rv {
+
+// entropy(law, q): advection q^2 / 2; Euler -rho log(p / rho^gamma) / (gamma - 1)
+template <int LAW, int DIM>
+__device__ inline double law_entropy(const double *q, double g)
+{
+    if constexpr (LAW == LAW_ADVECTION) {
+        return q[0] * q[0] / 2;
+    } else {
+        double dot = q[1] * q[1];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            dot = dot + q[1 + d] * q[1 + d];
+        const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
+        const double s = log(p / pow(q[0], g));
+        return -q[0] * s / (g - 1);
+    }
+}
+
+// entropyvariables(law, q) . k: advection q k; Euler
+// ((gamma - s)/(gamma - 1) - |u|^2 b, 2 b u, -2 b) . k with b = rho / 2p
+template <int LAW, int DIM>
+__device__ inline double law_entropy_dot(const double *q, const double *k, double g)
+{
+    if constexpr (LAW == LAW_ADVECTION) {
+        return q[0] * k[0];
+    } else {
+        double dot = q[1] * q[1];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            dot = dot + q[1 + d] * q[1 + d];
+        const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
+        const double s = log(p / pow(q[0], g));
+        const double b = q[0] / (2 * p);
+        double u[DIM], uu;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            u[d] = q[1 + d] / q[0];
+        uu = u[0] * u[0];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            uu = uu + u[d] * u[d];
+        double r = ((g - s) / (g - 1) - uu * b) * k[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            r = r + (2 * b * u[d]) * k[1 + d];
+        return r + (-2 * b) * k[DIM + 1];
+    }
+}
+
+}  // namespace rv

@@ -17,12 +17,18 @@
 // staged in LDS for the line partners.  Affine cells evaluate only the row
 // dir of the two-point flux (their l_g has no other component).
 //
+// A relaxation stage (RX, the RLSRK step of odesolvers.jl:117-151) keeps
+// k = L(q) apart from a dq, forms dq = a dq + k at the end and leaves each
+// workgroup's sum of MJ v(q) . k (the stage's entropy production) in
+// P.eprod[workgroup].
+//
 // Compiled once per (dim, precision) with -DFD_DIM and -DFD_REAL; api.hip asks
 // every unit for a launcher.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
 #include "dg_kernels.hpp"
+#include "entropy.hpp"
 
 #ifndef FD_DIM
 #error "FD_DIM must be defined"
@@ -134,13 +140,13 @@ struct FD {
     static constexpr int NP = IPow<NQ, DIM>::v;
     static constexpr int NFP = IPow<NQ, DIM - 1>::v;
     static constexpr int NF = 2 * DIM;
-    static constexpr int CPB = NP >= 256 ? 1 : 256 / NP;  // cells per workgroup
+    static constexpr int CPB = fd_cells_per_workgroup(NP);
     static constexpr int THREADS = CPB * NP;
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX_RUSANOV>;
 };
 
-template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
-__global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF, bool RX>
+__device__ __forceinline__ void fd_stage(const StageParams &P)
 {
     using T = FD<DIM, NQ, LAW, R, GEO, VF, SF>;
     using S = typename T::S;
@@ -153,13 +159,16 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     __shared__ PrimT ps[EC ? T::CPB : 1][EC ? NP : 1];
     const int t = threadIdx.x;
     const int c = t / NP, node = t - c * NP;
-    const int64_t e = (int64_t)blockIdx.x * T::CPB + c;
-    const bool active = e < P.K;
+    const int64_t e0 = (int64_t)blockIdx.x * T::CPB + c;
+    const bool active = e0 < P.K;
+    // a relaxation stage keeps every thread to the workgroup sum: a thread
+    // past the last cell repeats the last cell's work and stores nothing
+    const int64_t e = RX && !active ? P.K - 1 : e0;
     const Law<DIM, LAW, R> law(P);
     const R *qin = static_cast<const R *>(P.q_in);
     R q1[NS], acc[NS];
     PrimT p1{};
-    if (active) {
+    if (RX || active) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             q1[s] = qin[(e * NS + s) * NP + node];
@@ -171,7 +180,7 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
         }
     }
     __syncthreads();
-    if (!active)
+    if (!RX && !active)
         return;
     const int ijk[3] = {node % NQ, DIM > 1 ? (node / NQ) % NQ : 0,
                         DIM > 2 ? node / (NQ * NQ) : 0};
@@ -181,7 +190,7 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     R *dq = static_cast<R *>(P.dq);
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-        acc[s] = P.read_dq ? a * dq[(e * NS + s) * NP + node] : R(0);
+        acc[s] = P.read_dq && !RX ? a * dq[(e * NS + s) * NP + node] : R(0);
 
     // ---- volume: one direction after another (esvolumeterm_dir!)
 #pragma unroll
@@ -333,6 +342,31 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
     }
 
     // ---- update
+    if constexpr (RX) {
+        // dq .*= rka; dq .+= k; the dEta term v(q) . k (odesolvers.jl:130-134)
+        double x[NS], kk[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            x[s] = (double)q1[s];
+            kk[s] = (double)acc[s];
+            if (P.read_dq)
+                acc[s] = a * dq[(e * NS + s) * NP + node] + acc[s];
+        }
+        const double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
+        // the workgroup's sum: a fixed tree over the threads
+        __shared__ double red[T::THREADS];
+        red[t] = prod;
+        __syncthreads();
+        for (int w = 256; w > 0; w >>= 1) {
+            if (t < w && t + w < T::THREADS)
+                red[t] += red[t + w];
+            __syncthreads();
+        }
+        if (t == 0)
+            P.eprod[blockIdx.x] = red[0];
+        if (!active)
+            return;
+    }
     const R bdt = R(P.b_dt);
     R *qo = static_cast<R *>(P.q_out);
 #pragma unroll
@@ -344,33 +378,56 @@ __global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
 }
 
 template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
+__global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
+{
+    fd_stage<DIM, NQ, LAW, R, GEO, VF, SF, false>(P);
+}
+
+// The relaxation stage holds a few more registers; a 512-thread workgroup
+// (two waves per SIMD) is kept at two workgroups per CU (four waves per SIMD,
+// 128 VGPRs)
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
+__global__ void __launch_bounds__(512)
+    __attribute__((amdgpu_waves_per_eu(FD<DIM, NQ, LAW, R, GEO, VF, SF>::THREADS == 512 ? 4 : 1)))
+        dg_fd_relax_kernel(const StageParams P)
+{
+    fd_stage<DIM, NQ, LAW, R, GEO, VF, SF, true>(P);
+}
+
+template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF, bool RX>
 int launch_fd(const StageParams &P, void *stream, int)
 {
     using T = FD<DIM, NQ, LAW, R, GEO, VF, SF>;
     if (P.K <= 0)
         return 0;
     const int64_t blocks = (P.K + T::CPB - 1) / T::CPB;
-    hipLaunchKernelGGL((dg_fd_kernel<DIM, NQ, LAW, R, GEO, VF, SF>), dim3((unsigned)blocks),
-                       dim3(T::THREADS), 0, static_cast<hipStream_t>(stream), P);
+    if constexpr (RX)
+        hipLaunchKernelGGL((dg_fd_relax_kernel<DIM, NQ, LAW, R, GEO, VF, SF>),
+                           dim3((unsigned)blocks), dim3(T::THREADS), 0,
+                           static_cast<hipStream_t>(stream), P);
+    else
+        hipLaunchKernelGGL((dg_fd_kernel<DIM, NQ, LAW, R, GEO, VF, SF>), dim3((unsigned)blocks),
+                           dim3(T::THREADS), 0, static_cast<hipStream_t>(stream), P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int NQ, int LAW, int VF, int SF>
+template <int NQ, int LAW, int VF, int SF, bool RX>
 StageLauncher pick_geo(const KernelKey &k)
 {
     using R = FD_REAL;
-    return k.geo == GEO_GENERAL ? &launch_fd<FD_DIM, NQ, LAW, R, GEO_GENERAL, VF, SF>
-                                : &launch_fd<FD_DIM, NQ, LAW, R, GEO_AFFINE, VF, SF>;
+    return k.geo == GEO_GENERAL ? &launch_fd<FD_DIM, NQ, LAW, R, GEO_GENERAL, VF, SF, RX>
+                                : &launch_fd<FD_DIM, NQ, LAW, R, GEO_AFFINE, VF, SF, RX>;
 }
 
-// surface flux: Rusanov, or the volume form's own two-point flux
+// surface flux: Rusanov, or the volume form's own two-point flux; the fused
+// relaxation stage is built for the latter (the entropy-conservative pairing)
 template <int NQ, int LAW, int VF>
 StageLauncher pick_surface(const KernelKey &k)
 {
     if (k.flux == FLUX_RUSANOV)
-        return pick_geo<NQ, LAW, VF, FLUX_RUSANOV>(k);
+        return k.relax ? nullptr : pick_geo<NQ, LAW, VF, FLUX_RUSANOV, false>(k);
     if (k.flux == VF)
-        return pick_geo<NQ, LAW, VF, VF>(k);
+        return k.relax ? pick_geo<NQ, LAW, VF, VF, true>(k) : pick_geo<NQ, LAW, VF, VF, false>(k);
     return nullptr;
 }
 

@@ -44,7 +44,8 @@ StageLauncher pick(const KernelKey &k)
 
 StageLauncher INST_FN(const KernelKey &k)
 {
-    if (k.dim != INST_DIM || k.law != INST_LAW || k.real_bytes != (int)sizeof(INST_REAL))
+    if (k.dim != INST_DIM || k.law != INST_LAW || k.real_bytes != (int)sizeof(INST_REAL) ||
+        k.relax)
         return nullptr;
     switch (k.nq) {
     case 2: return pick<2>(k);

@@ -133,4 +133,35 @@ function lsrk_solve!(dg::HIPDGSEM, q, dq, dt, timeend; t0 = 0.0, scheme = 54)
     return tout[], nsteps[]
 end
 
+# entropyintegral(dg, q) / entropyproduct(dg, p, q) (measures.jl:23-33)
+function entropyintegral(dg::HIPDGSEM, q)
+    out = Ref{Cdouble}(0)
+    check(ccall((:rv_entropy_integral, libravenhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Cdouble}),
+                dg.handle, pointer(parent(q)), out), dg.handle)
+    return out[]
+end
+function entropyproduct(dg::HIPDGSEM, p, q)
+    out = Ref{Cdouble}(0)
+    check(ccall((:rv_entropy_product, libravenhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Cdouble}),
+                dg.handle, pointer(parent(p)), pointer(parent(q)), out), dg.handle)
+    return out[]
+end
+
+# dostep!(q, rlsrk::RLSRK) (odesolvers.jl:117-151) for an RLSRK built on a
+# HIPDGSEM: the solver's dq, k, q0 and gamma are used in place, so Raven's
+# solve! drives it unchanged through this method.
+function rlsrk_dostep!(q, rlsrk, dg::HIPDGSEM; scheme = 54)
+    γ = Ref{Cdouble}(rlsrk.γ)
+    iters = Ref{Int32}(0)
+    check(ccall((:rv_rlsrk_step, libravenhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble,
+                 Cdouble, Ref{Cdouble}, Ref{Int32}),
+                dg.handle, pointer(parent(q)), pointer(parent(rlsrk.dq)), pointer(parent(rlsrk.k)),
+                pointer(parent(rlsrk.q0)), scheme, rlsrk.time, rlsrk.dt, γ, iters), dg.handle)
+    rlsrk.γ = γ[]
+    rlsrk.time += rlsrk.dt
+    return iters[]
+end
+
 end # module
