@@ -1447,7 +1447,6 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
     if (int rc = ensure_relax(h))
         return rc;
     const bool f32 = h->dtype == RV_F32;
-    const size_t bytes = (size_t)h->Np * h->ns * h->K * (f32 ? 4 : 8);
     auto rnd = [f32](double v) { return f32 ? (double)(float)v : v; };
     // standard RK weights of the low-storage scheme (odesolvers.jl:96-101)
     double bfull[14];
@@ -1456,10 +1455,10 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
         bfull[i] = rnd(rnd(A[i + 1]) * bfull[i + 1] + rnd(B[i]));
     const double step_dt = rnd(dt);
 
-    // q0 .= q; eta0 = entropyintegral(q0)
-    HIP_TRY(h, hipMemcpyAsync(q0, q, bytes, hipMemcpyDeviceToDevice, h->stream));
+    // q0 .= q; eta0 = entropyintegral(q0), one pass
     RelaxParams E = relax_params(h, RELAX_ETA, 1);
-    E.q = q0;
+    E.q = q;
+    E.q0 = q0;
     if (int rc = relax(h, E))
         return rc;
     double stage_prod[14];

@@ -107,13 +107,13 @@ constexpr int fd_cells_per_workgroup(int nodes) { return nodes >= 256 ? 1 : 256 
 // two Neumaier sums (value, compensation) in partial[4*block ..]: the entropy
 // integral sum MJ eta(x) at +0 and the entropy product sum MJ v(x).k at +2.
 enum RelaxMode : int {
-    RELAX_ETA = 0,      // x = q; eta sum
+    RELAX_ETA = 0,      // x = q; eta sum; q0 = q when q0 is given (the step's copy)
     RELAX_PRODUCT = 1,  // x = q (p of entropyproduct); product sum with k
     RELAX_STAGE = 2,    // x = q; product with k; dq = a dq + k; q += b dt dq
     RELAX_NEWTON = 3,   // (first: k = qf - q0); q = x = q0 + gamma k; both sums
     RELAX_AXPY = 4      // q = q0 + gamma k; no sums
 };
-constexpr int RELAX_BLOCKS = 512;
+constexpr int RELAX_BLOCKS = 2048;  // 8 workgroups of 4 waves per CU
 constexpr int RELAX_THREADS = 256;
 struct RelaxParams {
     int mode, law, dim, real_bytes;

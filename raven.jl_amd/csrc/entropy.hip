@@ -59,9 +59,14 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
         const int64_t base = node + Np * NS * e;
         double x[NS], kk[NS];
         if constexpr (MODE == RELAX_ETA) {
+            R *copy = const_cast<R *>(q0);
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
-                x[s] = (double)q[base + Np * s];
+            for (int s = 0; s < NS; ++s) {
+                const R v = q[base + Np * s];
+                if (copy)
+                    copy[base + Np * s] = v;
+                x[s] = (double)v;
+            }
             eta = cadd(eta, P.MJ[i] * law_entropy<LAW, DIM>(x, g));
         } else if constexpr (MODE == RELAX_PRODUCT) {
 #pragma unroll

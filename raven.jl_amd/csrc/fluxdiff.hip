@@ -11,11 +11,12 @@
 // (surfaceflux, numericalfluxes.jl:27-37), the pairing of Raven's
 // entropy-conservation tests.
 //
-// The form is compute-bound (Nq two-point fluxes per node and direction, two
-// logarithms each for the entropy-conservative flux), so the layout is the
-// plain one: a thread per node, whole cells per workgroup, the cells' states
-// staged in LDS for the line partners.  Affine cells evaluate only the row
-// dir of the two-point flux (their l_g has no other component).
+// The form is compute-bound (Nq (Nq - 1) / 2 two-point fluxes per line and
+// direction, two logarithms each for the entropy-conservative flux).  Whole
+// cells fill a workgroup with their states staged in LDS; each symmetric pair
+// is evaluated once by threads packed over the workgroup's pairs, and a thread
+// per node gathers the pair terms.  Affine cells evaluate only the row dir of
+// the two-point flux (their l_g has no other component).
 //
 // A relaxation stage (RX, the RLSRK step of odesolvers.jl:117-151) keeps
 // k = L(q) apart from a dq, forms dq = a dq + k at the end and leaves each
@@ -141,6 +142,16 @@ struct FD {
     static constexpr int NFP = IPow<NQ, DIM - 1>::v;
     static constexpr int NF = 2 * DIM;
     static constexpr int CPB = fd_cells_per_workgroup(NP);
+    // volume pairs: the Nq (Nq - 1) / 2 symmetric pairs of each line, NL lines
+    // per cell and direction; HL lines per pass through the LDS pair buffer of
+    // at most 36 KB (two 512-thread FP64 workgroups per CU, as the register
+    // bound allows; a smaller FP32 buffer for more workgroups per CU measured
+    // slower: 2.03 against 1.80 ms per stage at Nq = 8)
+    static constexpr int NL = NP / NQ;
+    static constexpr int NPAIR = NQ * (NQ - 1) / 2;
+    static constexpr int BL = CPB * NL;
+    static constexpr int HL_FIT = 36864 / (NPAIR * NS * (int)sizeof(R));
+    static constexpr int HL = HL_FIT < 1 ? 1 : (HL_FIT < BL ? HL_FIT : BL);
     static constexpr int THREADS = CPB * NP;
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX_RUSANOV>;
 };
@@ -161,27 +172,24 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
     const int c = t / NP, node = t - c * NP;
     const int64_t e0 = (int64_t)blockIdx.x * T::CPB + c;
     const bool active = e0 < P.K;
-    // a relaxation stage keeps every thread to the workgroup sum: a thread
-    // past the last cell repeats the last cell's work and stores nothing
-    const int64_t e = RX && !active ? P.K - 1 : e0;
+    // every thread takes part in the pair passes (and a relaxation stage's
+    // workgroup sum): a thread past the last cell repeats the last cell's work
+    // and stores nothing
+    const int64_t e = active ? e0 : P.K - 1;
     const Law<DIM, LAW, R> law(P);
     const R *qin = static_cast<const R *>(P.q_in);
-    R q1[NS], acc[NS];
-    PrimT p1{};
-    if (RX || active) {
+    R acc[NS];
+    {
+        R q[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            q1[s] = qin[(e * NS + s) * NP + node];
-            qs[c][s][node] = q1[s];
+            q[s] = qin[(e * NS + s) * NP + node];
+            qs[c][s][node] = q[s];
         }
-        if constexpr (EC) {
-            p1 = TP::prim(law, q1);
-            ps[c][node] = p1;
-        }
+        if constexpr (EC)
+            ps[c][node] = TP::prim(law, q);
     }
     __syncthreads();
-    if (!RX && !active)
-        return;
     const int ijk[3] = {node % NQ, DIM > 1 ? (node / NQ) % NQ : 0,
                         DIM > 2 ? node / (NQ * NQ) : 0};
     const double MJ1 = P.mj[e * NP + node];
@@ -192,7 +200,15 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
     for (int s = 0; s < NS; ++s)
         acc[s] = P.read_dq && !RX ? a * dq[(e * NS + s) * NP + node] : R(0);
 
-    // ---- volume: one direction after another (esvolumeterm_dir!)
+    // ---- volume: one direction after another (esvolumeterm_dir!).  Node i's
+    // term from line partner n is MJI_i (D[n,i] G_n - D[i,n] G_i) with
+    // G_x = l_g(x) . f#(q_i, q_n): with f# symmetric, the pair's
+    // H = D[n,i] G_n - D[i,n] G_i serves both ends (+H at i, -H at n), so each
+    // pair is evaluated once (as the reference's :per_dir_symmetric kernel),
+    // by threads packed densely over the workgroup's pairs, and every node
+    // then adds its partners' +-MJI H in partner order from LDS.  The
+    // diagonal term is zero.
+    __shared__ R hs[T::HL][T::NPAIR][NS];
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
         if (!((P.dir_mask >> dir) & 1))
@@ -202,61 +218,96 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
 #pragma unroll
         for (int d = 0; d < dir; ++d)
             stride *= NQ;
-        const int line0 = node - id * stride;
-        R lg1[DIM];
-        if constexpr (GEO == GEO_GENERAL) {
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-                lg1[d] = R(P.vgeo[((e * DIM + dir) * DIM + d) * NP + node]);
-        } else {
-            lg1[0] = R(MJ1 * P.affine[e * DIM + dir]);
-        }
-        R part[NS];
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-            part[s] = R(-0.0);
-        for (int n = 0; n < NQ; ++n) {
-            const int ild = line0 + n * stride;
-            R q2[NS];
-#pragma unroll
-            for (int s = 0; s < NS; ++s)
-                q2[s] = qs[c][s][ild];
-            const R Ddn = R(MJI * P.Dt[n * NQ + id]);   // MJI D[id, n]
-            const R Dnd = R(MJI * P.Dt[id * NQ + n]);   // MJI D[n, id]
-            if constexpr (GEO == GEO_GENERAL) {
-                R F[DIM][NS], lg2[DIM];
-                if constexpr (EC)
-                    TP::ec_rows(law, q1, p1, q2, ps[c][ild], 0, DIM, F);
-                else
-                    TP::rows(law, q1, q2, 0, DIM, F);
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                    lg2[d] = R(P.vgeo[((e * DIM + dir) * DIM + d) * NP + ild]);
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        part[s] = part[s] - (Ddn * lg1[d]) * F[d][s];
-                        part[s] = part[s] + (F[d][s] * lg2[d]) * Dnd;
-                    }
-            } else {
-                R F[1][NS];
-                if constexpr (EC)
-                    TP::ec_rows(law, q1, p1, q2, ps[c][ild], dir, 1, F);
-                else
-                    TP::rows(law, q1, q2, dir, 1, F);
-                const R lg2 = R(P.mj[e * NP + ild] * P.affine[e * DIM + dir]);
+        const int myline = c * T::NL + node % stride + (node / (stride * NQ)) * stride;
+        const R mji = R(MJI);
+        for (int l0 = 0; l0 < T::BL; l0 += T::HL) {
+            const int nl = T::BL - l0 < T::HL ? T::BL - l0 : T::HL;
+            for (int item = t; item < nl * T::NPAIR; item += T::THREADS) {
+                const int cl = item / T::NPAIR, pr = item - cl * T::NPAIR;
+                int rem = pr, i = 0;
+                while (rem >= NQ - 1 - i) {
+                    rem -= NQ - 1 - i;
+                    ++i;
+                }
+                const int n = i + 1 + rem;
+                const int gl = l0 + cl, pc = gl / T::NL, l = gl - pc * T::NL;
+                const int base = l % stride + (l / stride) * stride * NQ;
+                const int na = base + i * stride, nb = base + n * stride;
+                int64_t pe = (int64_t)blockIdx.x * T::CPB + pc;
+                pe = pe < P.K ? pe : P.K - 1;
+                R qa[NS], qb[NS];
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    part[s] = part[s] - (Ddn * lg1[0]) * F[0][s];
-                    part[s] = part[s] + (F[0][s] * lg2) * Dnd;
+                    qa[s] = qs[pc][s][na];
+                    qb[s] = qs[pc][s][nb];
+                }
+                const R Din = R(P.Dt[n * NQ + i]);  // D[i, n]
+                const R Dni = R(P.Dt[i * NQ + n]);  // D[n, i]
+                R h[NS];
+                if constexpr (GEO == GEO_GENERAL) {
+                    R F[DIM][NS], la[DIM], lb[DIM];
+                    if constexpr (EC)
+                        TP::ec_rows(law, qa, ps[pc][na], qb, ps[pc][nb], 0, DIM, F);
+                    else
+                        TP::rows(law, qa, qb, 0, DIM, F);
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        la[d] = R(P.vgeo[((pe * DIM + dir) * DIM + d) * NP + na]);
+                        lb[d] = R(P.vgeo[((pe * DIM + dir) * DIM + d) * NP + nb]);
+                    }
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        R ga = la[0] * F[0][s], gb = F[0][s] * lb[0];
+#pragma unroll
+                        for (int d = 1; d < DIM; ++d) {
+                            ga = ga + la[d] * F[d][s];
+                            gb = gb + F[d][s] * lb[d];
+                        }
+                        h[s] = Dni * gb - Din * ga;
+                    }
+                } else {
+                    R F[1][NS];
+                    if constexpr (EC)
+                        TP::ec_rows(law, qa, ps[pc][na], qb, ps[pc][nb], dir, 1, F);
+                    else
+                        TP::rows(law, qa, qb, dir, 1, F);
+                    const double aff = P.affine[pe * DIM + dir];
+                    const R la = R(P.mj[pe * NP + na] * aff), lb = R(P.mj[pe * NP + nb] * aff);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        h[s] = Dni * (F[0][s] * lb) - Din * (la * F[0][s]);
+                }
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    hs[cl][pr][s] = h[s];
+            }
+            __syncthreads();
+            const int ml = myline - l0;
+            if (ml >= 0 && ml < nl) {
+                for (int n = 0; n < NQ; ++n) {
+                    if (n == id)
+                        continue;
+                    const int lo = id < n ? id : n, hi = id < n ? n : id;
+                    const int pr = lo * (2 * NQ - lo - 1) / 2 + (hi - lo - 1);
+                    const R w = id < n ? mji : -mji;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        acc[s] = acc[s] + w * hs[ml][pr][s];
                 }
             }
+            __syncthreads();
         }
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-            acc[s] = acc[s] + part[s];
     }
+
+    // the node's own state (and primitives) back from LDS: not held in
+    // registers through the pair passes
+    R q1[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        q1[s] = qs[c][s][node];
+    PrimT p1{};
+    if constexpr (EC && SF == FLUX_EC)
+        p1 = ps[c][node];
 
     // ---- surface: the faces this node lies on, in order d = 1..dim, f = 1, 2
     // (surfaceterm!)
@@ -364,9 +415,9 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
         }
         if (t == 0)
             P.eprod[blockIdx.x] = red[0];
-        if (!active)
-            return;
     }
+    if (!active)
+        return;
     const R bdt = R(P.b_dt);
     R *qo = static_cast<R *>(P.q_out);
 #pragma unroll
@@ -377,15 +428,19 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
     }
 }
 
+// A 512-thread workgroup (two waves per SIMD) is held to 128 VGPRs so that two
+// workgroups share a CU: measured on MI355X at Nq = 8 (3-D Euler, entropy
+// conservative, FP64) 3.6 ms per stage against 5.0 ms at the compiler's own
+// 140 VGPRs (one workgroup per CU), a few spilled registers included
 template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
-__global__ void __launch_bounds__(512) dg_fd_kernel(const StageParams P)
+__global__ void __launch_bounds__(512)
+    __attribute__((amdgpu_waves_per_eu(FD<DIM, NQ, LAW, R, GEO, VF, SF>::THREADS == 512 ? 4 : 1)))
+        dg_fd_kernel(const StageParams P)
 {
     fd_stage<DIM, NQ, LAW, R, GEO, VF, SF, false>(P);
 }
 
-// The relaxation stage holds a few more registers; a 512-thread workgroup
-// (two waves per SIMD) is kept at two workgroups per CU (four waves per SIMD,
-// 128 VGPRs)
+// the relaxation stage, under the same register bound
 template <int DIM, int NQ, int LAW, typename R, int GEO, int VF, int SF>
 __global__ void __launch_bounds__(512)
     __attribute__((amdgpu_waves_per_eu(FD<DIM, NQ, LAW, R, GEO, VF, SF>::THREADS == 512 ? 4 : 1)))
