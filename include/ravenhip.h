@@ -349,6 +349,16 @@ int rv_curved_metrics(int dim, int nq, int64_t ncells, const double *points, con
                       const double *weights, double *metrics, double *surface_metrics,
                       void *stream);
 
+/* min_node_distance(grid; dims) (grids.jl:94-145): the least distance between
+ * a node and its neighbours at linear index ijk -+ stride(d) of the same cell,
+ * over every node and every direction d whose bit is set in dims_mask (bit 0
+ * is the first direction; the reference's default is all of them).  points
+ * (Np, dim, K) is a DEVICE pointer; runs on `stream` and returns the minimum
+ * in *out.  A mask naming no direction or one beyond dim is an invalid
+ * argument (the reference's ArgumentError). */
+int rv_min_node_distance(int dim, int nq, int64_t ncells, const double *points,
+                         int32_t dims_mask, double *out, void *stream);
+
 /* The integer lattice position of every cell of the brick at the finest level,
  * ijk[e*dim + d] (cell e spans [lo + h*ijk, lo + h*(ijk+1)) along d): the
  * quadrant coordinates generate() reads from the forest

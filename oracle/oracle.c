@@ -1365,6 +1365,38 @@ double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q
     return sum;
 }
 
+/* min_node_distance(grid; dims) (grids.jl:94-145): per node, the distance to
+ * the nodes at linear index ijk -+ stride(d) inside [0, Np) for every d in the
+ * mask (min_neighbour_distance_kernel, :94-119), minimised over all nodes */
+double orc_min_node_distance(int dim, int nq, int64_t K, const double *points, int mask)
+{
+    const int np = dim == 3 ? nq * nq * nq : dim == 2 ? nq * nq : nq;
+    double md = INFINITY;
+    for (int64_t e = 0; e < K; ++e)
+        for (int ijk = 0; ijk < np; ++ijk) {
+            const double *x = points + (int64_t)dim * np * e;
+            int stride = 1;
+            for (int d = 0; d < dim; ++d, stride *= nq) {
+                if (!((mask >> d) & 1))
+                    continue;
+                for (int m = -1; m <= 1; m += 2) {
+                    const int nb = ijk + stride * m;
+                    if (nb < 0 || nb >= np)
+                        continue;
+                    double ss = 0;
+                    for (int c = 0; c < dim; ++c) {
+                        const double dx = x[ijk + np * c] - x[nb + np * c];
+                        ss += dx * dx;
+                    }
+                    const double dist = sqrt(ss);
+                    if (dist < md)
+                        md = dist;
+                }
+            }
+        }
+    return md;
+}
+
 int orc_rlsrk_weights(int scheme, double *bfull);
 
 /* One relaxation LSRK step, dostep!(q, ::RLSRK) (odesolvers.jl:117-151): the

@@ -126,6 +126,8 @@ double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q
  * dq carries between steps, k and q0 are work arrays of the state's size */
 int orc_dg_rlsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double *k, double *q0,
                       double time, double dt, double *gamma);
+/* min_node_distance(grid; dims) (grids.jl:94-145), dims as a bit mask */
+double orc_min_node_distance(int dim, int nq, int64_t K, const double *points, int mask);
 /* Butcher weights of the low-storage scheme (odesolvers.jl:96-101); returns #stages */
 int orc_rlsrk_weights(int scheme, double *bfull);
 

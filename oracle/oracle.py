@@ -74,6 +74,9 @@ def lib():
                                         ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_double)]
         L.orc_dg_rlsrk_step.restype = ctypes.c_int
+        L.orc_min_node_distance.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, f64p,
+                                            ctypes.c_int]
+        L.orc_min_node_distance.restype = ctypes.c_double
         L.orc_rlsrk_weights.argtypes = [ctypes.c_int, f64p]
         L.orc_rlsrk_weights.restype = ctypes.c_int
         _LIB = L
@@ -143,6 +146,12 @@ class BrickGrid:
         self.dtoc = np.zeros(self.Np * K, dtype=np.int64)
         self.nc = L.orc_brick_dtoc(dim, nq, self.n, self.periodic, level, K, self.tree,
                                    self.quad, self.dtoc)
+
+    def min_node_distance(self, dims=None):
+        """min_node_distance(grid; dims) (grids.jl:94-145); dims 1-based."""
+        dims = range(1, self.dim + 1) if dims is None else dims
+        mask = sum(1 << (d - 1) for d in dims)
+        return lib().orc_min_node_distance(self.dim, self.nq, self.K, self.points, mask)
 
     def point_fields(self):
         """(K, dim, Np) view of the node coordinates."""

@@ -1716,6 +1716,88 @@ int rv_brick_arrays(const rv_brick_desc *bd, double *points, double *metrics,
     return RV_OK;
 }
 
+namespace {
+constexpr int MND_BLOCKS = 1024;
+constexpr int MND_THREADS = 256;
+
+// min_neighbour_distance_kernel (grids.jl:94-119): per node, the distance to
+// the nodes at linear index ijk -+ stride(d) inside the cell for every d in
+// the mask (the linear index may leave the node's line, as there); one
+// minimum per workgroup
+__global__ void __launch_bounds__(MND_THREADS)
+    min_node_distance_kernel(int dim, int nq, int64_t ncells, const double *points, int mask,
+                             double *partial)
+{
+    __shared__ double sh[MND_THREADS];
+    const int np = dim == 3 ? nq * nq * nq : dim == 2 ? nq * nq : nq;
+    const int64_t n = (int64_t)np * ncells;
+    double md = INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * MND_THREADS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * MND_THREADS) {
+        const int64_t e = i / np;
+        const int ijk = (int)(i - e * np);
+        const double *x = points + (int64_t)dim * np * e;
+        int stride = 1;
+        for (int d = 0; d < dim; ++d, stride *= nq) {
+            if (!((mask >> d) & 1))
+                continue;
+            for (int m = -1; m <= 1; m += 2) {
+                const int nb = ijk + stride * m;
+                if (nb < 0 || nb >= np)
+                    continue;
+                double ss = 0;
+                for (int c = 0; c < dim; ++c) {
+                    const double dx = x[ijk + np * c] - x[nb + np * c];
+                    ss += dx * dx;
+                }
+                md = fmin(md, sqrt(ss));
+            }
+        }
+    }
+    sh[threadIdx.x] = md;
+    __syncthreads();
+    for (int w = MND_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = sh[0];
+}
+}  // namespace
+
+int rv_min_node_distance(int dim, int nq, int64_t ncells, const double *points,
+                         int32_t dims_mask, double *out, void *stream)
+{
+    if (dim < 1 || dim > 3 || nq < 2 || nq > 16 || ncells < 0 || !points || !out ||
+        dims_mask <= 0 || dims_mask >= (1 << dim))
+        return fail(nullptr, RV_E_ARG, "rv_min_node_distance: invalid argument (dims not valid)");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double *part = nullptr;
+    hipError_t e = hipMallocAsync((void **)&part, sizeof(double) * MND_BLOCKS, st);
+    std::vector<double> host(MND_BLOCKS);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(min_node_distance_kernel, dim3(MND_BLOCKS), dim3(MND_THREADS), 0, st,
+                           dim, nq, ncells, points, (int)dims_mask, part);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host.data(), part, sizeof(double) * MND_BLOCKS, hipMemcpyDeviceToHost,
+                           st);
+    if (part)
+        (void)hipFreeAsync(part, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return fail(nullptr, RV_E_HIP, std::string("rv_min_node_distance: ") +
+                                           hipGetErrorString(e));
+    double md = INFINITY;
+    for (double v : host)
+        md = std::fmin(md, v);
+    *out = md;
+    return RV_OK;
+}
+
 int rv_curved_metrics(int dim, int nq, int64_t ncells, const double *points, const double *D,
                       const double *weights, double *metrics, double *surface_metrics,
                       void *stream)

@@ -141,6 +141,8 @@ def lib():
         "rv_forest_mortars": (ctypes.c_int, [H, c_i64p]),
         "rv_lobatto_halves": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p]),
         "rv_enable_mortars": (ctypes.c_int, [H, c_i32p, c_i64p]),
+        "rv_min_node_distance": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp,
+                                                ctypes.c_int32, c_f64p, vp]),
         "rv_curved_metrics": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp,
                                              c_f64p, c_f64p, vp, vp, vp]),
         "rv_lsrk_coefficients": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p, c_f64p,

@@ -272,6 +272,24 @@ def _generate_forest(gm, warp):
     return grid
 
 
+def min_node_distance(grid, dims=None):
+    """min_node_distance(grid; dims = 1:ndims(cell)) (grids.jl:94-145): the
+    least distance between neighbouring nodes of a cell along the given
+    directions (1-based), reduced on the device (rv_min_node_distance)."""
+    import torch
+    dims = range(1, grid.dim + 1) if dims is None else tuple(dims)
+    if not dims or max(dims) > grid.dim or min(dims) < 1:
+        raise ValueError("dims are not valid")
+    mask = sum(1 << (d - 1) for d in dims)
+    pd = torch.from_numpy(np.ascontiguousarray(grid.points)).cuda()
+    out = np.zeros(1)
+    c = L.ctypes
+    L.check(L.lib().rv_min_node_distance(grid.dim, grid.nq, grid.K, c.c_void_p(pd.data_ptr()),
+                                         mask, L.ptr(out, c.c_double),
+                                         c.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return float(out[0])
+
+
 def curved_metrics(cell, points):
     """Volume and surface metrics of curved cells (rv_curved_metrics) for
     host node coordinates (K, dim, Np); returns host arrays in Raven's

@@ -595,3 +595,30 @@ def test_mortar_stage_on_adapted_grid_matches_oracle(torch, R, nq, dtype):
         scale = dg.integrals(torch.abs(Ld).contiguous())
         say(f"   sum MJ dq = {tot}")
         assert np.all(np.abs(tot) <= 1e-11 * scale)
+
+
+@pytest.mark.parametrize("dim,nq", [(2, 5), (3, 5), (3, 8), (1, 6)])
+def test_min_node_distance_matches_oracle(torch, R, dim, nq):
+    """min_node_distance(grid; dims) (grids.jl:94-145) on the warped bricks of
+    test/grids.jl:753-790 (1-D: a plain brick): the device minimum against the
+    oracle's and the closed form Delta xi / (2K)."""
+    say(f"min_node_distance dim={dim} nq={nq}")
+    import test_oracle as TO
+    Kh, Kv = 10, 4
+    if dim == 1:
+        coords, warp = [PB.julia_range(0, 1, Kh + 1)], None
+    else:
+        coords = [PB.julia_range(0, 1, Kh + 1)] * (dim - 1) + [PB.julia_range(1, 2, Kv + 1)]
+        warp = TO.min_node_dist_warp(dim)
+    grid = R.generate(R.GridManager(R.LobattoCell(*([nq] * dim)), R.brick(coords, (True,) * dim)),
+                      warp=warp)
+    og = O.BrickGrid(nq, coords, [True] * dim, warp=warp)
+    for dims in [None, (dim,), tuple(range(1, dim))] if dim > 1 else [None]:
+        got = R.min_node_distance(grid, dims=dims)
+        want = og.min_node_distance(dims=dims)
+        say(f"   dims={dims}: {got!r} vs {want!r}")
+        assert got == pytest.approx(want, rel=1e-14)
+    dxi = og.x1d[1] - og.x1d[0]
+    assert R.min_node_distance(grid) == pytest.approx(dxi / (2 * Kh), rel=1e-12)
+    with pytest.raises(ValueError):
+        R.min_node_distance(grid, dims=(dim + 1,))

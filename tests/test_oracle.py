@@ -526,3 +526,32 @@ def test_oracle_rlsrk_conserves_entropy_as_ravens_test(law):
     q = s["ic"](g.point_fields())
     dg.solve(q, s["dt"], s["T"])
     assert abs((dg.entropy_integral(q) - eta0) / abs(eta0)) > 1e-6
+
+
+# --------------------------------------------------------------------------
+# min_node_distance (grids.jl:94-145)
+# --------------------------------------------------------------------------
+def min_node_dist_warp(dim):
+    """test/grids.jl:3-17: stretch by 2 past 1/2 (horizontal) and 3/2 (last)."""
+    def stretch(x, at):
+        return np.where(x >= at, at + 2 * (x - at), x)
+
+    def warp(*X):
+        return tuple(stretch(X[d], 0.5 if d < dim - 1 else 1.5) for d in range(dim))
+    return warp
+
+
+@pytest.mark.parametrize("dim,nq", [(2, 5), (3, 5), (2, 4), (3, 3)])
+def test_min_node_distance_closed_form(dim, nq):
+    """test/grids.jl:753-790: Kh = 10 cells across [0, 1] horizontally and
+    Kv = 4 across [1, 2] vertically, warped; the minimum spacing is the first
+    GLL gap over the cell width: Delta xi / (2 Kh) horizontally, Delta xi /
+    (2 Kv) vertically (the unstretched cells are the narrow ones)."""
+    Kh, Kv = 10, 4
+    coords = [PB.julia_range(0, 1, Kh + 1)] * (dim - 1) + [PB.julia_range(1, 2, Kv + 1)]
+    g = O.BrickGrid(nq, coords, [True] * dim, warp=min_node_dist_warp(dim))
+    dxi = g.x1d[1] - g.x1d[0]
+    hmnd, vmnd = dxi / (2 * Kh), dxi / (2 * Kv)
+    assert g.min_node_distance() == pytest.approx(hmnd, rel=1e-12)
+    assert g.min_node_distance(dims=(dim,)) == pytest.approx(vmnd, rel=1e-12)
+    assert g.min_node_distance(dims=range(1, dim)) == pytest.approx(hmnd, rel=1e-12)
