@@ -152,6 +152,10 @@ struct FD {
     static constexpr int BL = CPB * NL;
     static constexpr int HL_FIT = 36864 / (NPAIR * NS * (int)sizeof(R));
     static constexpr int HL = HL_FIT < 1 ? 1 : (HL_FIT < BL ? HL_FIT : BL);
+    // the pair buffer is stored line-fastest in rows of HLP values, HLP odd, so
+    // that lanes reading one pair of consecutive lines, or different pairs of
+    // one line, fall in different LDS banks
+    static constexpr int HLP = HL % 2 == 0 ? HL + 1 : HL;
     static constexpr int THREADS = CPB * NP;
     using S = Stage<DIM, NQ, LAW, R, GEO, FLUX_RUSANOV>;
 };
@@ -208,7 +212,11 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
     // by threads packed densely over the workgroup's pairs, and every node
     // then adds its partners' +-MJI H in partner order from LDS.  The
     // diagonal term is zero.
-    __shared__ R hs[T::HL][T::NPAIR][NS];
+    // the pair buffer; a relaxation stage's workgroup sum reuses it at the end
+    constexpr int HS_BYTES = T::NPAIR * NS * T::HLP * (int)sizeof(R);
+    constexpr int RED_BYTES = RX ? T::THREADS * (int)sizeof(double) : 0;
+    __shared__ __align__(16) char pool[HS_BYTES > RED_BYTES ? HS_BYTES : RED_BYTES];
+    R *hs = reinterpret_cast<R *>(pool);
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
         if (!((P.dir_mask >> dir) & 1))
@@ -220,10 +228,24 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
             stride *= NQ;
         const int myline = c * T::NL + node % stride + (node / (stride * NQ)) * stride;
         const R mji = R(MJI);
+        // pair order over the lanes: along x, the pairs of one line side by
+        // side (a line's nodes are contiguous); along y and z, one pair of
+        // consecutive lines side by side (lines are contiguous there), so
+        // that lanes read different LDS banks
+        const bool line_major = dir == 0;
         for (int l0 = 0; l0 < T::BL; l0 += T::HL) {
             const int nl = T::BL - l0 < T::HL ? T::BL - l0 : T::HL;
-            for (int item = t; item < nl * T::NPAIR; item += T::THREADS) {
-                const int cl = item / T::NPAIR, pr = item - cl * T::NPAIR;
+            for (int item = t; item < T::HL * T::NPAIR; item += T::THREADS) {
+                int cl, pr;
+                if (line_major) {
+                    cl = item / T::NPAIR;
+                    pr = item - cl * T::NPAIR;
+                } else {
+                    pr = item / T::HL;
+                    cl = item - pr * T::HL;
+                }
+                if (cl >= nl)
+                    continue;
                 int rem = pr, i = 0;
                 while (rem >= NQ - 1 - i) {
                     rem -= NQ - 1 - i;
@@ -279,7 +301,7 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
                 }
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    hs[cl][pr][s] = h[s];
+                    hs[(pr * NS + s) * T::HLP + cl] = h[s];
             }
             __syncthreads();
             const int ml = myline - l0;
@@ -292,7 +314,7 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
                     const R w = id < n ? mji : -mji;
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
-                        acc[s] = acc[s] + w * hs[ml][pr][s];
+                        acc[s] = acc[s] + w * hs[(pr * NS + s) * T::HLP + ml];
                 }
             }
             __syncthreads();
@@ -404,8 +426,9 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
                 acc[s] = a * dq[(e * NS + s) * NP + node] + acc[s];
         }
         const double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
-        // the workgroup's sum: a fixed tree over the threads
-        __shared__ double red[T::THREADS];
+        // the workgroup's sum: a fixed tree over the threads (in the pair
+        // buffer, free since the last pass ended with a barrier)
+        double *red = reinterpret_cast<double *>(pool);
         red[t] = prod;
         __syncthreads();
         for (int w = 256; w > 0; w >>= 1) {
