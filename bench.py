@@ -303,15 +303,53 @@ def run_fluxdiff(R, torch, gm, q0, dt, steps=4):
     return out
 
 
+def _rlsrk_case(R, torch, dg, q0, dt, steps):
+    """RLSRK54 against LSRK54 on one handle from one state (see run_rlsrk)."""
+    qi = torch.from_numpy(q0).to(dg.torch_dtype).cuda().reshape(dg.K, dg.ns, dg.Np)
+    qi = qi.contiguous()
+    eta0 = R.entropyintegral(dg, qi)
+    q = qi.clone()
+    solver = R.RLSRK54(dg, q, dt)
+    R.dostep_(q, solver)  # first step untimed (allocations)
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.dostep_(q, solver)
+        iters.append(solver.newton_iterations)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    d_eta = R.entropyintegral(dg, q) - eta0
+    q.copy_(qi)
+    plain = R.LSRK54(dg, q, dt)
+    R.solve_(q, dt * 0.75, plain)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n = R.solve_(q, plain.time + (steps - 0.25) * dt, plain)
+    torch.cuda.synchronize()
+    sec_plain = time.perf_counter() - t1
+    d_eta_plain = R.entropyintegral(dg, q) - eta0
+    return {"ms_per_step": sec / steps * 1e3,
+            "ms_per_lsrk54_step": sec_plain / n * 1e3,
+            "relaxation_overhead": (sec / steps) / (sec_plain / n) - 1.0,
+            "newton_iterations_per_step": float(np.mean(iters)),
+            "gamma": solver.gamma,
+            "eta0": eta0,
+            "delta_eta": d_eta,
+            "delta_eta_lsrk54": d_eta_plain,
+            "finite": bool(torch.isfinite(q).all())}
+
+
 def run_rlsrk(R, torch, gm, q0, dt, steps=4):
-    """RLSRK54 (relaxation LSRK, odesolvers.jl:72-162) on the C2 problem with
-    the entropy-conservative pairing (EC two-point flux in the volume and on
-    the faces), FP64 and FP32: wall time per step (each step reads back one
-    pair of sums per Newton iteration) next to the same handle's plain LSRK54
-    step, Newton iterations per step, and the entropy change eta(t) - eta(0)
-    of both runs over the same steps from the same state.  The isentropic
-    vortex has p = rho^gamma up to the density noise, so eta(0) itself is a
-    small number; the changes are reported as they are, not relative to it."""
+    """RLSRK54 (relaxation LSRK, odesolvers.jl:72-162) on the C2 problem, FP64
+    and FP32: with the entropy-conservative pairing (EC two-point flux in the
+    volume and on the faces), and with the weak form and Rusanov faces (the
+    headline operator).  Wall time per step (each step reads back one pair of
+    sums per Newton iteration) next to the same handle's plain LSRK54 step,
+    Newton iterations per step, and the entropy change eta(t) - eta(0) of both
+    runs over the same steps from the same state.  The isentropic vortex has
+    p = rho^gamma up to the density noise, so eta(0) itself is a small number;
+    the changes are reported as they are, not relative to it."""
     out = {"workload": "32^3 Nq=8 3-D Euler, FluxDifferencingForm(EntropyConservativeFlux()), "
                        "EntropyConservativeFlux surface flux, RLSRK54 (fused relaxation stages)",
            "steps": steps + 1}
@@ -319,40 +357,14 @@ def run_rlsrk(R, torch, gm, q0, dt, steps=4):
         ec = R.EntropyConservativeFlux()
         dg = R.DGSEM.brick(gm, R.EulerLaw(3, GAMMA), surface_numericalflux=ec, dtype=dtype,
                            volume_form=R.FluxDifferencingForm(ec))
-        qi = torch.from_numpy(q0).to(dg.torch_dtype).cuda().reshape(dg.K, dg.ns, dg.Np)
-        qi = qi.contiguous()
-        eta0 = R.entropyintegral(dg, qi)
-        q = qi.clone()
-        solver = R.RLSRK54(dg, q, dt)
-        R.dostep_(q, solver)  # first step untimed (allocations)
-        torch.cuda.synchronize()
-        iters = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            R.dostep_(q, solver)
-            iters.append(solver.newton_iterations)
-        torch.cuda.synchronize()
-        sec = time.perf_counter() - t0
-        d_eta = R.entropyintegral(dg, q) - eta0
-        q.copy_(qi)
-        plain = R.LSRK54(dg, q, dt)
-        R.solve_(q, dt * 0.75, plain)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        n = R.solve_(q, plain.time + (steps - 0.25) * dt, plain)
-        torch.cuda.synchronize()
-        sec_plain = time.perf_counter() - t1
-        d_eta_plain = R.entropyintegral(dg, q) - eta0
-        out[dtype] = {"ms_per_step": sec / steps * 1e3,
-                      "ms_per_lsrk54_step": sec_plain / n * 1e3,
-                      "relaxation_overhead": (sec / steps) / (sec_plain / n) - 1.0,
-                      "newton_iterations_per_step": float(np.mean(iters)),
-                      "gamma": solver.gamma,
-                      "eta0": eta0,
-                      "delta_eta": d_eta,
-                      "delta_eta_lsrk54": d_eta_plain,
-                      "finite": bool(torch.isfinite(q).all())}
-        del dg, q, qi, solver, plain
+        out[dtype] = _rlsrk_case(R, torch, dg, q0, dt, steps)
+        del dg
+    out["weak_form"] = {"workload": "the same brick, WeakForm(), Rusanov surface flux, RLSRK54 "
+                                    "(right-hand side, then a separate update pass per stage)"}
+    for dtype in ("f64", "f32"):
+        dg = R.DGSEM.brick(gm, R.EulerLaw(3, GAMMA), dtype=dtype)
+        out["weak_form"][dtype] = _rlsrk_case(R, torch, dg, q0, dt, steps)
+        del dg
     return out
 
 

@@ -38,7 +38,7 @@ StageLauncher find_fd_d3_f32(const KernelKey &);
 StageLauncher find_stage_launcher(const KernelKey &k)
 {
     using Finder = StageLauncher (*)(const KernelKey &);
-    if (k.vflux != 0 || k.relax) {
+    if (k.vflux != 0) {
         // flux differencing form (fluxdiff.hip)
         static const Finder fd[] = {find_fd_d1_f64, find_fd_d1_f32, find_fd_d2_f64,
                                     find_fd_d2_f32, find_fd_d3_f64, find_fd_d3_f32};
@@ -81,7 +81,7 @@ struct rv_handle {
     double *cellJ = nullptr;     // brick handles: per-cell J that MJ is filled from
     double *partials = nullptr;  // reduction scratch
     double *relax_partials = nullptr;  // entropy passes: RELAX_SLOTS slots of 4*RELAX_BLOCKS
-    double *eprod = nullptr;           // fused relaxation stages: 14 slots of fd_workgroups(h)
+    double *eprod = nullptr;           // fused relaxation stages: 14 slots of relax_partials_count
     void *qtmp = nullptr;        // ping-pong state of rv_lsrk_run
     int64_t *dtoc = nullptr;     // 0-based
     int64_t *ctod_ptr = nullptr;
@@ -884,8 +884,9 @@ int ensure_relax(rv_handle *h)
     return dalloc(h, &h->relax_partials, (size_t)4 * RELAX_BLOCKS * RELAX_SLOTS);
 }
 
-// workgroups of one flux-differencing stage launch
-int64_t fd_workgroups(const rv_handle *h)
+// partial sums one fused relaxation stage leaves: one per workgroup of the
+// flux-differencing launch
+int64_t relax_partials_count(const rv_handle *h)
 {
     const int cpb = fd_cells_per_workgroup((int)h->Np);
     return (h->K + cpb - 1) / cpb;
@@ -1469,9 +1470,10 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
         // dq = a dq + k and q_out = q + b dt dq, and leaves its workgroups'
         // sums of MJ v(q) . k; the stages ping-pong between q and k (k is
         // needed as an array only after the stages)
-        const int64_t nwg = fd_workgroups(h);
+        const int64_t nwg = relax_partials_count(h);
+        const int64_t cap = nwg;
         if (!h->eprod)
-            if (int rc = dalloc(h, &h->eprod, (size_t)14 * nwg))
+            if (int rc = dalloc(h, &h->eprod, (size_t)14 * cap))
                 return rc;
         void *cur = q, *alt = k;
         for (int st = 0; st < nst; ++st) {
@@ -1483,13 +1485,13 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
             S.read_dq = S.a != 0.0 ? 1 : 0;
             S.b_dt = f32 ? (double)((float)rnd(B[st]) * (float)step_dt) : B[st] * step_dt;
             S.write_q = 1;
-            S.eprod = as_global(h->eprod + (size_t)st * nwg);
+            S.eprod = as_global(h->eprod + (size_t)st * cap);
             if (int rc = launch(h, S, h->relax_launcher))
                 return rc;
             std::swap(cur, alt);
         }
         qfinal = cur;
-        std::vector<double> part((size_t)nst * nwg);
+        std::vector<double> part((size_t)nst * cap);
         HIP_TRY(h, hipMemcpyAsync(part.data(), h->eprod, part.size() * sizeof(double),
                                   hipMemcpyDeviceToHost, h->stream));
         double e2[4];
@@ -1499,7 +1501,7 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
         for (int st = 0; st < nst; ++st) {
             double sum = 0, comp = 0;
             for (int64_t b = 0; b < nwg; ++b) {
-                const double x = part[(size_t)st * nwg + b];
+                const double x = part[(size_t)st * cap + b];
                 const double t = sum + x;
                 comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
                 sum = t;

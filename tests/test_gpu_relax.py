@@ -136,26 +136,28 @@ def test_two_point_surface_flux_rhs_matches_oracle(torch, R, dim, nq, law, flux,
     assert err <= (1e-12 if dtype == "f64" else 1e-4)
 
 
-# fused: flux differencing with its own flux on the faces (the stage kernel
-# forms the entropy production); separate passes: every other operator
+# fused (the stage kernel forms the entropy production): flux differencing
+# with its own flux on the faces; separate update passes: every other operator
 RLSRK_CASES = [
-    # law, dim, volume flux, surface flux, scheme
-    ("euler", 2, "ec", "ec", 54),
-    ("euler", 1, "ec", "ec", 144),
-    ("advection", 2, "central", "central", 54),
-    ("euler", 3, "ec", "ec", 54),
-    ("euler", 2, "ec", "rusanov", 54),
-    ("euler", 3, None, "rusanov", 54),
-    ("advection", 1, None, "rusanov", 144),
+    # law, dim, volume flux, surface flux, scheme, geometry
+    ("euler", 2, "ec", "ec", 54, "auto"),
+    ("euler", 1, "ec", "ec", 144, "auto"),
+    ("advection", 2, "central", "central", 54, "auto"),
+    ("euler", 3, "ec", "ec", 54, "auto"),
+    ("euler", 2, "ec", "rusanov", 54, "auto"),
+    ("euler", 3, None, "rusanov", 54, "auto"),
+    ("euler", 3, None, "central", 144, "general"),
+    ("advection", 3, None, "rusanov", 54, "auto"),
+    ("advection", 1, None, "rusanov", 144, "auto"),
 ]
 
 
-@pytest.mark.parametrize("law,dim,flux,sflux,scheme", RLSRK_CASES)
-def test_rlsrk_steps_match_oracle(torch, R, law, dim, flux, sflux, scheme):
-    say(f"rlsrk steps {law} dim={dim} volume {flux} surface {sflux} LSRK{scheme}")
+@pytest.mark.parametrize("law,dim,flux,sflux,scheme,geometry", RLSRK_CASES)
+def test_rlsrk_steps_match_oracle(torch, R, law, dim, flux, sflux, scheme, geometry):
+    say(f"rlsrk steps {law} dim={dim} volume {flux} surface {sflux} LSRK{scheme} {geometry}")
     coords = [PB.julia_range(-1, 1, 4)] * dim
     u = [1.0, 0.5, 0.25][:dim] if law == "advection" else None
-    _, dg, og, odg = make_pair(R, dim, 5, coords, law, flux, sflux, u=u)
+    _, dg, og, odg = make_pair(R, dim, 5, coords, law, flux, sflux, u=u, geometry=geometry)
     q = state(og, law, dim, seed=4)
     dt = 0.02
     qd = to_dev(torch, dg, q)
