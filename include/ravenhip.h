@@ -147,6 +147,19 @@ int rv_synchronize(rv_handle *h);
 /* the hipStream_t every call of this handle is ordered on */
 void *rv_stream(const rv_handle *h);
 
+/* Resident device states for callers without a device allocator of their own
+ * (a C driver; Julia without AMDGPU.jl): one state array of the handle's shape
+ * (Nq^dim, Ns, K) and precision, Raven's GridArray parent layout (a11).
+ * Upload and download copy one whole state between host memory and such an
+ * array on the handle's stream and return when the copy is complete; free
+ * waits for the handle's stream first.  The reference's GridArray needs no
+ * counterpart (gridarrays.jl:29-108): these are the allocation side of the
+ * boundary's device-pointer convention. */
+int rv_state_alloc(rv_handle *h, void **state);
+int rv_state_free(rv_handle *h, void *state);
+int rv_state_upload(rv_handle *h, void *state, const void *host);
+int rv_state_download(rv_handle *h, void *host, const void *state);
+
 /* Launch shape of the stage kernels: waves per workgroup (1, 2 or 4; each wave
  * keeps its own cells and LDS slab, so results do not depend on it).  The
  * library picks a default per configuration; an autotuner that times the

@@ -1159,6 +1159,52 @@ int rv_synchronize(rv_handle *h)
 
 void *rv_stream(const rv_handle *h) { return h ? (void *)h->stream : nullptr; }
 
+int rv_state_alloc(rv_handle *h, void **state)
+{
+    if (!h || !state)
+        return fail(h, RV_E_ARG, "null argument");
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    *state = nullptr;
+    hipError_t e = hipMalloc(state, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *state = nullptr;
+        return fail(h, RV_E_NOMEM, "hipMalloc of a state failed");
+    }
+    return RV_OK;
+}
+
+int rv_state_free(rv_handle *h, void *state)
+{
+    if (!h)
+        return fail(nullptr, RV_E_ARG, "null handle");
+    if (!state)
+        return RV_OK;
+    // work queued on the handle's stream may still read or write the state
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, hipFree(state));
+    return RV_OK;
+}
+
+int rv_state_upload(rv_handle *h, void *state, const void *host)
+{
+    if (!h || !state || !host)
+        return fail(h, RV_E_ARG, "null argument");
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    HIP_TRY(h, hipMemcpyAsync(state, host, bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return RV_OK;
+}
+
+int rv_state_download(rv_handle *h, void *host, const void *state)
+{
+    if (!h || !state || !host)
+        return fail(h, RV_E_ARG, "null argument");
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    HIP_TRY(h, hipMemcpyAsync(host, state, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return RV_OK;
+}
+
 int rv_set_waves_per_block(rv_handle *h, int waves_per_block)
 {
     if (!h)

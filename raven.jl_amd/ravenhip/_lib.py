@@ -109,6 +109,10 @@ def lib():
         "rv_get_info": (ctypes.c_int, [H, ctypes.POINTER(RvInfo)]),
         "rv_synchronize": (ctypes.c_int, [H]),
         "rv_stream": (vp, [H]),
+        "rv_state_alloc": (ctypes.c_int, [H, ctypes.POINTER(vp)]),
+        "rv_state_free": (ctypes.c_int, [H, vp]),
+        "rv_state_upload": (ctypes.c_int, [H, vp, vp]),
+        "rv_state_download": (ctypes.c_int, [H, vp, vp]),
         "rv_set_waves_per_block": (ctypes.c_int, [H, ctypes.c_int]),
         "rv_get_waves_per_block": (ctypes.c_int, [H]),
         "rv_rhs": (ctypes.c_int, [H, vp, vp, ctypes.c_double, ctypes.c_int]),

@@ -622,3 +622,40 @@ def test_min_node_distance_matches_oracle(torch, R, dim, nq):
     assert R.min_node_distance(grid) == pytest.approx(dxi / (2 * Kh), rel=1e-12)
     with pytest.raises(ValueError):
         R.min_node_distance(grid, dims=(dim + 1,))
+
+
+def test_resident_states_through_the_c_abi(torch, R):
+    """rv_state_alloc / upload / download with rv_rhs and rv_lsrk_run, with no
+    torch tensor in between (a C or Julia caller's path), against the oracle."""
+    import ctypes
+    from ravenhip import _lib as L
+    say("resident states: 2-D Euler, Nq=5, 4x4 periodic")
+    coords = [PB.julia_range(0, 2, 5)] * 2
+    grid, dg, og, odg = make_pair(R, 2, 5, coords, (True, True), "euler")
+    q = initial_state(og, "euler", 2, seed=11)
+    lib, h = L.lib(), dg.handle
+    qs, dqs = ctypes.c_void_p(), ctypes.c_void_p()
+    L.check(lib.rv_state_alloc(h, ctypes.byref(qs)), h)
+    L.check(lib.rv_state_alloc(h, ctypes.byref(dqs)), h)
+    try:
+        L.check(lib.rv_state_upload(h, qs, q.ctypes.data_as(ctypes.c_void_p)), h)
+        L.check(lib.rv_rhs(h, dqs, qs, 0.0, 0), h)
+        got = np.zeros(q.size)
+        L.check(lib.rv_state_download(h, got.ctypes.data_as(ctypes.c_void_p), dqs), h)
+        want = np.zeros(q.size)
+        odg.rhs(want, q.copy(), 0.0, increment=False)
+        err = rel_l2(got, want)
+        say(f"   rhs rel L2 {err:.2e}")
+        assert err <= 1e-12
+        dt = 0.01
+        t, n = ctypes.c_double(0), ctypes.c_int64(0)
+        L.check(lib.rv_lsrk_run(h, qs, dqs, 54, 0.0, dt, 0.05, ctypes.byref(t), ctypes.byref(n)), h)
+        L.check(lib.rv_state_download(h, got.ctypes.data_as(ctypes.c_void_p), qs), h)
+        qo = q.copy()
+        nref, tref = odg.solve(qo, dt, 0.05)
+        err = rel_l2(got, qo)
+        say(f"   {n.value} LSRK54 steps to t = {t.value}: rel L2 {err:.2e}")
+        assert n.value == nref and t.value == tref and err <= 1e-12
+    finally:
+        L.check(lib.rv_state_free(h, qs), h)
+        L.check(lib.rv_state_free(h, dqs), h)
