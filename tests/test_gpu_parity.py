@@ -659,3 +659,36 @@ def test_resident_states_through_the_c_abi(torch, R):
     finally:
         L.check(lib.rv_state_free(h, qs), h)
         L.check(lib.rv_state_free(h, dqs), h)
+
+
+def test_c_driver_reproduces_solve_and_dostep(tmp_path):
+    """raven.jl_amd/ravenhip/rv_driver, a C caller through the C ABI alone
+    (rv_create_brick, rv_state_*, rv_lsrk_run or rv_lsrk_stage with
+    rv_lsrk_coefficients): Raven's solve! and its stage-by-stage dostep! give
+    the same state, which matches the oracle's LSRK54 run."""
+    import json as js
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "raven.jl_amd", "ravenhip", "rv_driver")
+    if not os.path.exists(exe):
+        pytest.fail("rv_driver is not built (make -C raven.jl_amd/csrc)")
+    out = {}
+    for mode in ("run", "stage"):
+        say(f"C driver: {mode}, 20 steps")
+        path = str(tmp_path / f"{mode}.bin")
+        r = subprocess.run([exe, mode, "20", path], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out[mode] = (js.loads(r.stdout.strip().splitlines()[-1]), np.fromfile(path))
+        say(f"   {out[mode][0]}")
+    assert np.array_equal(out["run"][1], out["stage"][1])
+    assert out["run"][0]["steps"] == out["stage"][0]["steps"] == 20
+    coords = [PB.julia_range(0, 1, 17)] * 2
+    og = O.BrickGrid(5, coords, (True, True))
+    odg = O.DG(og, "advection", u=[1.0, 0.5])
+    P = og.point_fields()
+    q = np.exp(-((P[:, 0, :] - 0.5) ** 2 + (P[:, 1, :] - 0.5) ** 2) / 0.01).reshape(-1).copy()
+    dt = out["run"][0]["dt"]
+    n, t = odg.solve(q, dt, 20 * dt)
+    err = rel_l2(out["run"][1], q)
+    say(f"   vs oracle: rel L2 {err:.2e} ({n} steps to {t})")
+    assert n == 20 and err <= 1e-12
