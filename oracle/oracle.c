@@ -1,7 +1,8 @@
 /*
  * oracle.c -- CPU restatement of Raven.jl's DGSEM hot path (see oracle.h).
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never part of it.
+ * TEST INFRASTRUCTURE ONLY: the reference the HIP library is tested against,
+ * never part of it.
  *
  * Build with -ffp-contract=off: Julia evaluates a*b + c with two roundings
  * unless muladd/fma is written, and its muladd is restated as fma() here.

@@ -1,7 +1,8 @@
 /*
  * oracle.h -- CPU restatement of Raven.jl's DGSEM hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * TEST INFRASTRUCTURE ONLY.  This library is what results are checked against:
+ * tests/, the smoke
  * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load
  * it; the HIP library (raven.jl_amd/csrc) never links, loads or calls it.
  *

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker / CPU baseline.  The product
+cpu_baseline leg of bench.py, as the reference results are checked against
+and as the CPU baseline.  The product
 (raven.jl_amd) never imports it.
 
 Arrays are numpy arrays in Raven's layout (Fortran-ordered views of Julia's
