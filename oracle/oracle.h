@@ -1,10 +1,10 @@
 /*
  * oracle.h -- CPU restatement of Raven.jl's DGSEM hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is what results are checked against:
- * tests/, the smoke
- * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load
- * it; the HIP library (raven.jl_amd/csrc) never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This library is what results are checked
+ * against: tests/, the smoke check in __graft_entry__.py and the cpu_baseline
+ * leg of bench.py may load it; the HIP library (raven.jl_amd/csrc) never
+ * links, loads or calls it.
  *
  * Every function names the Raven.jl source it restates (file:line, paths
  * relative to the Raven.jl repository root).  Arrays use Raven's memory
