@@ -213,9 +213,12 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
     // then adds its partners' +-MJI H in partner order from LDS.  The
     // diagonal term is zero.
     // the pair buffer; a relaxation stage's workgroup sum reuses it at the end
+    // (and the surface pass's face terms after it)
     constexpr int HS_BYTES = T::NPAIR * NS * T::HLP * (int)sizeof(R);
+    constexpr int FS_BYTES = T::CPB * NF * NFP * NS * (int)sizeof(R);
     constexpr int RED_BYTES = RX ? T::THREADS * (int)sizeof(double) : 0;
-    __shared__ __align__(16) char pool[HS_BYTES > RED_BYTES ? HS_BYTES : RED_BYTES];
+    constexpr int POOL0 = HS_BYTES > FS_BYTES ? HS_BYTES : FS_BYTES;
+    __shared__ __align__(16) char pool[POOL0 > RED_BYTES ? POOL0 : RED_BYTES];
     R *hs = reinterpret_cast<R *>(pool);
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
@@ -321,31 +324,36 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
         }
     }
 
-    // the node's own state (and primitives) back from LDS: not held in
-    // registers through the pair passes
-    R q1[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-        q1[s] = qs[c][s][node];
-    PrimT p1{};
-    if constexpr (EC && SF == FLUX_EC)
-        p1 = ps[c][node];
-
-    // ---- surface: the faces this node lies on, in order d = 1..dim, f = 1, 2
-    // (surfaceterm!)
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
+    // ---- surface (surfaceterm!): each face node's flux is evaluated once, by
+    // threads packed densely over the workgroup's 2 dim Nq^(dim-1) face nodes
+    // per cell (a thread per volume node would run the flux at the width of
+    // the wave for the eight nodes of a cell edge); lift * nf goes to LDS,
+    // m fastest, and every node subtracts its faces' terms in face order
+    // d = 1..dim, f = 1, 2
+    R *fs = reinterpret_cast<R *>(pool);
+    for (int item = t; item < T::CPB * NF * NFP; item += T::THREADS) {
+        const int fc = item / (NF * NFP), r = item - fc * (NF * NFP);
+        const int f = r / NFP, m = r - f * NFP;
         const int d = f >> 1;
         if (!((P.dir_mask >> d) & 1))
             continue;
-        if (ijk[d] != ((f & 1) ? NQ - 1 : 0))
-            continue;
-        int m = 0;
-        if constexpr (DIM == 2)
-            m = d == 0 ? ijk[1] : ijk[0];
-        if constexpr (DIM == 3)
-            m = d == 0 ? ijk[1] + NQ * ijk[2] : d == 1 ? ijk[0] + NQ * ijk[2] : ijk[0] + NQ * ijk[1];
-        const FaceRec rec = P.faces[e * NF + f];
+        int64_t fe = (int64_t)blockIdx.x * T::CPB + fc;
+        fe = fe < P.K ? fe : P.K - 1;
+        const int side = (f & 1) ? NQ - 1 : 0;
+        int nm;  // the face node's volume node
+        if constexpr (DIM == 1)
+            nm = side;
+        else if constexpr (DIM == 2)
+            nm = d == 0 ? side + NQ * m : m + NQ * side;
+        else
+            nm = d == 0   ? side + NQ * m
+                 : d == 1 ? m % NQ + NQ * side + NQ * NQ * (m / NQ)
+                          : m + NQ * NQ * side;
+        R qm[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            qm[s] = qs[fc][s][nm];
+        const FaceRec rec = P.faces[fe * NF + f];
         R qp[NS], nf[NS];
         if (rec.x >= 0) {
             const int fn = rec.y & 7, o = (rec.y >> 3) & 7;
@@ -361,7 +369,7 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
         } else {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-                qp[s] = q1[s];
+                qp[s] = qm[s];
         }
         R lift;
         if constexpr (SF != FLUX_RUSANOV) {
@@ -371,20 +379,20 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
             if constexpr (GEO == GEO_AFFINE) {
                 R F[1][NS];
                 if constexpr (SF == FLUX_EC && LAW == LAW_EULER)
-                    TS::ec_rows(law, q1, p1, qp, TS::prim(law, qp), d, 1, F);
+                    TS::ec_rows(law, qm, ps[fc][nm], qp, TS::prim(law, qp), d, 1, F);
                 else
-                    TS::rows(law, q1, qp, d, 1, F);
+                    TS::rows(law, qm, qp, d, 1, F);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     nf[s] = (f & 1) ? F[0][s] : -F[0][s];
-                lift = R(P.affine[e * DIM + d] * P.winv[(f & 1) ? NQ - 1 : 0]);
+                lift = R(P.affine[fe * DIM + d] * P.winv[side]);
             } else {
-                const double *sg = P.sgeo + ((e * NF + f) * (DIM + 1)) * NFP + m;
+                const double *sg = P.sgeo + ((fe * NF + f) * (DIM + 1)) * NFP + m;
                 R F[DIM][NS];
                 if constexpr (SF == FLUX_EC && LAW == LAW_EULER)
-                    TS::ec_rows(law, q1, p1, qp, TS::prim(law, qp), 0, DIM, F);
+                    TS::ec_rows(law, qm, ps[fc][nm], qp, TS::prim(law, qp), 0, DIM, F);
                 else
-                    TS::rows(law, q1, qp, 0, DIM, F);
+                    TS::rows(law, qm, qp, 0, DIM, F);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     R v = F[0][s] * R(sg[0]);
@@ -393,26 +401,49 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
                         v = v + F[cc][s] * R(sg[cc * NFP]);
                     nf[s] = v;
                 }
-                lift = R(sg[DIM * NFP] * MJI);
+                lift = R(sg[DIM * NFP] * P.mji[fe * NP + nm]);
             }
         } else if constexpr (GEO == GEO_AFFINE) {
             R fm[NS];
-            law.flux_row(d, q1, fm);
-            S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, q1, qp, nf);
-            lift = R(P.affine[e * DIM + d] * P.winv[(f & 1) ? NQ - 1 : 0]);
+            law.flux_row(d, qm, fm);
+            S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, qm, qp, nf);
+            lift = R(P.affine[fe * DIM + d] * P.winv[side]);
         } else {
-            const double *sg = P.sgeo + ((e * NF + f) * (DIM + 1)) * NFP + m;
+            const double *sg = P.sgeo + ((fe * NF + f) * (DIM + 1)) * NFP + m;
             R nv[DIM];
 #pragma unroll
             for (int cc = 0; cc < DIM; ++cc)
                 nv[cc] = R(sg[cc * NFP]);
-            S::nflux_general(law, nv, q1, qp, nf);
-            lift = R(sg[DIM * NFP] * MJI);
+            S::nflux_general(law, nv, qm, qp, nf);
+            lift = R(sg[DIM * NFP] * P.mji[fe * NP + nm]);
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-            acc[s] = acc[s] - lift * nf[s];
+            fs[((fc * NF + f) * NS + s) * NFP + m] = lift * nf[s];
     }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int d = f >> 1;
+        if (!((P.dir_mask >> d) & 1))
+            continue;
+        if (ijk[d] != ((f & 1) ? NQ - 1 : 0))
+            continue;
+        int m = 0;
+        if constexpr (DIM == 2)
+            m = d == 0 ? ijk[1] : ijk[0];
+        if constexpr (DIM == 3)
+            m = d == 0 ? ijk[1] + NQ * ijk[2] : d == 1 ? ijk[0] + NQ * ijk[2] : ijk[0] + NQ * ijk[1];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            acc[s] = acc[s] - fs[((c * NF + f) * NS + s) * NFP + m];
+    }
+    // the node's own state back from LDS (not held in registers through the
+    // pair and face passes)
+    R q1[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        q1[s] = qs[c][s][node];
 
     // ---- update
     if constexpr (RX) {
@@ -427,8 +458,9 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
         }
         const double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
         // the workgroup's sum: a fixed tree over the threads (in the pair
-        // buffer, free since the last pass ended with a barrier)
+        // buffer, once every thread has read its face terms)
         double *red = reinterpret_cast<double *>(pool);
+        __syncthreads();
         red[t] = prod;
         __syncthreads();
         for (int w = 256; w > 0; w >>= 1) {
