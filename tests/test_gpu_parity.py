@@ -597,6 +597,49 @@ def test_mortar_stage_on_adapted_grid_matches_oracle(torch, R, nq, dtype):
         assert np.all(np.abs(tot) <= 1e-11 * scale)
 
 
+def test_mortar_stage_with_flux_differencing_matches_oracle(torch, R):
+    """FluxDifferencingForm(EntropyConservativeFlux()) on the warped 2:1 forest
+    with mortar-coupled hanging faces: the flux-differencing stage kernel
+    followed by the mortar kernel against the oracle (its flux-differencing
+    right-hand side plus its mortar restatement), FP64, Nq = 6."""
+    nq = 6
+    grid = _adapted_grid(R, 3, nq, 6, 2, 2.1, _warp10)
+    ec = R.EntropyConservativeFlux()
+    dg = R.DGSEM(law=R.EulerLaw(3), grid=grid, surface_numericalflux=R.RusanovFlux(),
+                 volume_form=R.FluxDifferencingForm(ec))
+    dg.enable_mortars(grid)
+    K, nfp = grid.K, nq * nq
+    ag = _ArrayGrid(grid)
+    ag.points = grid.points
+    surf = np.array(grid.surface).reshape(K, 6, nfp, 5)
+    surf_full = surf.reshape(-1).copy()
+    surf[grid.face_kind != 0, :, 4] = 0.0
+    ag.surf = surf.reshape(-1)
+    odg = O.DG(ag, "euler", volume_flux="ec")
+    lower, upper = np.zeros(nq * nq), np.zeros(nq * nq)
+    R._lib.check(R.lib().rv_lobatto_halves(nq, R._lib.ptr(lower, R._lib.ctypes.c_double),
+                                           R._lib.ptr(upper, R._lib.ctypes.c_double)))
+    rng = np.random.default_rng(13)
+    P = grid.points
+    q = PB.isentropic_vortex_3d(P, 0.0, centre=(5.0, 5.0),
+                                noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    dq0 = 1e-3 * rng.standard_normal(q.size)
+    a, bdt = -0.6, 2e-3
+    L_o = np.zeros(q.size)
+    odg.rhs(L_o, q.copy(), 0.0, increment=False)
+    odg.mortar_add(grid.face_kind, grid.mortar_partner, lower, upper, surf_full, q, L_o)
+    want_dq = a * dq0 + L_o
+    want_q = q + bdt * want_dq
+    qd, dqd = to_dev(torch, dg, q), to_dev(torch, dg, dq0)
+    qo = torch.empty_like(qd)
+    dg.lsrk_stage(qo, qd, dqd, a, bdt)
+    dg.synchronize()
+    e1 = rel_l2(dqd.cpu().numpy().reshape(-1), want_dq)
+    e2 = rel_l2(qo.cpu().numpy().reshape(-1), want_q)
+    say(f"   flux differencing + mortars: dq {e1:.2e}, q_out {e2:.2e}")
+    assert e1 <= RTOL_F64 and e2 <= RTOL_F64
+
+
 @pytest.mark.parametrize("dim,nq", [(2, 5), (3, 5), (3, 8), (1, 6)])
 def test_min_node_distance_matches_oracle(torch, R, dim, nq):
     """min_node_distance(grid; dims) (grids.jl:94-145) on the warped bricks of
