@@ -224,7 +224,9 @@ int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstag
  * Newton iterations taken.  dq carries between steps like rv_lsrk_stage's
  * (not read at a stage with rka = 0); k and q0 are work arrays.  All four
  * arrays have the state's shape and must be distinct.  Blocks on the handle's
- * stream (each Newton iteration reads back its two sums). */
+ * stream (each Newton iteration reads back its two sums).  A Newton update
+ * that is not finite (a vanishing derivative) fails with RV_E_STATE and
+ * leaves q at the last iterate q0 + gamma k, q0 holding the step's start. */
 int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme, double time,
                   double dt, double *gamma, int32_t *iterations);
 
