@@ -121,7 +121,7 @@ struct RelaxParams {
     double gas_gamma;     // the Euler law's ratio of specific heats
     const double *MJ;     // (K, Np)
     void *q, *k, *dq;     // (K, Ns, Np) states of the handle's precision
-    const void *q0;
+    void *q0;             // read by RELAX_NEWTON / RELAX_AXPY, written by RELAX_ETA
     double a, b_dt;       // RELAX_STAGE
     int read_dq;          // RELAX_STAGE: a != 0
     int first;            // RELAX_NEWTON: form k = qf - q0 first

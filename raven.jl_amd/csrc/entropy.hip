@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
     R *q = static_cast<R *>(P.q);
     R *k = static_cast<R *>(P.k);
     R *dq = static_cast<R *>(P.dq);
-    const R *q0 = static_cast<const R *>(P.q0);
+    R *q0 = static_cast<R *>(P.q0);
     const R *qf = static_cast<const R *>(P.qf);
     const R a = R(P.a), bdt = R(P.b_dt), gam = R(P.relax);
     const double g = P.gas_gamma;
@@ -59,12 +59,11 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
         const int64_t base = node + Np * NS * e;
         double x[NS], kk[NS];
         if constexpr (MODE == RELAX_ETA) {
-            R *copy = const_cast<R *>(q0);
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 const R v = q[base + Np * s];
-                if (copy)
-                    copy[base + Np * s] = v;
+                if (q0)
+                    q0[base + Np * s] = v;
                 x[s] = (double)v;
             }
             eta = cadd(eta, P.MJ[i] * law_entropy<LAW, DIM>(x, g));
