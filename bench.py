@@ -168,6 +168,7 @@ def cpu_baseline(q0, dt, budget_s):
     os.environ["OMP_NUM_THREADS"] = str(cores)
     import problems as PB
     g = O.BrickGrid(NQ, [PB.julia_range(0, 10, NCELL + 1)] * 3, (True,) * 3)
+    _ORACLE_GRID["grid"], _ORACLE_GRID["cores"] = g, cores
     dg = O.DG(g, "euler", gamma=GAMMA)
     dg.set_threads(cores)
     q = q0.copy()
@@ -194,6 +195,43 @@ def cpu_baseline(q0, dt, budget_s):
     return {"value": dofs * stages / t_total, "unit": "DOF-updates/s", "cores": cores,
             "kind": "port", "sample": sample, "seconds": round(t_total, 3),
             "note": "C restatement of Raven's CPU path (oracle/), not Raven itself"}
+
+
+_ORACLE_GRID = {}
+
+
+def cpu_baseline_fd(q0, dt):
+    """The CPU oracle on the flux-differencing and relaxation workloads of the
+    same grid and state (after cpu_baseline, whose oracle grid it reuses): one
+    LSRK54 stage of FluxDifferencingForm(EntropyConservativeFlux()) with
+    Rusanov faces, and one RLSRK54 step with the entropy-conservative pairing."""
+    import oracle as O
+    g, cores = _ORACLE_GRID["grid"], _ORACLE_GRID["cores"]
+    dofs = g.K * g.Np * 5
+    fd = O.DG(g, "euler", gamma=GAMMA, volume_flux="ec")
+    fd.set_threads(cores)
+    q, dq = q0.copy(), np.zeros_like(q0)
+    t0 = time.perf_counter()
+    fd.rhs(dq, q, 0.0, increment=True)
+    fd_s = time.perf_counter() - t0
+    out_fd = {"ms_per_stage": fd_s * 1e3, "dof_updates_per_s": dofs / fd_s, "cores": cores,
+              "kind": "port", "sample": "one right-hand side (a stage's work) of the full "
+                                       "32^3 Nq=8 FP64 flux-differencing workload"}
+    del fd
+    rl = O.DG(g, "euler", gamma=GAMMA, volume_flux="ec", surface_flux="ec")
+    rl.set_threads(cores)
+    q = q0.copy()
+    n = q.size
+    import ctypes
+    gam = ctypes.c_double(1.0)
+    t0 = time.perf_counter()
+    O.lib().orc_dg_rlsrk_step(rl.h, 54, q, np.zeros(n), np.zeros(n), np.zeros(n), 0.0, dt,
+                              ctypes.byref(gam))
+    rl_s = time.perf_counter() - t0
+    out_rl = {"ms_per_step": rl_s * 1e3, "cores": cores, "kind": "port",
+              "sample": "one RLSRK54 step of the full 32^3 Nq=8 FP64 workload, "
+                        "entropy-conservative volume and surface fluxes"}
+    return out_fd, out_rl
 
 
 def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
@@ -630,6 +668,13 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(q0, dt, args.cpu_budget)
+        if "fluxdiff" in out or "rlsrk" in out:
+            log("bench: CPU baseline of the flux-differencing and relaxation legs (oracle)")
+            cfd, crl = cpu_baseline_fd(q0, dt)
+            if "fluxdiff" in out:
+                out["fluxdiff"]["cpu_baseline"] = cfd
+            if "rlsrk" in out:
+                out["rlsrk"]["cpu_baseline"] = crl
     print(json.dumps(out), flush=True)
 
 

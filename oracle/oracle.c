@@ -1336,6 +1336,10 @@ double orc_dg_entropy_integral(const orc_dg *dg, const double *q)
     const int dim = dg->dim, ns = dg->ns, L = dim * dim + 2;
     const int64_t Np = dg->Np;
     double sum = 0;
+#ifdef _OPENMP
+    const int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
+#endif
+#pragma omp parallel for schedule(static) reduction(+ : sum) num_threads(nth)
     for (int64_t e = 0; e < dg->K; ++e)
         for (int64_t n = 0; n < Np; ++n) {
             double qv[5];
@@ -1352,6 +1356,10 @@ double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q
     const int dim = dg->dim, ns = dg->ns, L = dim * dim + 2;
     const int64_t Np = dg->Np;
     double sum = 0;
+#ifdef _OPENMP
+    const int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
+#endif
+#pragma omp parallel for schedule(static) reduction(+ : sum) num_threads(nth)
     for (int64_t e = 0; e < dg->K; ++e)
         for (int64_t n = 0; n < Np; ++n) {
             double pv[5], v[5], dot;
