@@ -756,6 +756,7 @@ struct orc_dg {
     int dim, nq, ns, law, nthreads;
     int volume_form, volume_flux;  /* 0 weak / 1 flux differencing; 2 central / 3 EC */
     int surface_flux;              /* 1 Rusanov (default), 2 central, 3 EC */
+    int dir_mask;                  /* directions (bit d: direction d + 1), all by default */
     int64_t K, Np, nfp;
     double gamma, u[3];
     const double *D, *metrics, *surf;
@@ -788,6 +789,7 @@ orc_dg *orc_dg_new(int dim, int nq, int64_t K, int law, double gamma, const doub
     dg->nfp = face_nodes(dim, nq);
     dg->nthreads = 0;
     dg->surface_flux = 1;
+    dg->dir_mask = (1 << dim) - 1;
     /* MJI = 1 ./ MJ with MJ = wJ (dgsem.jl:55-56) */
     const int L = dim * dim + 2;
     dg->MJI = malloc(sizeof(double) * dg->Np * K);
@@ -814,6 +816,12 @@ void orc_dg_set_volume_form(orc_dg *dg, int form, int volume_flux)
 }
 
 void orc_dg_set_surface_flux(orc_dg *dg, int flux) { dg->surface_flux = flux; }
+
+/* DGSEM(...; directions) (dgsem.jl:32-36, 46-81): the volume term keeps the
+ * derivatives along these directions (volumeterm! :298-340, the per-direction
+ * loop of esvolumeterm_dir! :207-224) and the surface term their faces
+ * (surfaceterm! :756) */
+void orc_dg_set_directions(orc_dg *dg, int mask) { dg->dir_mask = mask; }
 void orc_dg_set_threads(orc_dg *dg, int nthreads) { dg->nthreads = nthreads; }
 
 /* flux(law, q) as a dim x Ns matrix F[dd + dim*s]
@@ -972,8 +980,8 @@ static void twopoint(const orc_dg *dg, int flux, const double *q1, const double 
 /* FluxDifferencingForm volume term, one direction at a time as
  * esvolumeterm_dir! (dgsem.jl:514-608): for every node and line partner n,
  *   dq -= MJI D[id,n] l_g(ijk) . f  and  dq += f . l_g(n) MJI D[n,id],
- * l_g = MJ dRdX[dir, :]; the first direction sets (or increments) dq, the
- * others add to it. */
+ * l_g = MJ dRdX[dir, :]; the first of the operator's directions sets (or
+ * increments) dq, the others add to it. */
 static void fd_volume(orc_dg *dg, double *dq, const double *q, int increment, int nth)
 {
     const int dim = dg->dim, nq = dg->nq, ns = dg->ns;
@@ -981,8 +989,12 @@ static void fd_volume(orc_dg *dg, double *dq, const double *q, int increment, in
     const int L = dim * dim + 2;
     const double *D = dg->D;
     (void)nth;
+    int first = 1;
     for (int dir = 0; dir < dim; ++dir) {
-        const int inc = dir == 0 ? increment : 1;
+        if (!((dg->dir_mask >> dir) & 1))
+            continue;
+        const int inc = first ? increment : 1;
+        first = 0;
 #pragma omp parallel for schedule(static) num_threads(nth)
         for (int64_t e = 0; e < K; ++e) {
             for (int64_t node = 0; node < Np; ++node) {
@@ -1077,15 +1089,17 @@ void orc_dg_rhs(orc_dg *dg, double *dq, const double *q, double t, int increment
                 double acc[5];
                 for (int s = 0; s < ns; ++s)
                     acc[s] = -0.0;
+                const int m = dg->dir_mask;
                 for (int n = 0; n < nq; ++n) {
                     double Dni = D[n + nq * i] * MJI;
                     double Dnj = D[n + nq * j] * MJI;
                     double Dnk = D[n + nq * k] * MJI;
                     for (int s = 0; s < ns; ++s) {
-                        acc[s] += Dni * lF[(n + nq * j + (int64_t)nq * nq * k) + Np * (0 + dim * s)];
-                        if (dim > 1)
+                        if (m & 1)
+                            acc[s] += Dni * lF[(n + nq * j + (int64_t)nq * nq * k) + Np * (0 + dim * s)];
+                        if (dim > 1 && (m & 2))
                             acc[s] += Dnj * lF[(i + nq * n + (int64_t)nq * nq * k) + Np * (1 + dim * s)];
-                        if (dim > 2)
+                        if (dim > 2 && (m & 4))
                             acc[s] += Dnk * lF[(i + nq * j + (int64_t)nq * nq * n) + Np * (2 + dim * s)];
                     }
                 }
@@ -1104,6 +1118,8 @@ surface:;
     for (int64_t e = 0; e < K; ++e)
         for (int face = 0; face < nfaces; ++face)
             for (int64_t m = 0; m < nfp; ++m) {
+                if (!((dg->dir_mask >> (face / 2)) & 1))
+                    continue;
                 int64_t slot = face * nfp + m + nfpt * e;
                 int64_t idm = dg->vmapM[slot] - 1;
                 const double *nv = dg->surf + (dim + 2) * slot;

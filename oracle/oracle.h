@@ -120,6 +120,8 @@ void orc_dg_set_volume_form(orc_dg *dg, int form, int volume_flux);
 /* surface numerical flux: 1 Rusanov, 2 central, 3 entropy conservative
  * (the last two as twopointflux(q-, q+)' n, numericalfluxes.jl:27-37) */
 void orc_dg_set_surface_flux(orc_dg *dg, int flux);
+/* DGSEM(...; directions) as a bit mask (bit d: direction d + 1) */
+void orc_dg_set_directions(orc_dg *dg, int mask);
 /* sum MJ eta(q) and sum MJ v(p).q (measures.jl:23-33) */
 double orc_dg_entropy_integral(const orc_dg *dg, const double *q);
 double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q);

@@ -67,6 +67,7 @@ def lib():
         L.orc_dg_solve.restype = ctypes.c_int64
         L.orc_dg_weightednorm.argtypes = [ctypes.c_void_p, f64p, f64p]
         L.orc_dg_set_surface_flux.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_dg_set_directions.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_dg_entropy_integral.argtypes = [ctypes.c_void_p, f64p]
         L.orc_dg_entropy_integral.restype = ctypes.c_double
         L.orc_dg_entropy_product.argtypes = [ctypes.c_void_p, f64p, f64p]
@@ -170,10 +171,12 @@ FLUXES = {"rusanov": 1, "central": 2, "ec": 3}
 class DG:
     """Oracle DGSEM operator for law 'advection' or 'euler'."""
 
-    def __init__(self, grid, law, gamma=1.4, u=None, volume_flux=None, surface_flux="rusanov"):
+    def __init__(self, grid, law, gamma=1.4, u=None, volume_flux=None, surface_flux="rusanov",
+                 directions=None):
         """volume_flux None: WeakForm; "central" or "ec": FluxDifferencingForm
         with CentralFlux or EntropyConservativeFlux (dgsem.jl:1-17).
-        surface_flux "rusanov", "central" or "ec" (numericalfluxes.jl:27-37)."""
+        surface_flux "rusanov", "central" or "ec" (numericalfluxes.jl:27-37).
+        directions: DGSEM's `directions` (1-based), all by default."""
         self.grid = grid
         self.law = {"advection": 1, "euler": 2}[law]
         u = np.zeros(3) if u is None else np.asarray(u, dtype=np.float64)
@@ -186,6 +189,8 @@ class DG:
         if volume_flux is not None:
             lib().orc_dg_set_volume_form(self.h, 1, {"central": 2, "ec": 3}[volume_flux])
         lib().orc_dg_set_surface_flux(self.h, FLUXES[surface_flux])
+        if directions is not None:
+            lib().orc_dg_set_directions(self.h, sum(1 << (d - 1) for d in directions))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -735,3 +735,40 @@ def test_c_driver_reproduces_solve_and_dostep(tmp_path):
     err = rel_l2(out["run"][1], q)
     say(f"   vs oracle: rel L2 {err:.2e} ({n} steps to {t})")
     assert n == 20 and err <= 1e-12
+
+
+@pytest.mark.parametrize("dim,nq,law,vflux,geometry,dirs", [
+    (2, 5, "euler", None, "auto", (2,)),
+    (2, 6, "advection", None, "general", (1,)),
+    (3, 8, "euler", None, "auto", (1, 2)),
+    (3, 8, "euler", None, "auto", (3,)),
+    (3, 5, "euler", None, "general", (2, 3)),
+    (3, 6, "euler", "ec", "auto", (3,)),
+    (3, 4, "euler", "central", "general", (1, 2)),
+])
+def test_directions_match_oracle(torch, R, dim, nq, law, vflux, geometry, dirs):
+    """DGSEM(...; directions) (dgsem.jl:32-36): the volume derivatives and the
+    faces of the given directions only, weak and flux-differencing forms."""
+    say(f"directions {dirs}: dim={dim} nq={nq} {law} {vflux or 'weak'} {geometry}")
+    coords = [PB.julia_range(-1, 1, 4)] * dim
+    u = [1.0, 0.5, 0.25][:dim] if law == "advection" else None
+    cell = R.LobattoCell(*([nq] * dim))
+    grid = R.generate(R.GridManager(cell, R.brick(coords, (True,) * dim)))
+    lawobj = R.AdvectionLaw(dim, u) if law == "advection" else R.EulerLaw(dim)
+    vf = None
+    if vflux is not None:
+        vf = R.FluxDifferencingForm(R.EntropyConservativeFlux() if vflux == "ec"
+                                    else R.CentralFlux())
+    dg = R.DGSEM(law=lawobj, grid=grid, surface_numericalflux=R.RusanovFlux(),
+                 geometry=geometry, directions=dirs, volume_form=vf)
+    og = O.BrickGrid(nq, coords, (True,) * dim)
+    odg = O.DG(og, law, u=u, volume_flux=vflux, directions=dirs)
+    q = initial_state(og, law, dim, seed=21)
+    want = np.zeros(q.size)
+    odg.rhs(want, q.copy(), 0.0, increment=False)
+    qd, dqd = to_dev(torch, dg, q), dg.GridArray(0.0)
+    dg(dqd, qd, 0.0, increment=False)
+    dg.synchronize()
+    err = rel_l2(dqd.cpu().numpy().reshape(-1), want)
+    say(f"   rel L2 {err:.2e}")
+    assert err <= RTOL_F64

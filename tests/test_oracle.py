@@ -555,3 +555,35 @@ def test_min_node_distance_closed_form(dim, nq):
     assert g.min_node_distance() == pytest.approx(hmnd, rel=1e-12)
     assert g.min_node_distance(dims=(dim,)) == pytest.approx(vmnd, rel=1e-12)
     assert g.min_node_distance(dims=range(1, dim)) == pytest.approx(hmnd, rel=1e-12)
+
+
+# --------------------------------------------------------------------------
+# directions (dgsem.jl:32-36, 298-340, 756)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("law,dim,vflux", [("euler", 3, None), ("euler", 2, None),
+                                           ("advection", 3, None), ("euler", 3, "ec"),
+                                           ("euler", 2, "central")])
+def test_directions_split_the_right_hand_side(law, dim, vflux):
+    """Volume and surface terms are sums over directions, so the right-hand
+    sides of a partition of the directions add up to the full one (to
+    rounding), and each part differs from the full one: a mask ignored, or a
+    volume or surface term taken from the wrong direction, fails this."""
+    coords = [PB.julia_range(-1, 1, 4)] * dim
+    g = O.BrickGrid(4, coords, [True] * dim, warp=None)
+    u = [1.0, 0.5, 0.25][:dim] if law == "advection" else None
+    P = g.point_fields()
+    rng = np.random.default_rng(5)
+    if law == "advection":
+        q = PB.advection_wave(P, dim, 0.0) + 0.05 * rng.standard_normal(g.K * g.Np)
+    else:
+        q = PB.euler_wave(P, dim, 0.0) + 1e-2 * rng.standard_normal(g.K * g.Np * (dim + 2))
+    full = np.zeros(q.size)
+    O.DG(g, law, u=u, volume_flux=vflux).rhs(full, q.copy(), increment=False)
+    parts = [tuple(range(1, dim)), (dim,)]
+    tot = np.zeros(q.size)
+    for dirs in parts:
+        r = np.zeros(q.size)
+        O.DG(g, law, u=u, volume_flux=vflux, directions=dirs).rhs(r, q.copy(), increment=False)
+        assert np.linalg.norm(r - full) > 1e-3 * np.linalg.norm(full)
+        tot += r
+    assert np.linalg.norm(tot - full) <= 1e-12 * np.linalg.norm(full)
