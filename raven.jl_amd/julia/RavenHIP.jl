@@ -84,7 +84,7 @@ are read once (`Array(...)` of device arrays is fine); the library keeps
 device copies and per-face neighbour records.
 """
 function HIPDGSEM(; law, grid, surface_numericalflux, volume_form = WeakForm(),
-                  dtype = Float64, stream = C_NULL)
+                  directions = 1:ndims(law), dtype = Float64, stream = C_NULL)
     cell = referencecell(grid)
     dim = ndims(cell)
     Nq = size(cell, 1)
@@ -103,8 +103,10 @@ function HIPDGSEM(; law, grid, surface_numericalflux, volume_form = WeakForm(),
     handle = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve D w metrics surf vmapM vmapP bc dtoc begin
         form, vflux = formid(volume_form)
+        # DGSEM's `directions` (dgsem.jl:46-81) as the descriptor's bit mask
+        dirmask = Int32(sum(1 << (d - 1) for d in directions))
         desc = RvDesc(2, dim, Nq, lawid(law), fluxid(surface_numericalflux),
-                      dtype == Float64 ? RV_F64 : RV_F32, 0, 0, length(grid), γ, vel,
+                      dtype == Float64 ? RV_F64 : RV_F32, 0, dirmask, length(grid), γ, vel,
                       pointer(D), pointer(w), pointer(metrics), pointer(surf),
                       pointer(vmapM), pointer(vmapP), pointer(bc), pointer(dtoc),
                       numcnodes(grid), stream, form, vflux)
@@ -162,6 +164,37 @@ function rlsrk_dostep!(q, rlsrk, dg::HIPDGSEM; scheme = 54)
     rlsrk.γ = γ[]
     rlsrk.time += rlsrk.dt
     return iters[]
+end
+
+# Resident device states for a host without AMDGPU.jl (rv_state_*): a state
+# of the operator's shape and precision, uploaded from / downloaded to a host
+# array in Raven's parent layout.
+function state_alloc(dg::HIPDGSEM)
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:rv_state_alloc, libravenhip), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                dg.handle, p), dg.handle)
+    return p[]
+end
+state_free(dg::HIPDGSEM, p::Ptr{Cvoid}) =
+    check(ccall((:rv_state_free, libravenhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), dg.handle, p),
+          dg.handle)
+state_upload!(dg::HIPDGSEM, p::Ptr{Cvoid}, host::Array) =
+    check(ccall((:rv_state_upload, libravenhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dg.handle, p, host), dg.handle)
+state_download!(dg::HIPDGSEM, host::Array, p::Ptr{Cvoid}) =
+    check(ccall((:rv_state_download, libravenhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dg.handle, host, p), dg.handle)
+
+# min_node_distance(grid; dims) (grids.jl:94-145) on the device: `points` is
+# the device copy of parent(points(grid)) (Np, dim, K), FP64.
+function min_node_distance(points_device::Ptr{Cvoid}, dim, Nq, K; dims = 1:dim,
+                           stream = C_NULL)
+    out = Ref{Cdouble}(0)
+    mask = Int32(sum(1 << (d - 1) for d in dims))
+    check(ccall((:rv_min_node_distance, libravenhip), Cint,
+                (Cint, Cint, Int64, Ptr{Cvoid}, Int32, Ref{Cdouble}, Ptr{Cvoid}),
+                dim, Nq, K, points_device, mask, out, stream), C_NULL)
+    return out[]
 end
 
 end # module
