@@ -74,11 +74,17 @@ def _rlsrk_dostep(q, solver):
     solver.time += solver.dt
 
 
-def solve_(q, timeend, solver, after_step=None):
-    """solve!(q, timeend, solver) with adjust_final = true; returns #steps.
-    after_step(step, time, q) is called after every step of an RLSRK solver
-    (LSRK solvers run all their steps inside the library and take none)."""
+def solve_(q, timeend, solver, after_step=None, after_stage=None):
+    """solve!(q, timeend, solver; after_step, after_stage) with adjust_final =
+    true (odesolvers.jl:9-31); returns #steps.  Without callbacks an LSRK solve
+    runs inside the library (rv_lsrk_run).  With them the steps are taken from
+    here: after_step(step, time, q) after every step (time at the step's
+    start), after_stage(stagetime, q_stage) after every LSRK stage, q_stage
+    being the stage's output (q itself after a step's last stage).  An RLSRK
+    step is one library call, so RLSRK solvers take after_step only."""
     if isinstance(solver, RLSRK):
+        if after_stage is not None:
+            raise ValueError("RLSRK steps are single library calls: no after_stage")
         step = 0
         while True:
             step += 1
@@ -91,19 +97,33 @@ def solve_(q, timeend, solver, after_step=None):
                 after_step(step, time, q)
             if final:
                 return step
-    if after_step is not None:
-        raise ValueError("after_step needs an RLSRK solver")
-    t = ctypes.c_double(0.0)
-    n = ctypes.c_int64(0)
-    dg = solver.dg
-    L.check(L.lib().rv_lsrk_run(dg.handle, dg._check_state(q), dg._check_state(solver.dq),
-                                solver.scheme, solver.time, solver.dt, float(timeend),
-                                ctypes.byref(t), ctypes.byref(n)), dg.handle)
-    solver.time = t.value
-    return n.value
+    if after_step is None and after_stage is None:
+        t = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        dg = solver.dg
+        L.check(L.lib().rv_lsrk_run(dg.handle, dg._check_state(q), dg._check_state(solver.dq),
+                                    solver.scheme, solver.time, solver.dt, float(timeend),
+                                    ctypes.byref(t), ctypes.byref(n)), dg.handle)
+        solver.time = t.value
+        return n.value
+    # LSRK's dq starts at zero (odesolvers.jl:43-44), as rv_lsrk_run makes it
+    solver.dq.zero_()
+    qtmp = q.new_empty(q.shape)
+    step = 0
+    while True:
+        step += 1
+        time = solver.time
+        final = time + solver.dt >= timeend
+        if final:
+            solver.dt = timeend - time
+        dostep_(q, solver, qtmp, after_stage=after_stage)
+        if after_step is not None:
+            after_step(step, time, q)
+        if final:
+            return step
 
 
-def dostep_(q, solver, qtmp=None):
+def dostep_(q, solver, qtmp=None, after_stage=None):
     """dostep!(q, lsrk) (odesolvers.jl:50-61) through the per-stage entry
     point, stage times time + rkc*dt; q updated in place (qtmp is the
     ping-pong buffer; an odd stage count ends with a copy).  An RLSRK solver
@@ -127,6 +147,11 @@ def dostep_(q, solver, qtmp=None):
         solver.dg.lsrk_stage(alt, cur, solver.dq, a[s], bdt[s],
                              solver.time + c[s] * solver.dt)
         cur, alt = alt, cur
+        if after_stage is not None:
+            if s == len(a) - 1 and cur is not q:
+                q.copy_(cur)
+                cur = q
+            after_stage(solver.time + c[s] * solver.dt, cur)
     if cur is not q:
         q.copy_(cur)
     solver.time += solver.dt

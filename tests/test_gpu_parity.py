@@ -772,3 +772,29 @@ def test_directions_match_oracle(torch, R, dim, nq, law, vflux, geometry, dirs):
     err = rel_l2(dqd.cpu().numpy().reshape(-1), want)
     say(f"   rel L2 {err:.2e}")
     assert err <= RTOL_F64
+
+
+def test_solve_with_callbacks_matches_library_loop(torch, R):
+    """solve!(q, T, LSRK54; after_step, after_stage) stepped from Python gives
+    the library loop's (rv_lsrk_run) state bit for bit, with one after_step per
+    step and one after_stage per stage, the last stage's state being q."""
+    say("solve_ with callbacks: 2-D Euler, Nq=5")
+    coords = [PB.julia_range(0, 2, 5)] * 2
+    grid, dg, og, odg = make_pair(R, 2, 5, coords, (True, True), "euler")
+    q0 = initial_state(og, "euler", 2, seed=31)
+    dt, T = 0.01, 0.035
+    qa = to_dev(torch, dg, q0)
+    na = R.solve_(qa, T, R.LSRK54(dg, qa, dt))
+    qb = to_dev(torch, dg, q0)
+    steps, stages = [], []
+    solver = R.LSRK54(dg, qb, dt)
+    nb = R.solve_(qb, T, solver, after_step=lambda n, t, q: steps.append((n, t)),
+                  after_stage=lambda t, q: stages.append((t, q.data_ptr())))
+    dg.synchronize()
+    assert na == nb == len(steps) == 4 and len(stages) == 5 * nb
+    assert stages[-1][1] == qb.data_ptr()
+    assert [t for _, t in steps] == pytest.approx([0.0, 0.01, 0.02, 0.03])
+    assert solver.time == pytest.approx(T)
+    assert torch.equal(qa, qb)
+    with pytest.raises(ValueError):
+        R.solve_(qb, T, R.RLSRK54(dg, qb, dt), after_stage=lambda t, q: None)
