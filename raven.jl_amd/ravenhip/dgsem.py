@@ -61,14 +61,23 @@ class WeakForm:
 
 
 class FluxDifferencingForm:
-    """FluxDifferencingForm(volume_numericalflux) (dgsem.jl:8-17): the volume
-    term from a two-point flux (CentralFlux, or EntropyConservativeFlux for
-    Euler), direction by direction as the :per_dir kernel (dgsem.jl:514-608)."""
+    """FluxDifferencingForm(volume_numericalflux, kernel_type = :per_dir)
+    (dgsem.jl:8-17): the volume term from a two-point flux (CentralFlux, or
+    EntropyConservativeFlux for Euler).  Raven's three kernel types (:naive,
+    :per_dir, :per_dir_symmetric; dgsem.jl:163-255) compute the same operator
+    in different orders; the library evaluates every symmetric pair once, as
+    :per_dir_symmetric does, for all of them.  :naive keeps Raven's
+    restriction to all directions (dgsem.jl:180-181)."""
     form_id = L.RV_FORM_FLUX_DIFFERENCING
+    KERNEL_TYPES = ("naive", "per_dir", "per_dir_symmetric")
 
-    def __init__(self, volume_numericalflux):
+    def __init__(self, volume_numericalflux, kernel_type="per_dir"):
+        kernel_type = kernel_type.lstrip(":")
+        if kernel_type not in self.KERNEL_TYPES:
+            raise ValueError(f"unknown kernel_type {kernel_type!r}")
         self.volume_numericalflux = volume_numericalflux
         self.flux_id = volume_numericalflux.flux_id
+        self.kernel_type = kernel_type
 
 
 _DTYPES = {"f64": L.RV_F64, "f32": L.RV_F32}
@@ -101,6 +110,10 @@ class DGSEM:
         self.dtype = dtype
         self.volume_form = volume_form or WeakForm()
         flux = surface_numericalflux or RusanovFlux()
+        if (getattr(self.volume_form, "kernel_type", None) == "naive" and directions is not None
+                and sorted(directions) != list(range(1, law.dim + 1))):
+            # esvolumeterm! (:naive) has no directions (dgsem.jl:180-181)
+            raise ValueError("the :naive flux-differencing kernel needs all directions")
         if _handle is not None:
             self._h = _handle
         else:

@@ -114,3 +114,14 @@ def test_lsrk_coefficients_of_the_library(scheme, nst, order):
 
     e1, e2 = run(1 / 8), run(1 / 16)
     assert np.log2(e1 / e2) > order - 0.3
+
+
+def test_flux_differencing_kernel_types():
+    """FluxDifferencingForm(flux, kernel_type) takes Raven's three kernel types
+    (dgsem.jl:8-17, 163-255) and rejects others."""
+    import ravenhip as R
+    for kt in ("naive", ":per_dir", "per_dir_symmetric"):
+        assert R.FluxDifferencingForm(R.CentralFlux(), kt).kernel_type == kt.lstrip(":")
+    assert R.FluxDifferencingForm(R.CentralFlux()).kernel_type == "per_dir"
+    with pytest.raises(ValueError):
+        R.FluxDifferencingForm(R.CentralFlux(), "per_cell")
