@@ -315,6 +315,10 @@ int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, 
 /* weightednorm(dg, q; componentwise = true) (measures.jl:6-14), p = 2, with
  * compensated (Neumaier) FP64 sums in a fixed order: out[s], s < Ns. */
 int rv_weightednorm(rv_handle *h, const void *q, double *out);
+/* weightednorm(dg, q, p; componentwise = true) (measures.jl:6-21) for any
+ * p > 0: (sum MJ |q_s|^p)^(1/p), compensated as above; p = +infinity gives
+ * max |q_s| (no weights, as Raven's Val{Inf} method).  out[s], s < Ns. */
+int rv_weightednorm_p(rv_handle *h, const void *q, double p, double *out);
 /* Compensated conservation integrals sum(MJ .* q_s): out[s], s < Ns. */
 int rv_integrals(rv_handle *h, const void *q, double *out);
 /* entropyintegral(dg, q) = sum(MJ .* entropy.(q)) (measures.jl:23-27) and

@@ -236,11 +236,12 @@ constexpr int RED_BLOCKS = 512;
 constexpr int RED_THREADS = 256;
 
 // partial[s * RED_BLOCKS + block] = compensated sum over the block's share of
-// MJ * f(q_s); f = square (power 2) or identity (power 1)
+// MJ * f(q_s); f = square (power 2), identity (power 1) or |q|^p (power 3);
+// power 4: the block's max |q_s| (in the sum slot, compensation 0)
 template <typename R>
 __global__ void __launch_bounds__(RED_THREADS)
     reduce_kernel(const R *q, const double *MJ, int64_t Np, int ns, int64_t K, int power,
-                  double *partial)
+                  double pexp, double *partial)
 {
     __shared__ Comp sh[RED_THREADS];
     const int64_t n = Np * K;
@@ -252,14 +253,24 @@ __global__ void __launch_bounds__(RED_THREADS)
     for (int64_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) {
         const int64_t e = i / Np, node = i - e * Np;
         const double v = (double)q[node + Np * (s + (int64_t)ns * e)];
-        const double x = power == 2 ? MJ[i] * (v * v) : MJ[i] * v;
+        if (power == 4) {
+            acc.s = fmax(acc.s, fabs(v));
+            continue;
+        }
+        const double x = power == 2   ? MJ[i] * (v * v)
+                         : power == 3 ? MJ[i] * pow(fabs(v), pexp)
+                                      : MJ[i] * v;
         acc = comp_add(acc, x);
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            sh[threadIdx.x] = comp_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
+        if ((int)threadIdx.x < w) {
+            if (power == 4)
+                sh[threadIdx.x].s = fmax(sh[threadIdx.x].s, sh[threadIdx.x + w].s);
+            else
+                sh[threadIdx.x] = comp_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -835,7 +846,7 @@ int launch(rv_handle *h, const StageParams &P, StageLauncher fn = nullptr)
     return RV_OK;
 }
 
-int reduce(rv_handle *h, const void *q, int power, double *out)
+int reduce(rv_handle *h, const void *q, int power, double *out, double pexp = 2.0)
 {
     if (!q)
         return fail(h, RV_E_ARG, "null state");
@@ -843,17 +854,24 @@ int reduce(rv_handle *h, const void *q, int power, double *out)
     if (h->dtype == RV_F64)
         hipLaunchKernelGGL(reduce_kernel<double>, grid, dim3(RED_THREADS), 0, h->stream,
                            static_cast<const double *>(q), h->MJ, h->Np, h->ns, h->K, power,
-                           h->partials);
+                           pexp, h->partials);
     else
         hipLaunchKernelGGL(reduce_kernel<float>, grid, dim3(RED_THREADS), 0, h->stream,
                            static_cast<const float *>(q), h->MJ, h->Np, h->ns, h->K, power,
-                           h->partials);
+                           pexp, h->partials);
     HIP_TRY(h, hipGetLastError());
     std::vector<double> part((size_t)2 * RED_BLOCKS * h->ns);
     HIP_TRY(h, hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double),
                               hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     for (int s = 0; s < h->ns; ++s) {
+        if (power == 4) {
+            double m = 0.0;
+            for (int b = 0; b < RED_BLOCKS; ++b)
+                m = std::fmax(m, part[2 * (s * RED_BLOCKS + b)]);
+            out[s] = m;
+            continue;
+        }
         // Neumaier over the block partials in block order
         double sum = 0, comp = 0;
         for (int b = 0; b < RED_BLOCKS; ++b) {
@@ -1649,6 +1667,22 @@ int rv_weightednorm(rv_handle *h, const void *q, double *out)
         return rc;
     for (int s = 0; s < h->ns; ++s)
         out[s] = std::sqrt(out[s]);
+    return RV_OK;
+}
+
+int rv_weightednorm_p(rv_handle *h, const void *q, double p, double *out)
+{
+    if (!h || !out)
+        return fail(h, RV_E_ARG, "null argument");
+    if (std::isinf(p) && p > 0)
+        return reduce(h, q, 4, out);
+    if (!(p > 0) || !std::isfinite(p))
+        return fail(h, RV_E_ARG, "weightednorm: p must be positive");
+    int rc = reduce(h, q, 3, out, p);
+    if (rc)
+        return rc;
+    for (int s = 0; s < h->ns; ++s)
+        out[s] = std::pow(out[s], 1.0 / p);
     return RV_OK;
 }
 

@@ -122,6 +122,7 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, c_f64p, c_i64p]),
         "rv_weightednorm": (ctypes.c_int, [H, vp, c_f64p]),
         "rv_integrals": (ctypes.c_int, [H, vp, c_f64p]),
+        "rv_weightednorm_p": (ctypes.c_int, [H, vp, ctypes.c_double, c_f64p]),
         "rv_entropy_integral": (ctypes.c_int, [H, vp, c_f64p]),
         "rv_entropy_product": (ctypes.c_int, [H, vp, vp, c_f64p]),
         "rv_rlsrk_step": (ctypes.c_int, [H, vp, vp, vp, vp, ctypes.c_int, ctypes.c_double,

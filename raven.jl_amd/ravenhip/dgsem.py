@@ -306,7 +306,18 @@ def entropyproduct(dg, p, q):
     return dg.entropy_product(p, q)
 
 
-def weightednorm(dg, q, componentwise=False):
-    """weightednorm(dg, q) (measures.jl:6-14), p = 2, compensated FP64 sums."""
-    comp = dg.weightednorm_components(q)
-    return comp if componentwise else float(np.sqrt(np.sum(comp * comp)))
+def weightednorm(dg, q, p=2, componentwise=False):
+    """weightednorm(dg, q, p = 2; componentwise = false) (measures.jl:6-21):
+    per state (sum MJ |q_s|^p)^(1/p) (p = inf: max |q_s|), compensated FP64
+    sums, then the p-norm over the states unless componentwise."""
+    if p == 2:
+        comp = dg.weightednorm_components(q)
+    else:
+        comp = np.zeros(dg.ns)
+        L.check(L.lib().rv_weightednorm_p(dg.handle, dg._check_state(q), float(p),
+                                          L.ptr(comp, ctypes.c_double)), dg.handle)
+    if componentwise:
+        return comp
+    if p == 2:
+        return float(np.sqrt(np.sum(comp * comp)))
+    return float(np.linalg.norm(comp, p))

@@ -798,3 +798,32 @@ def test_solve_with_callbacks_matches_library_loop(torch, R):
     assert torch.equal(qa, qb)
     with pytest.raises(ValueError):
         R.solve_(qb, T, R.RLSRK54(dg, qb, dt), after_stage=lambda t, q: None)
+
+
+@pytest.mark.parametrize("p", [1, 3, 2.5, float("inf")])
+def test_weightednorm_any_p(torch, R, p):
+    """weightednorm(dg, q, p; componentwise) (measures.jl:6-21): a constant
+    state c has the closed form |c| * (sum MJ)^(1/p) (|c| for p = inf); a
+    random state matches the same formula in numpy on the oracle's MJ."""
+    say(f"weightednorm p={p}")
+    coords = [PB.julia_range(0, 2, 4)] * 3
+    grid, dg, og, odg = make_pair(R, 3, 4, coords, (True,) * 3, "euler")
+    c = np.array([1.3, -0.4, 0.2, 0.7, 2.9])
+    qc = np.tile(np.repeat(c, og.Np), og.K)
+    comp = R.weightednorm(dg, to_dev(torch, dg, qc), p=p, componentwise=True)
+    vol = 8.0
+    want = np.abs(c) if np.isinf(p) else np.abs(c) * vol ** (1.0 / p)
+    assert comp == pytest.approx(want, rel=1e-13)
+    q = initial_state(og, "euler", 3, seed=41)
+    comp = R.weightednorm(dg, to_dev(torch, dg, q), p=p, componentwise=True)
+    Q = q.reshape(og.K, 5, og.Np)
+    MJ = og.MJ
+    if np.isinf(p):
+        ref = np.abs(Q).max(axis=(0, 2))
+    else:
+        ref = np.array([np.sum(MJ * np.abs(Q[:, s, :]) ** p) ** (1.0 / p) for s in range(5)])
+    assert comp == pytest.approx(ref, rel=1e-12)
+    assert R.weightednorm(dg, to_dev(torch, dg, q), p=p) == pytest.approx(
+        float(np.linalg.norm(ref, p)), rel=1e-12)
+    with pytest.raises(R.RavenHIPError):
+        R.weightednorm(dg, to_dev(torch, dg, q), p=-1.0)
