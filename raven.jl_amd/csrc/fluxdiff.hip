@@ -456,20 +456,37 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
             if (P.read_dq)
                 acc[s] = a * dq[(e * NS + s) * NP + node] + acc[s];
         }
-        const double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
+        double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
         // the workgroup's sum: a fixed tree over the threads (in the pair
         // buffer, once every thread has read its face terms)
         double *red = reinterpret_cast<double *>(pool);
         __syncthreads();
-        red[t] = prod;
-        __syncthreads();
-        for (int w = 256; w > 0; w >>= 1) {
-            if (t < w && t + w < T::THREADS)
-                red[t] += red[t + w];
+        if constexpr (T::THREADS % 64 == 0) {
+            // whole waves: a butterfly inside each wave, then the wave sums
+            // in wave order
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                prod += __shfl_xor(prod, off);
+            if ((t & 63) == 0)
+                red[t >> 6] = prod;
             __syncthreads();
+            if (t == 0) {
+                double sum = red[0];
+                for (int w = 1; w < T::THREADS / 64; ++w)
+                    sum += red[w];
+                P.eprod[blockIdx.x] = sum;
+            }
+        } else {
+            red[t] = prod;
+            __syncthreads();
+            for (int w = 256; w > 0; w >>= 1) {
+                if (t < w && t + w < T::THREADS)
+                    red[t] += red[t + w];
+                __syncthreads();
+            }
+            if (t == 0)
+                P.eprod[blockIdx.x] = red[0];
         }
-        if (t == 0)
-            P.eprod[blockIdx.x] = red[0];
     }
     if (!active)
         return;
