@@ -117,15 +117,17 @@ def test_fd_rhs_matches_oracle(torch, R, dim, nq, law, vflux, coords, per, geome
     assert err <= RTOL_F64
 
 
-@pytest.mark.parametrize("dim,nq,vflux", [(2, 5, "ec"), (3, 8, "ec"), (3, 6, "central")])
-def test_fd_stage_matches_oracle_fp64_and_fp32(torch, R, dim, nq, vflux):
+@pytest.mark.parametrize("dim,nq,vflux,geometry", [(2, 5, "ec", "auto"), (3, 8, "ec", "auto"),
+                                                 (3, 6, "central", "auto"),
+                                                 (3, 5, "ec", "general")])
+def test_fd_stage_matches_oracle_fp64_and_fp32(torch, R, dim, nq, vflux, geometry):
     """One fused LSRK stage: dq = a dq + L(q), q_out = q + b dt dq."""
     coords = [PB.julia_range(0, 10, 4)] * dim
     # FP32: dq is a flux-difference sum with cancellation, rounded at the FP32
     # level as the weak form's (1e-4 there too); q_out = q + b dt dq is tight
     for dtype, tol_dq, tol_q in (("f64", RTOL_F64, RTOL_F64), ("f32", 1e-4, 1e-6)):
         grid, dg, og, odg = make_pair(R, dim, nq, coords, (True,) * dim, "euler", vflux,
-                                      dtype=dtype)
+                                      dtype=dtype, geometry=geometry)
         q = state(og, "euler", dim, seed=3)
         rng = np.random.default_rng(4)
         dq0 = 1e-2 * rng.standard_normal(q.size)

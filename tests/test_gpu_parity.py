@@ -342,6 +342,50 @@ def test_full_size_c2_properties(torch, R):
     assert np.all(np.abs(ints) <= 1e-10 * absints)
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_full_size_c2_flux_differencing_properties(torch, R, dtype):
+    """FluxDifferencingForm(EntropyConservativeFlux()) at the full 32^3 Nq = 8
+    size, with its own flux on the faces: free-stream preservation, discrete
+    conservation (sum MJ dq = 0 on a periodic grid) and semi-discrete entropy
+    conservation (sum MJ v(q) . dq = 0), the last two relative to the sums of
+    magnitudes; the FP32 bound is the FP32 rounding level."""
+    say(f"full-size flux differencing {dtype}")
+    cell = R.LobattoCell(8, 8, 8)
+    gm = R.GridManager(cell, R.brick([PB.julia_range(0, 10, 33)] * 3, (True,) * 3))
+    ec = R.EntropyConservativeFlux()
+    dg = R.DGSEM.brick(gm, R.EulerLaw(3), surface_numericalflux=ec, dtype=dtype,
+                       volume_form=R.FluxDifferencingForm(ec))
+    q = dg.GridArray(0.0)
+    for s, v in enumerate([1.1, 0.2, -0.1, 0.05, 2.5]):
+        q[:, s, :] = v
+    dq = dg.GridArray(0.0)
+    dg(dq, q, 0.0, increment=False)
+    dg.synchronize()
+    # FP32: the flux differences round at eps32 * (2/h) * sum|D| * |F| * dim
+    # = 1.2e-7 * 6.4 * 30 * 1 * 3 ~ 7e-5 for this state
+    tol = 1e-12 if dtype == "f64" else 2e-4
+    assert torch.max(torch.abs(dq)).item() < tol
+    import torch as T
+    T.manual_seed(1)
+    q[:, 0, :] += 1e-3 * (2 * T.rand_like(q[:, 0, :]) - 1)
+    dg(dq, q, 0.0, increment=False)
+    ints = dg.integrals(dq)
+    absints = dg.integrals(torch.abs(dq).contiguous())
+    prod = R.entropyproduct(dg, q, dq)
+    scale = R.entropyproduct(dg, q, torch.abs(dq).contiguous())
+    say(f"   sum MJ dq = {ints}; sum MJ v.dq = {prod:.3e} (scale {abs(scale):.3e})")
+    if dtype == "f64":
+        assert np.all(np.abs(ints) <= 1e-10 * absints)
+        assert abs(prod) <= 1e-10 * max(abs(scale), np.max(absints))
+    else:
+        # FP32 rounding of the flux differences is shared by all components;
+        # the density's own |dq| (from the 1e-3 noise alone) is far below it,
+        # so the bound is taken against the largest component's scale
+        assert np.all(np.abs(ints) <= 1e-4 * np.max(absints))
+        assert abs(prod) <= 1e-4 * max(abs(scale), np.max(absints))
+    assert bool(torch.isfinite(dq).all())
+
+
 def _gaussian_pulse_2d(P):
     """exp(-((x-0.5)^2 + (y-0.5)^2)/0.01) at every node."""
     x, y = P[:, 0, :], P[:, 1, :]
@@ -737,6 +781,7 @@ def test_c_driver_reproduces_solve_and_dostep(tmp_path):
     assert n == 20 and err <= 1e-12
 
 
+@pytest.mark.parametrize("increment", [False, True])
 @pytest.mark.parametrize("dim,nq,law,vflux,geometry,dirs", [
     (2, 5, "euler", None, "auto", (2,)),
     (2, 6, "advection", None, "general", (1,)),
@@ -746,7 +791,7 @@ def test_c_driver_reproduces_solve_and_dostep(tmp_path):
     (3, 6, "euler", "ec", "auto", (3,)),
     (3, 4, "euler", "central", "general", (1, 2)),
 ])
-def test_directions_match_oracle(torch, R, dim, nq, law, vflux, geometry, dirs):
+def test_directions_match_oracle(torch, R, dim, nq, law, vflux, geometry, dirs, increment):
     """DGSEM(...; directions) (dgsem.jl:32-36): the volume derivatives and the
     faces of the given directions only, weak and flux-differencing forms."""
     say(f"directions {dirs}: dim={dim} nq={nq} {law} {vflux or 'weak'} {geometry}")
@@ -764,10 +809,11 @@ def test_directions_match_oracle(torch, R, dim, nq, law, vflux, geometry, dirs):
     og = O.BrickGrid(nq, coords, (True,) * dim)
     odg = O.DG(og, law, u=u, volume_flux=vflux, directions=dirs)
     q = initial_state(og, law, dim, seed=21)
-    want = np.zeros(q.size)
-    odg.rhs(want, q.copy(), 0.0, increment=False)
-    qd, dqd = to_dev(torch, dg, q), dg.GridArray(0.0)
-    dg(dqd, qd, 0.0, increment=False)
+    dq0 = np.random.default_rng(22).standard_normal(q.size) if increment else np.zeros(q.size)
+    want = dq0.copy()
+    odg.rhs(want, q.copy(), 0.0, increment=increment)
+    qd, dqd = to_dev(torch, dg, q), to_dev(torch, dg, dq0)
+    dg(dqd, qd, 0.0, increment=increment)
     dg.synchronize()
     err = rel_l2(dqd.cpu().numpy().reshape(-1), want)
     say(f"   rel L2 {err:.2e}")
