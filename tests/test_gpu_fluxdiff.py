@@ -203,8 +203,6 @@ def _fd_golden_cases():
     out = []
     for e in entries:
         for K, want in zip(e["K"], e["expected"]):
-            if K > 20 and "1d" not in e["case"]:
-                continue
             out.append((e["case"], e["volume_flux"], K, want, 10.0 if "1d" in e["case"] else 1.0))
     return out
 

@@ -178,8 +178,6 @@ def _golden_cases():
     out = []
     for e in entries:
         for K, want in zip(e["K"], e["expected"]):
-            if K > 20 and e["case"] != "advection_1d":
-                continue
             out.append((e["case"], K, want, 10.0 if "1d" in e["case"] else 1.0))
     return out
 
