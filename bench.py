@@ -413,23 +413,8 @@ def run_adaptive(R, torch, steps=200, nq=6, seed=99):
     the all-FP64 run after `steps` LSRK54 steps, with hanging faces coupled
     through mortars, and Raven's self-mapped hanging faces for comparison."""
     import problems as PB
-    rng = np.random.default_rng(seed)
-    amp = rng.uniform(0.15, 0.3, 3)
-    ph = rng.uniform(0, 2 * np.pi, 3)
-    k = 2 * np.pi / 10
-
-    def warp(x, y, z):
-        return (x + amp[0] * np.sin(k * y + ph[0]) * np.sin(k * z),
-                y + amp[1] * np.sin(k * x + ph[1]) * np.sin(k * z),
-                z + amp[2] * np.sin(k * x) * np.sin(k * y + ph[2]))
-
-    cell = R.LobattoCell(nq, nq, nq)
-    gm = R.GridManager(cell, R.brick([np.linspace(0, 10, 17)] * 3, (True,) * 3))
-    for r in range(2):
-        c = gm.cell_centres()
-        flags = np.where((np.linalg.norm(c - 5.0, axis=1) < 2.0) & (gm.cells()[1] <= r),
-                         R.AdaptRefine, R.AdaptNone)
-        gm.adapt(flags)
+    warp, rng = PB.c3_warp(seed)
+    gm = PB.c3_gridmanager(R, nq=nq)
     t0 = time.perf_counter()
     grid = R.generate(gm, warp=warp)
     t_grid = time.perf_counter() - t0

@@ -121,7 +121,7 @@ typedef struct rv_info {
     int32_t geometry;           /* resolved RV_GEOM_AFFINE or RV_GEOM_GENERAL */
     int32_t faces_self;         /* faces with q+ = q- (boundary or self-mapped) */
     int32_t faces_explicit;     /* faces needing per-node neighbour indices */
-    int32_t reserved;
+    int32_t streaming;          /* 1: stages stream dq and q_out past L2 (3 states > 256 MB) */
     int64_t ncells;
     int64_t state_bytes;        /* bytes of one state array */
     int64_t device_bytes;       /* device memory held by the handle */
@@ -196,10 +196,17 @@ int rv_rhs(rv_handle *h, void *dq, const void *q, double t, int increment);
 int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double a,
                   double b_dt, double t);
 
-/* solve!(q, timeend, LSRK(dg, q, dt; t0)) (odesolvers.jl:9-61): dq is zeroed
- * first; the last step is shortened to land on timeend.  q is updated in
- * place (the library ping-pongs through an internal buffer).
- * t_out / nsteps_out may be NULL. */
+/* solve!(q, timeend, LSRK(dg, q, dt; t0); adjust_final) (odesolvers.jl:9-61):
+ * dq is zeroed first (odesolvers.jl:43-44); steps are taken until
+ * time + dt >= timeend, and with adjust_final the last one is shortened to
+ * timeend - time (odesolvers.jl:21-25), else it keeps dt and ends past
+ * timeend.  FP32 handles keep t0, dt and the clock in FP32 as LSRK{Float32}
+ * does, comparing with timeend as given.  q is updated in place (the library
+ * ping-pongs through a state-sized buffer it allocates on the first call).
+ * dt must be positive.  t_out / nsteps_out may be NULL. */
+int rv_lsrk_solve(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                  double timeend, int adjust_final, double *t_out, int64_t *nsteps_out);
+/* rv_lsrk_solve with adjust_final = 1, solve!'s default. */
 int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
                 double timeend, double *t_out, int64_t *nsteps_out);
 
@@ -234,14 +241,19 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
 
 /* The leaves of GridManager(cell, brick(...); min_level) after adapt!(gm,
  * flags) calls (gridmanager.jl:30-91, 160-172), without p4est: the brick's
- * trees in tree order, Morton order inside a tree, refinement by one level per
- * call (flag 0x10 = AdaptRefine; 0 keeps a leaf; coarsening is not supported)
- * followed by 2:1 balance across faces, edges and corners.  Host only. */
+ * trees in tree order, Morton order inside a tree.  rv_forest_adapt takes
+ * Raven's flags (gridmanager.jl:24-27), one per leaf: every complete family of
+ * siblings all flagged AdaptCoarsen (0x00) becomes its parent, then every
+ * other leaf flagged AdaptRefine (0x10) is split once; any other value
+ * (AdaptNone 0x01, AdaptTouched 0x11, ...) keeps the leaf.  The forest is then
+ * balanced 2:1 across faces, edges and corners, and `flags` is overwritten as
+ * adapt! overwrites it (fill_adapt_flags, :117-129): AdaptRefine, AdaptCoarsen
+ * or AdaptNone by what happened to each old leaf.  Host only. */
 typedef struct rv_forest rv_forest;
 int rv_forest_create(const rv_brick_desc *desc, rv_forest **out);
 void rv_forest_destroy(rv_forest *f);
 int64_t rv_forest_numcells(const rv_forest *f);
-int rv_forest_adapt(rv_forest *f, const uint8_t *flags);
+int rv_forest_adapt(rv_forest *f, uint8_t *flags);
 /* tree lattice coordinates (K x dim), levels (K) and coordinates inside the
  * tree at each leaf's level (K x dim); any pointer may be NULL */
 int rv_forest_cells(const rv_forest *f, int32_t *tree, int32_t *level, int32_t *quad);
@@ -263,6 +275,58 @@ int rv_forest_arrays(const rv_forest *f, int32_t nq, double *points, int64_t *vm
  * order; conforming faces hold -1.  These are the facts Raven keeps in
  * vmapNC / nctoface / nctypes (lobattocells.jl:2208-2275). */
 int rv_forest_mortars(const rv_forest *f, int64_t *partner);
+
+/* Raven's non-conforming face tables (materializefacemaps,
+ * lobattocells.jl:2121-2276) for the forest, one set per face group g < dim
+ * (faces normal to direction g), as Raven's tuples hold them:
+ *   vmapNC[g]   (Nfp, 1 + 2^(dim-1), nunique[g]) int64, 1-based volume
+ *               indices: the parent (coarse) face's nodes, then each finer
+ *               face's, in face node order; the finer faces in sub-face order
+ *               s1 + 2 s2 (halves along the first and second tangential
+ *               direction)
+ *   nctoface[g] (2, nunique[g]) int8: parent face, child face (1-based)
+ *   nctypes[g]  (nfaces[g]) int8, per non-conforming (cell, face) pair in
+ *               cell-then-face order: 1 for the parent, 2 + sub-face for a child
+ *   ncids[g]    (nfaces[g]) int64: the pair's interface (1-based column of
+ *               vmapNC), interfaces numbered in order of first encounter.
+ * rv_forest_nc_counts gives nunique and nfaces (dim entries each; either may
+ * be NULL); in rv_forest_nctables each argument is an array of dim pointers,
+ * and any of them, or any entry, may be NULL.  Brick trees are aligned, so
+ * every face orientation is the identity.  Host only. */
+int rv_forest_nc_counts(const rv_forest *f, int64_t *nunique, int64_t *nfaces);
+int rv_forest_nctables(const rv_forest *f, int32_t nq, int64_t *const *vmapNC,
+                       int8_t *const *nctoface, int8_t *const *nctypes, int64_t *const *ncids);
+/* The face kinds (2dim, K) and partners (4, 2dim, K) rv_enable_mortars takes,
+ * from Raven's own vmapNC and nctoface (the tables above, as facemaps(grid)
+ * holds them), so a Raven grid can couple its hanging faces: a parent face
+ * becomes kind 2 with its children as partners, each child face kind 1 with
+ * (parent, sub-face).  Tables whose node lists are not the aligned face
+ * pairing (rotated trees) are refused with RV_E_UNSUPPORTED.  Host only. */
+int rv_mortars_from_nctables(int32_t dim, int32_t nq, int64_t ncells, const int64_t *nunique,
+                             const int64_t *const *vmapNC, const int8_t *const *nctoface,
+                             int32_t *face_kind, int64_t *partner);
+/* discontinuoustocontinuous(grid) of the forest's grid (Np, K), 1-based, and
+ * numcnodes(grid): nodes on hanging faces and hanging edges take the ids of
+ * the coarse face's or edge's nodes at the same position (p4est lnodes,
+ * gridmanager.jl:349-379), the others are shared where they coincide, and ids
+ * are numbered by first appearance (lobattocells.jl:1718-1936).  Host only. */
+int rv_forest_dtoc(const rv_forest *f, int32_t nq, int64_t *dtoc, int64_t *ncontinuous);
+
+/* DGSEM(; law, grid = generate(warp, gm)) (dgsem.jl:46-81,
+ * gridmanager.jl:1084-1208) for an adapted forest, with the grid generated on
+ * the device: the node coordinates (the brick formula at each leaf's level,
+ * lobattocells.jl:1590-1674) unless `points` (a DEVICE array (Np, dim, K),
+ * e.g. the caller's warp of them) is given, the curved volume and surface
+ * metrics (as rv_curved_metrics) and the stage kernels' geometry, all on the
+ * descriptor's stream; the host builds one record per face from the leaves
+ * (hanging and covered faces self-mapped as Raven's vmapP).  No per-node array
+ * crosses PCIe.  `params` gives law, flux, dtype, Nq, gamma, velocity, stream
+ * and volume form (its dim must be the forest's; the brick fields are the
+ * forest's own).  The handle keeps a copy of the leaves for the continuous
+ * numbering, built on the first rv_gather / rv_scatter; rv_get_maps gives the
+ * face maps.  Nq <= 8. */
+int rv_create_forest(const rv_forest *f, const rv_brick_desc *params, const double *points,
+                     rv_handle **out);
 
 /* --- batches of independent solves -------------------------------------- */
 

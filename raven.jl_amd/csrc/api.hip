@@ -91,6 +91,7 @@ struct rv_handle {
     int faces_self = 0, faces_explicit = 0;
     bool is_brick = false;
     BrickSpec brick{};
+    Forest *forest = nullptr;  // forest handles: the leaves, for the continuous numbering
     // mortar coupling of hanging faces (rv_enable_mortars)
     MortarLauncher mortar = nullptr;
     int64_t mortar_cells = 0;
@@ -202,6 +203,8 @@ void free_handle(rv_handle *h)
             (void)hipFree(b);
     if (h->own_stream && h->stream)
         (void)hipStreamDestroy(h->stream);
+    if (h->forest)
+        forest_free(h->forest);
     delete h;
 }
 
@@ -235,6 +238,13 @@ __device__ inline Comp comp_merge(Comp a, Comp b)
 constexpr int RED_BLOCKS = 512;
 constexpr int RED_THREADS = 256;
 
+// max that returns NaN when either argument is NaN, as Julia's maximum does
+// (fmax would drop it and hide a diverged state)
+__host__ __device__ inline double max_nan(double a, double b)
+{
+    return a != a ? a : b != b ? b : (a > b ? a : b);
+}
+
 // partial[s * RED_BLOCKS + block] = compensated sum over the block's share of
 // MJ * f(q_s); f = square (power 2), identity (power 1) or |q|^p (power 3);
 // power 4: the block's max |q_s| (in the sum slot, compensation 0)
@@ -254,7 +264,7 @@ __global__ void __launch_bounds__(RED_THREADS)
         const int64_t e = i / Np, node = i - e * Np;
         const double v = (double)q[node + Np * (s + (int64_t)ns * e)];
         if (power == 4) {
-            acc.s = fmax(acc.s, fabs(v));
+            acc.s = max_nan(acc.s, fabs(v));
             continue;
         }
         const double x = power == 2   ? MJ[i] * (v * v)
@@ -267,7 +277,7 @@ __global__ void __launch_bounds__(RED_THREADS)
     for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             if (power == 4)
-                sh[threadIdx.x].s = fmax(sh[threadIdx.x].s, sh[threadIdx.x + w].s);
+                sh[threadIdx.x].s = max_nan(sh[threadIdx.x].s, sh[threadIdx.x + w].s);
             else
                 sh[threadIdx.x] = comp_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
         }
@@ -868,7 +878,7 @@ int reduce(rv_handle *h, const void *q, int power, double *out, double pexp = 2.
         if (power == 4) {
             double m = 0.0;
             for (int b = 0; b < RED_BLOCKS; ++b)
-                m = std::fmax(m, part[2 * (s * RED_BLOCKS + b)]);
+                m = max_nan(m, part[2 * (s * RED_BLOCKS + b)]);
             out[s] = m;
             continue;
         }
@@ -1150,6 +1160,7 @@ int rv_get_info(const rv_handle *h, rv_info *info)
     info->geometry = h->geo == GEO_AFFINE ? RV_GEOM_AFFINE : RV_GEOM_GENERAL;
     info->faces_self = h->faces_self;
     info->faces_explicit = h->faces_explicit;
+    info->streaming = stage_params(const_cast<rv_handle *>(h)).streaming;
     info->ncells = h->K;
     const int64_t b = h->dtype == RV_F64 ? 8 : 4;
     const int64_t nodes = h->Np * h->K;
@@ -1397,11 +1408,14 @@ int rv_lsrk_coefficients(int scheme, double *a, double *b, double *c, int *nstag
     return RV_OK;
 }
 
-int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
-                double timeend, double *t_out, int64_t *nsteps_out)
+namespace {
+
+// solve!(q, timeend, LSRK(dg, q, dt; t0); adjust_final) (odesolvers.jl:9-61)
+// with the ping-pong state `alt` (distinct from q and dq, one state's bytes)
+int lsrk_run_impl(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                  double timeend, int adjust_final, void *alt, double *t_out,
+                  int64_t *nsteps_out)
 {
-    if (!h || !q || !dq)
-        return fail(h, RV_E_ARG, "null argument");
     const double *A, *B, *C;
     const int nst = lsrk_tables(scheme, &A, &B, &C);
     if (nst == 0)
@@ -1409,28 +1423,23 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
     (void)C;  // the laws here do not depend on the stage time
     const bool f32 = h->dtype == RV_F32;
     const size_t bytes = (size_t)h->Np * h->ns * h->K * (f32 ? 4 : 8);
-    if (!h->qtmp) {
-        hipError_t e = hipMalloc(&h->qtmp, bytes ? bytes : 1);
-        if (e != hipSuccess) {
-            h->qtmp = nullptr;
-            return fail(h, RV_E_NOMEM, "hipMalloc of the ping-pong state failed");
-        }
-        h->device_bytes += (int64_t)bytes;
-    }
     HIP_TRY(h, hipMemsetAsync(dq, 0, bytes, h->stream));
     // LSRK(rhs!, ...) converts dt, t0 and the coefficients to the state type
-    // (odesolvers.jl:41-48); in FP32 the host arithmetic is FP32 too
+    // (odesolvers.jl:41-48); in FP32 the clock is FP32 too.  solve! compares
+    // time + dt with timeend as given and assigns timeend - time to the FP32
+    // dt field (rounding the difference), as Julia does for a Float64 timeend.
     auto rnd = [f32](double v) { return f32 ? (double)(float)v : v; };
-    double time = rnd(t0), step_dt = rnd(dt), tend = rnd(timeend);
-    void *cur = q, *alt = h->qtmp;
+    double time = rnd(t0), step_dt = rnd(dt);
+    void *cur = q;
     int64_t steps = 0;
     StageParams P = stage_params(h);
     for (;;) {
         ++steps;
         bool final_step = false;
         const double tdt = f32 ? (double)((float)time + (float)step_dt) : time + step_dt;
-        if (tdt >= tend) {
-            step_dt = f32 ? (double)((float)tend - (float)time) : tend - time;
+        if (tdt >= timeend) {
+            if (adjust_final)
+                step_dt = rnd(timeend - time);
             final_step = true;
         }
         for (int s = 0; s < nst; ++s) {
@@ -1459,6 +1468,34 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
     if (nsteps_out)
         *nsteps_out = steps;
     return RV_OK;
+}
+
+}  // namespace
+
+int rv_lsrk_solve(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                  double timeend, int adjust_final, double *t_out, int64_t *nsteps_out)
+{
+    if (!h || !q || !dq)
+        return fail(h, RV_E_ARG, "null argument");
+    if (!(dt > 0))
+        return fail(h, RV_E_ARG, "dt must be positive");
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    if (!h->qtmp) {
+        hipError_t e = hipMalloc(&h->qtmp, bytes ? bytes : 1);
+        if (e != hipSuccess) {
+            h->qtmp = nullptr;
+            return fail(h, RV_E_NOMEM, "hipMalloc of the ping-pong state failed");
+        }
+        h->device_bytes += (int64_t)bytes;
+    }
+    return lsrk_run_impl(h, q, dq, scheme, t0, dt, timeend, adjust_final, h->qtmp, t_out,
+                         nsteps_out);
+}
+
+int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                double timeend, double *t_out, int64_t *nsteps_out)
+{
+    return rv_lsrk_solve(h, q, dq, scheme, t0, dt, timeend, 1, t_out, nsteps_out);
 }
 
 int rv_entropy_integral(rv_handle *h, const void *q, double *out)
@@ -1699,10 +1736,11 @@ int ensure_gatherscatter(rv_handle *h)
 {
     if (h->dtoc)
         return RV_OK;
-    if (!h->is_brick)
+    if (!h->is_brick && !h->forest)
         return fail(h, RV_E_STATE, "no dtoc was given to rv_create");
     std::vector<int64_t> dtoc(h->K * h->Np);
-    const int64_t nc = brick_dtoc(h->brick, dtoc.data());
+    const int64_t nc =
+        h->forest ? forest_dtoc(h->forest, h->nq, dtoc.data()) : brick_dtoc(h->brick, dtoc.data());
     return build_gatherscatter(h, dtoc.data(), nc);
 }
 }  // namespace
@@ -1940,6 +1978,75 @@ int rv_brick_cells(const rv_brick_desc *bd, int32_t *ijk)
 
 }  // extern "C"
 
+namespace {
+
+// Node coordinates of forest leaves (Np, dim, K): the brick formula of
+// hexbrickpoints! / quadbrickpoints! (lobattocells.jl:1590-1674, 801-872) at
+// each leaf's level, x = fma(dx, fma(h, r + 1, c / 2^30), x1) with
+// h = 2^-(level+1), as the host generator computes it.
+struct NodesArg {
+    double r[32];
+};
+
+__global__ void forest_points_kernel(int dim, int nq, int64_t K, const int32_t *tree,
+                                     const int32_t *level, const int32_t *quad,
+                                     const double *verts, const int32_t *vert_off, NodesArg na,
+                                     double *points)
+{
+    int64_t Np = nq;
+    for (int d = 1; d < dim; ++d)
+        Np *= nq;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np * K)
+        return;
+    const int64_t e = i / Np, node = i - e * Np;
+    const int lv = level[e];
+    const double h = 1.0 / (double)(1 << (lv + 1));
+    int64_t rem = node;
+    for (int d = 0; d < dim; ++d) {
+        const int t = tree[e * dim + d];
+        const double *v = verts + vert_off[d];
+        const double c1 = v[t], dx = v[t + 1] - v[t];
+        const double cr = (double)((int64_t)quad[e * dim + d] << (30 - lv)) / (double)(1 << 30);
+        points[node + Np * (d + (int64_t)dim * e)] = fma(dx, fma(h, na.r[rem % nq] + 1, cr), c1);
+        rem /= nq;
+    }
+}
+
+// Raven's metric arrays (Np, L, K) and surface metrics (dim+2, 2dim Nfp, K)
+// into the stage kernels' general geometry: wJ dRdX (K, dim, dim, Np),
+// 1 / wJ and wJ (K, Np), normals and wsJ (K, 2dim, dim+1, Nfp).
+__global__ void general_geometry_kernel(int dim, int nq, int64_t K, const double *M,
+                                        const double *S, double *vgeo, double *mji, double *mj,
+                                        double *sgeo)
+{
+    int64_t Np = nq;
+    for (int d = 1; d < dim; ++d)
+        Np *= nq;
+    const int64_t nfp = Np / nq, nf = 2 * dim, nfpt = nf * nfp;
+    const int L = dim * dim + 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Np * K) {
+        const int64_t e = i / Np, n = i - e * Np;
+        const double *me = M + Np * (int64_t)L * e;
+        const double wJ = me[n + Np * (dim * dim + 1)];
+        mji[i] = 1.0 / wJ;
+        mj[i] = wJ;
+        for (int a = 0; a < dim; ++a)
+            for (int c = 0; c < dim; ++c)
+                vgeo[((e * dim + a) * dim + c) * Np + n] = wJ * me[n + Np * (a + dim * c)];
+    }
+    if (i < nfpt * K) {
+        const int64_t e = i / nfpt, r = i - e * nfpt, f = r / nfp, m = r - f * nfp;
+        const double *sp = S + (dim + 2) * (f * nfp + m + nfpt * e);
+        for (int c = 0; c < dim; ++c)
+            sgeo[((e * nf + f) * (dim + 1) + c) * nfp + m] = sp[c];
+        sgeo[((e * nf + f) * (dim + 1) + dim) * nfp + m] = sp[dim + 1];
+    }
+}
+
+}  // namespace
+
 struct rv_forest {
     Forest *f = nullptr;
     int dim = 0;
@@ -1977,13 +2084,12 @@ int64_t rv_forest_numcells(const rv_forest *f)
     return f ? forest_size(f->f) : fail(nullptr, RV_E_ARG, "null forest");
 }
 
-int rv_forest_adapt(rv_forest *f, const uint8_t *flags)
+int rv_forest_adapt(rv_forest *f, uint8_t *flags)
 {
     if (!f || !flags)
         return fail(nullptr, RV_E_ARG, "null argument");
     if (forest_adapt(f->f, flags) != 0)
-        return fail(nullptr, RV_E_UNSUPPORTED,
-                    "adapt flags must be 0 (keep) or 0x10 (refine) below the finest level");
+        return fail(nullptr, RV_E_UNSUPPORTED, "a leaf flagged AdaptRefine is at the finest level");
     return RV_OK;
 }
 
@@ -2000,6 +2106,211 @@ int rv_forest_mortars(const rv_forest *f, int64_t *partner)
     if (!f || !partner)
         return fail(nullptr, RV_E_ARG, "null argument");
     forest_mortars(f->f, partner);
+    return RV_OK;
+}
+
+int rv_forest_nc_counts(const rv_forest *f, int64_t *nunique, int64_t *nfaces)
+{
+    if (!f)
+        return fail(nullptr, RV_E_ARG, "null forest");
+    forest_nc_counts(f->f, nunique, nfaces);
+    return RV_OK;
+}
+
+int rv_forest_nctables(const rv_forest *f, int32_t nq, int64_t *const *vmapNC,
+                       int8_t *const *nctoface, int8_t *const *nctypes, int64_t *const *ncids)
+{
+    if (!f || nq < 2 || nq > 32)
+        return fail(nullptr, RV_E_ARG, "invalid argument");
+    forest_nctables(f->f, nq, vmapNC, nctoface, nctypes, ncids);
+    return RV_OK;
+}
+
+int rv_mortars_from_nctables(int32_t dim, int32_t nq, int64_t ncells, const int64_t *nunique,
+                             const int64_t *const *vmapNC, const int8_t *const *nctoface,
+                             int32_t *face_kind, int64_t *partner)
+{
+    if (dim < 2 || dim > 3 || nq < 2 || nq > 32 || ncells < 0 || !nunique || !vmapNC ||
+        !nctoface || !face_kind || !partner)
+        return fail(nullptr, RV_E_ARG, "rv_mortars_from_nctables: invalid argument");
+    const int rc = mortars_from_nctables(dim, nq, ncells, nunique, vmapNC, nctoface, face_kind,
+                                         partner);
+    if (rc == -1)
+        return fail(nullptr, RV_E_ARG, "rv_mortars_from_nctables: a node index is out of range");
+    if (rc == -2)
+        return fail(nullptr, RV_E_UNSUPPORTED,
+                    "rv_mortars_from_nctables: a table is not an aligned face pairing "
+                    "(rotated faces are not supported)");
+    return RV_OK;
+}
+
+int rv_create_forest(const rv_forest *f, const rv_brick_desc *bd, const double *points,
+                     rv_handle **out)
+{
+    if (!out)
+        return fail(nullptr, RV_E_ARG, "out is null");
+    *out = nullptr;
+    if (!f || !bd)
+        return fail(nullptr, RV_E_ARG, "null argument");
+    const BrickSpec b = forest_spec(f->f);
+    const int dim = b.dim, nq = bd->nq;
+    if (bd->dim != dim || nq < 2 || nq > 8)
+        return fail(nullptr, RV_E_ARG, "rv_create_forest: dim must be the forest's, 2 <= Nq <= 8");
+    if (bd->law != RV_LAW_ADVECTION && bd->law != RV_LAW_EULER)
+        return fail(nullptr, RV_E_ARG, "unknown law");
+    if (bd->dtype != RV_F64 && bd->dtype != RV_F32)
+        return fail(nullptr, RV_E_ARG, "unknown dtype");
+    rv_handle *h = new rv_handle();
+    h->dim = dim;
+    h->nq = nq;
+    h->law = bd->law;
+    h->flux = bd->flux ? bd->flux : RV_FLUX_RUSANOV;
+    if (bd->abi_version >= 2 &&
+        volume_flux_of(bd->volume_form, bd->volume_flux, bd->law, &h->vflux) != RV_OK) {
+        free_handle(h);
+        return fail(nullptr, RV_E_ARG, "invalid volume form or volume flux");
+    }
+    h->dtype = bd->dtype;
+    h->K = forest_size(f->f);
+    h->ns = bd->law == RV_LAW_ADVECTION ? 1 : dim + 2;
+    h->Np = ipow(nq, dim);
+    h->nfp = ipow(nq, dim - 1);
+    h->gamma = bd->gamma;
+    h->dir_mask = (1 << dim) - 1;
+    for (int c = 0; c < 3; ++c)
+        h->u[c] = bd->velocity[c];
+    h->geo = GEO_GENERAL;
+    const int64_t K = h->K, Np = h->Np, nfp = h->nfp, nf = 2 * dim, nfpt = nf * nfp;
+    const int L = dim * dim + 2;
+    int rc = resolve_stream(h, bd->stream);
+    // host: the leaves and one record per face (conforming neighbour, or the
+    // face itself: boundary faces, and hanging and covered faces as Raven's
+    // self-mapped vmapP, lobattocells.jl:2072-2074)
+    std::vector<int32_t> tree(K * dim), level(K), quad(K * dim);
+    std::vector<int64_t> nbr(K * nf), bc(K * nf);
+    std::vector<FaceRec> recs(K * nf);
+    std::vector<double> x(nq), w(nq), D(nq * nq), verts;
+    std::vector<int32_t> voff(3, 0);
+    if (!rc) {
+        forest_cells(f->f, tree.data(), level.data(), quad.data());
+        forest_faces(f->f, nbr.data(), nullptr, bc.data());
+        for (int64_t e = 0; e < K; ++e)
+            for (int fc = 0; fc < nf; ++fc) {
+                const int64_t en = nbr[e * nf + fc];
+                recs[e * nf + fc] = en < 0 ? FaceRec{FACE_SELF, (int32_t)bc[e * nf + fc]}
+                                           : FaceRec{(int32_t)en, fc ^ 1};
+                if (en < 0)
+                    h->faces_self++;
+            }
+        lobatto_operators(nq, x.data(), w.data(), D.data());
+        for (int d = 0; d < dim; ++d) {
+            voff[d] = (int32_t)verts.size();
+            std::vector<double> v(b.n[d] + 1);
+            brick_vertices(b, d, v.data());
+            verts.insert(verts.end(), v.begin(), v.end());
+        }
+    }
+    // device: points (unless given), curved metrics, the stage kernels'
+    // geometry; temporaries are stream-ordered and freed on the stream
+    hipStream_t st = h->stream;
+    double *dpts = nullptr, *dM = nullptr, *dS = nullptr, *dDw = nullptr, *dverts = nullptr;
+    int32_t *dcells = nullptr;
+    hipError_t e = hipSuccess;
+    auto amalloc = [&](void **p, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipMallocAsync(p, std::max<size_t>(bytes, 1), st);
+    };
+    if (!rc) {
+        amalloc((void **)&dM, sizeof(double) * Np * L * K);
+        amalloc((void **)&dS, sizeof(double) * (dim + 2) * nfpt * K);
+        amalloc((void **)&dDw, sizeof(double) * (nq * nq + nq));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dDw, D.data(), sizeof(double) * nq * nq, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dDw + nq * nq, w.data(), sizeof(double) * nq,
+                               hipMemcpyHostToDevice, st);
+        if (!points) {
+            amalloc((void **)&dpts, sizeof(double) * Np * dim * K);
+            amalloc((void **)&dverts, sizeof(double) * verts.size());
+            amalloc((void **)&dcells, sizeof(int32_t) * (K * (2 * dim + 1) + 3));
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dverts, verts.data(), sizeof(double) * verts.size(),
+                                   hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dcells, tree.data(), sizeof(int32_t) * K * dim,
+                                   hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dcells + K * dim, quad.data(), sizeof(int32_t) * K * dim,
+                                   hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dcells + 2 * K * dim, level.data(), sizeof(int32_t) * K,
+                                   hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dcells + K * (2 * dim + 1), voff.data(), sizeof(int32_t) * 3,
+                                   hipMemcpyHostToDevice, st);
+            if (e == hipSuccess && K > 0) {
+                NodesArg na{};
+                for (int i = 0; i < nq; ++i)
+                    na.r[i] = x[i];
+                const int64_t n = Np * K;
+                hipLaunchKernelGGL(forest_points_kernel, dim3((unsigned)((n + 255) / 256)),
+                                   dim3(256), 0, st, dim, nq, K, dcells, dcells + 2 * K * dim,
+                                   dcells + K * dim, dverts, dcells + K * (2 * dim + 1), na, dpts);
+                e = hipGetLastError();
+            }
+        }
+        const double *P = points ? points : dpts;
+        if (e == hipSuccess && K > 0) {
+            const int threads = (int)((Np + 63) / 64 * 64);
+            hipLaunchKernelGGL(curved_volume_metrics_kernel, dim3((unsigned)K), dim3(threads),
+                               sizeof(double) * 12 * Np, st, dim, nq, P, dDw, dDw + nq * nq, dM);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && K > 0) {
+            const int64_t n = nfpt * K;
+            hipLaunchKernelGGL(curved_surface_metrics_kernel, dim3((unsigned)((n + 255) / 256)),
+                               dim3(256), 0, st, dim, nq, K, dM, dDw + nq * nq, dS);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess)
+            rc = fail(h, RV_E_HIP, std::string("rv_create_forest: ") + hipGetErrorString(e));
+        if (!rc && !(rc = dalloc(h, &h->vgeo, (size_t)K * dim * dim * Np)) &&
+            !(rc = dalloc(h, &h->mji, K * Np)) && !(rc = dalloc(h, &h->MJ, K * Np)) &&
+            !(rc = dalloc(h, &h->sgeo, (size_t)K * nf * (dim + 1) * nfp)) &&
+            !(rc = upload(h, &h->faces, recs.data(), recs.size())) && K > 0) {
+            const int64_t n = std::max(Np, nfpt) * K;
+            hipLaunchKernelGGL(general_geometry_kernel, dim3((unsigned)((n + 255) / 256)),
+                               dim3(256), 0, st, dim, nq, K, dM, dS, h->vgeo, h->mji, h->MJ,
+                               h->sgeo);
+            e = hipGetLastError();
+            if (e != hipSuccess)
+                rc = fail(h, RV_E_HIP, std::string("rv_create_forest: ") + hipGetErrorString(e));
+        }
+    }
+    for (void *p : {(void *)dpts, (void *)dM, (void *)dS, (void *)dDw, (void *)dverts,
+                    (void *)dcells})
+        if (p)
+            (void)hipFreeAsync(p, st);
+    if (!rc) {
+        h->forest = forest_clone(f->f);
+        rc = finish_setup(h, D.data(), w.data());
+    }
+    if (rc) {
+        g_last_error = h->err;
+        free_handle(h);
+        return rc;
+    }
+    *out = h;
+    return RV_OK;
+}
+
+int rv_forest_dtoc(const rv_forest *f, int32_t nq, int64_t *dtoc, int64_t *ncontinuous)
+{
+    if (!f || nq < 2 || nq > 32 || !dtoc)
+        return fail(nullptr, RV_E_ARG, "invalid argument");
+    const int64_t n = forest_dtoc(f->f, nq, dtoc);
+    if (ncontinuous)
+        *ncontinuous = n;
     return RV_OK;
 }
 

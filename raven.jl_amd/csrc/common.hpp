@@ -181,9 +181,17 @@ void brick_neighbours(const BrickSpec &b, int64_t *nbr);
 struct Forest;
 Forest *forest_new(const BrickSpec &b);
 void forest_free(Forest *f);
+Forest *forest_clone(const Forest *f);
+// per (cell, face), e * 2dim + f: the conforming neighbour (-1 when none),
+// the face kind (0 conforming or boundary, 1 hanging, 2 covered) and the
+// boundary code; any pointer may be null
+void forest_faces(const Forest *f, int64_t *nbr, int32_t *kind, int64_t *bc);
+BrickSpec forest_spec(const Forest *f);
 int64_t forest_size(const Forest *f);
-// flags per leaf: 0 keep, 0x10 refine; -1 on any other flag
-int forest_adapt(Forest *f, const uint8_t *flags);
+// adapt! with Raven's flags (0x00 coarsen, 0x10 refine, anything else keeps),
+// flags overwritten with the outcome; -1 when a refined leaf would pass the
+// finest level
+int forest_adapt(Forest *f, uint8_t *flags);
 // tree lattice coordinates (K x dim), levels (K), coordinates inside the tree
 // at each leaf's level (K x dim); any pointer may be null
 void forest_cells(const Forest *f, int32_t *tree, int32_t *level, int32_t *quad);
@@ -196,5 +204,23 @@ void forest_arrays(const Forest *f, int nq, double *points, int64_t *vmapM, int6
 // (coarse cell, sub-face index s1 + 2*s2, -1, -1); a covered face holds the
 // 2^(dim-1) finer cells in sub-face order; conforming faces hold -1
 void forest_mortars(const Forest *f, int64_t *partner);
+// Raven's non-conforming face tables per face group g < dim: the number of
+// distinct non-conforming interfaces and of non-conforming (cell, face) pairs
+void forest_nc_counts(const Forest *f, int64_t *nunique, int64_t *nfaces);
+// vmapNC[g] (Nfp, 1 + 2^(dim-1), nunique[g]) 1-based, nctoface[g] (2,
+// nunique[g]), nctypes[g] and ncids[g] (nfaces[g], ncids 1-based); any
+// group's pointers may be null
+void forest_nctables(const Forest *f, int nq, int64_t *const *vmapNC, int8_t *const *nctoface,
+                     int8_t *const *nctypes, int64_t *const *ncids);
+// face kinds and partners (as forest_arrays / forest_mortars give them) from
+// Raven's vmapNC / nctoface; 0, -1 (an index out of range) or -2 (a table that
+// is not an aligned face pairing)
+int mortars_from_nctables(int dim, int nq, int64_t K, const int64_t *nunique,
+                          const int64_t *const *vmapNC, const int8_t *const *nctoface,
+                          int32_t *kind, int64_t *partner);
+// continuous numbering of the forest's nodes, hanging nodes included, as
+// Raven's dtoc (1-based, first appearance); returns the number of continuous
+// nodes
+int64_t forest_dtoc(const Forest *f, int nq, int64_t *dtoc);
 
 }  // namespace rv

@@ -8,7 +8,7 @@ from .batch import Batch  # noqa: F401
 from .dgsem import (DGSEM, AdvectionLaw, CentralFlux, EntropyConservativeFlux,  # noqa: F401
                     EulerLaw, FluxDifferencingForm, RusanovFlux, WeakForm, entropyintegral,
                     entropyproduct, weightednorm)
-from .grids import (AdaptNone, AdaptRefine, GridManager, LobattoCell, brick, generate,  # noqa: F401
+from .grids import (AdaptCoarsen, AdaptNone, AdaptRefine, AdaptTouched, GridManager, LobattoCell, brick, generate,  # noqa: F401
                     min_node_distance)
 from .odesolvers import LSRK54, LSRK144, RLSRK54, RLSRK144, dostep_, solve_  # noqa: F401
 

@@ -18,6 +18,7 @@ RV_FORM_WEAK, RV_FORM_FLUX_DIFFERENCING = 0, 1
 RV_F64, RV_F32 = 1, 2
 RV_GEOM_AUTO, RV_GEOM_AFFINE, RV_GEOM_GENERAL = 0, 1, 2
 RV_LSRK54, RV_LSRK144 = 54, 144
+RV_E_ARG, RV_E_UNSUPPORTED = -1, -4
 
 ERRORS = {-1: "RV_E_ARG", -2: "RV_E_HIP", -3: "RV_E_NOMEM", -4: "RV_E_UNSUPPORTED",
           -5: "RV_E_MAPS", -6: "RV_E_STATE"}
@@ -25,6 +26,8 @@ ERRORS = {-1: "RV_E_ARG", -2: "RV_E_HIP", -3: "RV_E_NOMEM", -4: "RV_E_UNSUPPORTE
 c_i32p = ctypes.POINTER(ctypes.c_int32)
 c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_f64p = ctypes.POINTER(ctypes.c_double)
+c_i64pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_int64))
+c_i8pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_int8))
 
 
 class RvDesc(ctypes.Structure):
@@ -57,7 +60,7 @@ class RvInfo(ctypes.Structure):
     _fields_ = [
         ("dim", ctypes.c_int32), ("nq", ctypes.c_int32), ("ns", ctypes.c_int32),
         ("dtype", ctypes.c_int32), ("geometry", ctypes.c_int32), ("faces_self", ctypes.c_int32),
-        ("faces_explicit", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("faces_explicit", ctypes.c_int32), ("streaming", ctypes.c_int32),
         ("ncells", ctypes.c_int64), ("state_bytes", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64), ("stage_bytes", ctypes.c_int64),
         ("stage_bytes_first", ctypes.c_int64), ("rhs_bytes", ctypes.c_int64),
@@ -120,6 +123,9 @@ def lib():
                                          ctypes.c_double]),
         "rv_lsrk_run": (ctypes.c_int, [H, vp, vp, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, c_f64p, c_i64p]),
+        "rv_lsrk_solve": (ctypes.c_int, [H, vp, vp, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         c_f64p, c_i64p]),
         "rv_weightednorm": (ctypes.c_int, [H, vp, c_f64p]),
         "rv_integrals": (ctypes.c_int, [H, vp, c_f64p]),
         "rv_weightednorm_p": (ctypes.c_int, [H, vp, ctypes.c_double, c_f64p]),
@@ -144,6 +150,15 @@ def lib():
         "rv_forest_arrays": (ctypes.c_int, [H, ctypes.c_int32, c_f64p, c_i64p, c_i64p, c_i64p,
                                             c_i32p]),
         "rv_forest_mortars": (ctypes.c_int, [H, c_i64p]),
+        "rv_forest_nc_counts": (ctypes.c_int, [H, c_i64p, c_i64p]),
+        "rv_forest_nctables": (ctypes.c_int, [H, ctypes.c_int32, c_i64pp, c_i8pp, c_i8pp,
+                                              c_i64pp]),
+        "rv_mortars_from_nctables": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32,
+                                                    ctypes.c_int64, c_i64p, c_i64pp, c_i8pp,
+                                                    c_i32p, c_i64p]),
+        "rv_forest_dtoc": (ctypes.c_int, [H, ctypes.c_int32, c_i64p, c_i64p]),
+        "rv_create_forest": (ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.POINTER(H)]),
         "rv_lobatto_halves": (ctypes.c_int, [ctypes.c_int, c_f64p, c_f64p]),
         "rv_enable_mortars": (ctypes.c_int, [H, c_i32p, c_i64p]),
         "rv_min_node_distance": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp,

@@ -165,6 +165,34 @@ class DGSEM:
         return cls(law=law, grid=None, surface_numericalflux=flux, dtype=dtype, volume_form=vf,
                    _handle=h)
 
+    @classmethod
+    def forest(cls, gm, law, surface_numericalflux=None, dtype="f64", stream=None,
+               volume_form=None, points=None):
+        """DGSEM on an adapted GridManager's grid generated on the device
+        (rv_create_forest): points (a device tensor (K, dim, Np), e.g. warped)
+        or the unwarped brick points, curved metrics and geometry computed on
+        the GPU, no per-node host arrays."""
+        flux = surface_numericalflux or RusanovFlux()
+        vf = volume_form or WeakForm()
+        if not gm.adapted:
+            gm._ensure_forest()
+        desc = gm.brick_desc(law=law.law_id, dtype=_DTYPES[dtype], flux=flux.flux_id,
+                             gamma=law.gamma, velocity=law.u,
+                             stream=stream if stream is not None else _current_stream(),
+                             volume_form=vf.form_id, volume_flux=vf.flux_id)
+        pts = None
+        if points is not None:
+            if points.dtype != _torch().float64 or not points.is_contiguous() or \
+                    points.device.type != "cuda":
+                raise ValueError("points must be a contiguous float64 device tensor")
+            pts = ctypes.c_void_p(points.data_ptr())
+        h = L.H()
+        L.check(L.lib().rv_create_forest(gm._forest, ctypes.byref(desc), pts, ctypes.byref(h)))
+        dg = cls(law=law, grid=None, surface_numericalflux=flux, dtype=dtype, volume_form=vf,
+                 _handle=h)
+        dg._points_keep = points
+        return dg
+
     # -- lifetime ----------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
@@ -216,12 +244,18 @@ class DGSEM:
                                       self._check_state(dq), float(a), float(b_dt), float(time)),
                 self._h)
 
-    def enable_mortars(self, grid=None):
+    def enable_mortars(self, grid=None, from_nctables=False):
         """Couple the hanging faces of an adapted grid through mortars
-        (rv_enable_mortars) instead of Raven's self-mapped faces."""
+        (rv_enable_mortars) instead of Raven's self-mapped faces.  The face
+        kinds and partners come from the forest, or with from_nctables from
+        Raven's own facemaps tables vmapNC / nctoface (rv_mortars_from_nctables)."""
+        from .grids import mortars_from_nctables
         g = grid if grid is not None else self.grid
-        kind = np.ascontiguousarray(g.face_kind, dtype=np.int32)
-        partner = np.ascontiguousarray(g.mortar_partner, dtype=np.int64)
+        if from_nctables:
+            kind, partner = mortars_from_nctables(g.dim, g.nq, g.K, g.vmapNC, g.nctoface)
+        else:
+            kind = np.ascontiguousarray(g.face_kind, dtype=np.int32)
+            partner = np.ascontiguousarray(g.mortar_partner, dtype=np.int64)
         L.check(L.lib().rv_enable_mortars(self._h, L.ptr(kind, ctypes.c_int32),
                                           L.ptr(partner, ctypes.c_int64)), self._h)
 

@@ -62,7 +62,8 @@ def brick(coordinates, periodic=None):
     return BrickGrid(coordinates, periodic)
 
 
-AdaptNone, AdaptRefine = 0x00, 0x10  # gridmanager.jl:27
+# adapt! flags (gridmanager.jl:24-27)
+AdaptCoarsen, AdaptNone, AdaptRefine, AdaptTouched = 0x00, 0x01, 0x10, 0x11
 
 
 class GridManager:
@@ -101,13 +102,20 @@ class GridManager:
         return self._forest is not None
 
     def adapt(self, flags):
-        """adapt!(gm, flags) (gridmanager.jl:160-172): refine the leaves flagged
-        AdaptRefine by one level, then balance the forest 2:1."""
+        """adapt!(gm, flags) (gridmanager.jl:160-172): coarsen every complete
+        family of leaves flagged AdaptCoarsen, refine the other leaves flagged
+        AdaptRefine by one level, then balance the forest 2:1.  As adapt!, the
+        flags are overwritten with what happened to each old leaf (AdaptRefine,
+        AdaptCoarsen or AdaptNone): in place when `flags` is a contiguous uint8
+        array, and returned in any case."""
         f = self._ensure_forest()
-        fl = np.ascontiguousarray(flags, dtype=np.uint8)
+        inplace = (isinstance(flags, np.ndarray) and flags.dtype == np.uint8
+                   and flags.flags.c_contiguous and flags.flags.writeable)
+        fl = flags if inplace else np.array(flags, dtype=np.uint8).reshape(-1)
         if fl.size != len(self):
             raise ValueError("one flag per cell")
         L.check(L.lib().rv_forest_adapt(f, fl.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_uint8))))
+        return fl
 
     def cells(self):
         """(tree, level, quad): tree lattice coordinates (K, dim), levels (K)
@@ -201,16 +209,21 @@ def _check_uniform(cg, desc):
             raise NotImplementedError("only uniformly spaced brick coordinates are supported")
 
 
-def generate(gm, warp=None):
+def generate(gm, warp=None, metrics=True):
     """generate(gm) / generate(warp, gm) (gridmanager.jl:1084-1208).
 
     warp(x, y[, z]) maps arrays of node coordinates to a tuple of arrays (the
     vectorised form of Raven's point-wise warp); warped grids take the curved
     metric path (gridmanager.jl:1160-1166), which runs on the device
     (rv_curved_metrics), as do the refined cells of an adapted forest
-    (gm.adapt); those grids carry no continuous numbering (dtoc is None)."""
+    (gm.adapt).  An adapted forest's grid also carries Raven's non-conforming
+    tables (grid.vmapNC, .nctoface, .nctypes, .ncids, one entry per face
+    group) and its continuous numbering with hanging nodes (rv_forest_dtoc).
+    metrics=False skips the device-side metrics of curved or adapted grids
+    (grid.metrics and grid.surface are then None): the integer arrays and
+    points alone, without a device."""
     if gm.adapted:
-        return _generate_forest(gm, warp)
+        return _generate_forest(gm, warp, metrics)
     desc = gm.brick_desc()
     _check_uniform(gm.coarsegrid, desc)
     lib = L.lib()
@@ -238,11 +251,11 @@ def generate(gm, warp=None):
         points = np.ascontiguousarray(
             np.stack([np.broadcast_to(np.asarray(x, dtype=np.float64), (K, Np)) for x in X],
                      axis=1))
-        metrics, surface = curved_metrics(gm.cell, points)
+        metrics, surface = curved_metrics(gm.cell, points) if metrics else (None, None)
     return Grid(gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc.value)
 
 
-def _generate_forest(gm, warp):
+def _generate_forest(gm, warp, with_metrics=True):
     f = gm._forest
     dim, nq = gm.cell.dim, gm.cell.nq
     K = len(gm)
@@ -262,14 +275,56 @@ def _generate_forest(gm, warp):
         points = np.ascontiguousarray(
             np.stack([np.broadcast_to(np.asarray(x, dtype=np.float64), (K, Np)) for x in X],
                      axis=1))
-    metrics, surface = curved_metrics(gm.cell, points)
+    metrics, surface = curved_metrics(gm.cell, points) if with_metrics else (None, None)
     partner = np.zeros((K, 2 * dim, 4), dtype=np.int64)
     L.check(L.lib().rv_forest_mortars(f, L.ptr(partner, c.c_int64)))
-    grid = Grid(gm, points, metrics, surface, vmapM, vmapP, bc, None, 0)
+    dtoc = np.zeros((K, Np), dtype=np.int64)
+    nc = c.c_int64(0)
+    L.check(L.lib().rv_forest_dtoc(f, nq, L.ptr(dtoc, c.c_int64), c.byref(nc)))
+    grid = Grid(gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc.value)
     grid.face_kind = kind
     grid.mortar_partner = partner
     grid.levels = gm.cells()[1]
+    (grid.vmapNC, grid.nctoface, grid.nctypes, grid.ncids) = forest_nctables(f, dim, nq)
     return grid
+
+
+def forest_nctables(forest, dim, nq):
+    """Raven's facemaps(grid).vmapNC, .nctoface, .nctypes, .ncids
+    (lobattocells.jl:2121-2276) of a forest: tuples over the face groups of
+    arrays in Julia's column-major layout stored C-order reversed, i.e.
+    vmapNC[g] has shape (nunique, 1 + 2^(dim-1), Nfp), nctoface[g] (nunique, 2)."""
+    c = L.ctypes
+    nun = np.zeros(dim, dtype=np.int64)
+    nfc = np.zeros(dim, dtype=np.int64)
+    L.check(L.lib().rv_forest_nc_counts(forest, L.ptr(nun, c.c_int64), L.ptr(nfc, c.c_int64)))
+    nfp, nch = nq ** (dim - 1), 1 << (dim - 1)
+    vmapNC = tuple(np.zeros((int(n), 1 + nch, nfp), dtype=np.int64) for n in nun)
+    nctoface = tuple(np.zeros((int(n), 2), dtype=np.int8) for n in nun)
+    nctypes = tuple(np.zeros(int(n), dtype=np.int8) for n in nfc)
+    ncids = tuple(np.zeros(int(n), dtype=np.int64) for n in nfc)
+    P64, P8 = c.POINTER(c.c_int64), c.POINTER(c.c_int8)
+    arr = lambda T, xs: (T * dim)(*[x.ctypes.data_as(T) for x in xs])  # noqa: E731
+    L.check(L.lib().rv_forest_nctables(forest, nq, arr(P64, vmapNC), arr(P8, nctoface),
+                                       arr(P8, nctypes), arr(P64, ncids)))
+    return vmapNC, nctoface, nctypes, ncids
+
+
+def mortars_from_nctables(dim, nq, K, vmapNC, nctoface):
+    """(face_kind (K, 2dim), partner (K, 2dim, 4)) for DGSEM.enable_mortars
+    from Raven's vmapNC / nctoface tuples (rv_mortars_from_nctables)."""
+    c = L.ctypes
+    vm = [np.ascontiguousarray(v, dtype=np.int64) for v in vmapNC]
+    nf = [np.ascontiguousarray(v, dtype=np.int8) for v in nctoface]
+    nun = np.array([v.shape[0] for v in vm], dtype=np.int64)
+    P64, P8 = c.POINTER(c.c_int64), c.POINTER(c.c_int8)
+    kind = np.zeros((K, 2 * dim), dtype=np.int32)
+    partner = np.zeros((K, 2 * dim, 4), dtype=np.int64)
+    L.check(L.lib().rv_mortars_from_nctables(
+        dim, nq, K, L.ptr(nun, c.c_int64), (P64 * dim)(*[v.ctypes.data_as(P64) for v in vm]),
+        (P8 * dim)(*[v.ctypes.data_as(P8) for v in nf]), L.ptr(kind, c.c_int32),
+        L.ptr(partner, c.c_int64)))
+    return kind, partner
 
 
 def min_node_distance(grid, dims=None):

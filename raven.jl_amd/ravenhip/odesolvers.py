@@ -2,7 +2,7 @@
 
 ``LSRK54(dg, q, dt; t0)`` / ``LSRK144`` build a solver; ``solve_(q, timeend,
 solver)`` is Raven's ``solve!`` (final step shortened to land on timeend) and
-runs entirely inside libravenhip (rv_lsrk_run); ``dostep_`` runs one step
+runs entirely inside libravenhip (rv_lsrk_solve); ``dostep_`` runs one step
 stage by stage through rv_lsrk_stage.  ``RLSRK54`` / ``RLSRK144`` are the
 relaxation variants (entropy-conserving or -stable steps, odesolvers.jl:72-162):
 ``solve_`` steps them one rv_rlsrk_step at a time.
@@ -13,11 +13,23 @@ import numpy as np
 
 from . import _lib as L
 
+
+def _clock(dg):
+    """The arithmetic of the solver's clock: FT(v) rounding and time + dt in
+    the state type (LSRK{Float32} keeps dt and time in Float32,
+    odesolvers.jl:32-49), plain doubles for FP64."""
+    if getattr(dg, "dtype", "f64") == "f32":
+        f = np.float32
+        return (lambda v: float(f(v))), (lambda a, b: float(f(a) + f(b)))
+    return float, (lambda a, b: a + b)
+
+
 class LSRK:
     def __init__(self, dg, q, dt, t0, scheme):
         self.dg = dg
-        self.dt = float(dt)
-        self.time = float(t0)
+        rnd, _ = _clock(dg)
+        self.dt = rnd(dt)
+        self.time = rnd(t0)
         self.scheme = scheme
         self.dq = dg.GridArray(0.0)
 
@@ -71,52 +83,50 @@ def _rlsrk_dostep(q, solver):
                                   L.ptr(g, ctypes.c_double), ctypes.byref(it)), dg.handle)
     solver.gamma = float(g[0])
     solver.newton_iterations = it.value
-    solver.time += solver.dt
+    solver.time = _clock(dg)[1](solver.time, solver.dt)
 
 
-def solve_(q, timeend, solver, after_step=None, after_stage=None):
-    """solve!(q, timeend, solver; after_step, after_stage) with adjust_final =
-    true (odesolvers.jl:9-31); returns #steps.  Without callbacks an LSRK solve
-    runs inside the library (rv_lsrk_run).  With them the steps are taken from
-    here: after_step(step, time, q) after every step (time at the step's
-    start), after_stage(stagetime, q_stage) after every LSRK stage, q_stage
-    being the stage's output (q itself after a step's last stage).  An RLSRK
-    step is one library call, so RLSRK solvers take after_step only."""
+def solve_(q, timeend, solver, after_step=None, after_stage=None, adjust_final=True):
+    """solve!(q, timeend, solver; after_step, after_stage, adjust_final)
+    (odesolvers.jl:9-31); returns #steps.  Steps are taken until time + dt >=
+    timeend; with adjust_final the last one is shortened to timeend - time.
+    Without callbacks an LSRK solve runs inside the library (rv_lsrk_solve).
+    With them the steps are taken from here: after_step(step, time, q) after
+    every step (time at the step's start), after_stage(stagetime, q_stage)
+    after every LSRK stage, q_stage being the stage's output (q itself after a
+    step's last stage).  An RLSRK step is one library call, so RLSRK solvers
+    take after_step only.  FP32 solvers keep their clock in FP32, comparing
+    with timeend as given (the library loop's arithmetic)."""
+    rnd, add = _clock(solver.dg)
     if isinstance(solver, RLSRK):
         if after_stage is not None:
             raise ValueError("RLSRK steps are single library calls: no after_stage")
-        step = 0
-        while True:
-            step += 1
-            time = solver.time
-            final = time + solver.dt >= timeend
-            if final:
-                solver.dt = timeend - time
-            _rlsrk_dostep(q, solver)
-            if after_step is not None:
-                after_step(step, time, q)
-            if final:
-                return step
-    if after_step is None and after_stage is None:
+        step_fn = _rlsrk_dostep
+    elif after_step is None and after_stage is None:
         t = ctypes.c_double(0.0)
         n = ctypes.c_int64(0)
         dg = solver.dg
-        L.check(L.lib().rv_lsrk_run(dg.handle, dg._check_state(q), dg._check_state(solver.dq),
-                                    solver.scheme, solver.time, solver.dt, float(timeend),
-                                    ctypes.byref(t), ctypes.byref(n)), dg.handle)
+        L.check(L.lib().rv_lsrk_solve(dg.handle, dg._check_state(q), dg._check_state(solver.dq),
+                                      solver.scheme, solver.time, solver.dt, float(timeend),
+                                      1 if adjust_final else 0, ctypes.byref(t),
+                                      ctypes.byref(n)), dg.handle)
         solver.time = t.value
         return n.value
-    # LSRK's dq starts at zero (odesolvers.jl:43-44), as rv_lsrk_run makes it
-    solver.dq.zero_()
-    qtmp = q.new_empty(q.shape)
+    else:
+        # LSRK's dq starts at zero (odesolvers.jl:43-44), as rv_lsrk_solve makes it
+        solver.dq.zero_()
+        qtmp = q.new_empty(q.shape)
+
+        def step_fn(q, solver):
+            dostep_(q, solver, qtmp, after_stage=after_stage)
     step = 0
     while True:
         step += 1
         time = solver.time
-        final = time + solver.dt >= timeend
-        if final:
-            solver.dt = timeend - time
-        dostep_(q, solver, qtmp, after_stage=after_stage)
+        final = add(time, solver.dt) >= timeend
+        if final and adjust_final:
+            solver.dt = rnd(timeend - time)
+        step_fn(q, solver)
         if after_step is not None:
             after_step(step, time, q)
         if final:
@@ -154,4 +164,4 @@ def dostep_(q, solver, qtmp=None, after_stage=None):
             after_stage(solver.time + c[s] * solver.dt, cur)
     if cur is not q:
         q.copy_(cur)
-    solver.time += solver.dt
+    solver.time = _clock(solver.dg)[1](solver.time, solver.dt)

@@ -156,20 +156,21 @@ class _Active:
         self.b = [rnd(v) for v in b]
         self.time = rnd(0.0)
         self.dt = rnd(req.dt())
-        self.T = rnd(req.T)
+        self.T = float(req.T)
+        self.rnd = rnd
         self.done = False
 
     def next_step(self):
-        """dt of the coming step and whether it is the last (odesolvers.jl:21-25)."""
+        """dt of the coming step and whether it is the last (odesolvers.jl:21-25):
+        time + dt in the state type, compared with T as given; the last step
+        is T - time rounded to the state type."""
         if self.f32:
             f = np.float32
             tdt = float(f(self.time) + f(self.dt))
         else:
             tdt = self.time + self.dt
         if tdt >= self.T:
-            step = float(np.float32(self.T) - np.float32(self.time)) if self.f32 \
-                else self.T - self.time
-            return step, True
+            return self.rnd(self.T - self.time), True
         return self.dt, False
 
     def bdt(self, s, step):

@@ -129,3 +129,34 @@ def golden_setup(case, K):
         return dict(dim=2, coords=[vx, vy], law="euler", u=None,
                     ic=lambda P, t: isentropic_vortex_2d(P, t), dt=dt, T=5.0)
     raise KeyError(case)
+
+
+def c3_warp(seed=99):
+    """The seeded smooth warp of the adaptive 3-D Euler workload: a periodic
+    perturbation of [0,10]^3 with amplitudes in [0.15, 0.3] and random phases.
+    Returns (warp, rng), rng advanced past the warp's draws (the workload then
+    draws its density noise from it)."""
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(0.15, 0.3, 3)
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    k = 2 * np.pi / 10
+
+    def warp(x, y, z):
+        return (x + amp[0] * np.sin(k * y + ph[0]) * np.sin(k * z),
+                y + amp[1] * np.sin(k * x + ph[1]) * np.sin(k * z),
+                z + amp[2] * np.sin(k * x) * np.sin(k * y + ph[2]))
+    return warp, rng
+
+
+def c3_gridmanager(R, nq=6, n=16, radius=2.0, levels=2):
+    """GridManager of the adaptive workload: an n^3 periodic brick on [0,10]^3
+    refined, one level per adapt! call, inside the sphere of `radius` at the
+    centre up to `levels` (2:1 balance is part of each call)."""
+    gm = R.GridManager(R.LobattoCell(nq, nq, nq),
+                       R.brick([np.linspace(0, 10, n + 1)] * 3, (True,) * 3))
+    for r in range(levels):
+        c = gm.cell_centres()
+        flags = np.where((np.linalg.norm(c - 5.0, axis=1) < radius) & (gm.cells()[1] <= r),
+                         R.AdaptRefine, R.AdaptNone)
+        gm.adapt(flags)
+    return gm

@@ -134,7 +134,108 @@ def test_adapted_forest_invariants(dim, nq, n, rounds, radius):
     assert np.any(kind == 1) and np.any(kind == 2)
 
 
-def test_adapt_rejects_coarsening_flags():
-    gm = R.GridManager(R.LobattoCell(3, 3), R.brick([PB.julia_range(0, 1, 3)] * 2, (True, True)))
-    with pytest.raises(R.RavenHIPError):
-        gm.adapt(np.full(len(gm), 0x01, dtype=np.uint8))
+def leaf_set(gm):
+    tree, level, quad = gm.cells()
+    return [(tuple(t), int(lv), tuple(q)) for t, lv, q in zip(tree, level, quad)]
+
+
+def assert_balanced(gm, n):
+    """Every pair of leaves touching across a face, edge or corner differs by
+    at most one level (periodic brick of n cells per direction)."""
+    tree, level, quad = gm.cells()
+    dim, K = tree.shape[1], len(gm)
+    finest = int(level.max())
+    owner = {}
+    for e in range(K):
+        s = 1 << (finest - level[e])
+        lo = [(tree[e][d] << finest) + (quad[e][d] << (finest - level[e])) for d in range(dim)]
+        for off in itertools.product(range(s), repeat=dim):
+            owner[tuple(lo[d] + off[d] for d in range(dim))] = e
+    N = n << finest
+    for e in range(K):
+        s = 1 << (finest - level[e])
+        lo = [(tree[e][d] << finest) + (quad[e][d] << (finest - level[e])) for d in range(dim)]
+        for delta in itertools.product((-1, 0, 1), repeat=dim):
+            if any(delta):
+                p = tuple((lo[d] + (s if delta[d] > 0 else -1 if delta[d] < 0 else 0)) % N
+                          for d in range(dim))
+                assert abs(int(level[owner[p]]) - int(level[e])) <= 1
+
+
+def test_adapt_takes_ravens_flag_values():
+    """AdaptNone (0x01) and AdaptTouched (0x11) keep a leaf, AdaptRefine (0x10)
+    splits it (gridmanager.jl:24-27, refine_quads :133-136), and the flags
+    come back as fill_adapt_flags writes them (:117-129)."""
+    assert (R.AdaptCoarsen, R.AdaptNone, R.AdaptRefine, R.AdaptTouched) == (0, 1, 0x10, 0x11)
+    gm = R.GridManager(R.LobattoCell(3, 3), R.brick([PB.julia_range(0, 1, 5)] * 2, (True, True)))
+    before = leaf_set(gm)
+    flags = np.full(len(gm), R.AdaptNone, dtype=np.uint8)
+    flags[3] = R.AdaptTouched
+    gm.adapt(flags)
+    assert leaf_set(gm) == before
+    assert np.all(flags == R.AdaptNone)
+    # coarsening flags on level-0 leaves (trees) have no family: kept
+    flags = np.full(len(gm), R.AdaptCoarsen, dtype=np.uint8)
+    gm.adapt(flags)
+    assert leaf_set(gm) == before and np.all(flags == R.AdaptNone)
+    flags = np.full(len(gm), R.AdaptNone, dtype=np.uint8)
+    flags[5] = R.AdaptRefine
+    out = gm.adapt(list(flags))  # a list: the result is returned
+    assert len(gm) == 16 + 3
+    assert out[5] == R.AdaptRefine and np.sum(out == R.AdaptNone) == 15
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_refine_then_coarsen_restores_the_forest(dim):
+    """Coarsening every complete family of a uniformly refined forest gives the
+    coarse forest back leaf for leaf; the flags report AdaptCoarsen."""
+    gm = R.GridManager(R.LobattoCell(*([3] * dim)),
+                       R.brick([PB.julia_range(0, 1, 4)] * dim, (True,) * dim))
+    before = leaf_set(gm)
+    gm.adapt(np.full(len(gm), R.AdaptRefine, dtype=np.uint8))
+    assert len(gm) == len(before) * 2 ** dim
+    flags = np.full(len(gm), R.AdaptCoarsen, dtype=np.uint8)
+    gm.adapt(flags)
+    assert leaf_set(gm) == before
+    assert np.all(flags == R.AdaptCoarsen)
+
+
+def test_coarsening_needs_the_whole_family():
+    """A family with one member not flagged AdaptCoarsen stays (coarsen_quads
+    requires every child, gridmanager.jl:138-146); coarsening is not recursive
+    (one level per call)."""
+    gm = R.GridManager(R.LobattoCell(3, 3), R.brick([PB.julia_range(0, 1, 3)] * 2, (True, True)),
+                       min_level=2)
+    K = len(gm)
+    flags = np.full(K, R.AdaptCoarsen, dtype=np.uint8)
+    flags[0] = R.AdaptNone  # breaks the first family only
+    gm.adapt(flags)
+    tree, level, _ = gm.cells()
+    assert len(gm) == K - 3 * (K // 4 - 1)
+    assert np.sum(level == 2) == 4 and np.sum(level == 1) == K // 4 - 1
+    assert flags[0] == R.AdaptNone and np.sum(flags == R.AdaptCoarsen) == K - 4
+    assert np.all(flags[1:4] == R.AdaptNone)
+
+
+def test_coarsening_is_rebalanced():
+    """A coarsened family next to a two-level-finer region is refined again by
+    the 2:1 balance (coarsen!, refine!, balance! in that order,
+    gridmanager.jl:160-172), and every other family coarsens."""
+    n = 4
+    gm = sphere_forest(2, 3, n, 2, 1.2)
+    tree, level, quad = gm.cells()
+    assert level.max() == 2
+    flags = np.where(level == 1, R.AdaptCoarsen, R.AdaptNone).astype(np.uint8)
+    gm.adapt(flags)
+    assert_balanced(gm, n)
+    _, level2, _ = gm.cells()
+    assert np.sum(level2 == 2) == np.sum(level == 2)  # the finest leaves are kept
+    # families next to level-2 leaves come back at level 1 (AdaptNone); the
+    # others report AdaptCoarsen
+    assert np.any(flags == R.AdaptCoarsen) and np.any((level == 1) & (flags == R.AdaptNone))
+    # the balanced forest is the coarsest one: coarsening again changes nothing
+    # where balance forbids it
+    before = leaf_set(gm)
+    tree, lv, quad = gm.cells()
+    gm.adapt(np.where(lv == 1, R.AdaptCoarsen, R.AdaptNone).astype(np.uint8))
+    assert leaf_set(gm) == before

@@ -844,6 +844,78 @@ def test_solve_with_callbacks_matches_library_loop(torch, R):
         R.solve_(qb, T, R.RLSRK54(dg, qb, dt), after_stage=lambda t, q: None)
 
 
+def test_fp32_solve_with_callbacks_matches_library_loop(torch, R):
+    """The FP32 clock: LSRK{Float32} keeps dt and time in Float32, so with
+    dt = 0.1 and T = 1.0 the loop takes 10 steps (in doubles it would take an
+    11th step about 1e-16 long).  The Python loop (callbacks) and the library
+    loop agree on the step count and bit for bit on the state."""
+    coords = [PB.julia_range(0, 2, 5)] * 2
+    grid, dg, og, odg = make_pair(R, 2, 5, coords, (True, True), "advection", u=[0.01, 0.02],
+                                  dtype="f32")
+    q0 = initial_state(og, "advection", 2, seed=32)
+    dt, T = 0.1, 1.0
+    qa = to_dev(torch, dg, q0)
+    sa = R.LSRK54(dg, qa, dt)
+    na = R.solve_(qa, T, sa)
+    qb = to_dev(torch, dg, q0)
+    steps = []
+    sb = R.LSRK54(dg, qb, dt)
+    nb = R.solve_(qb, T, sb, after_step=lambda n, t, q: steps.append(t))
+    dg.synchronize()
+    say(f"   FP32 clock: library {na} steps, callbacks {nb}")
+    assert na == nb == len(steps) == 10
+    assert sa.time == sb.time
+    assert torch.equal(qa, qb)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_solve_without_adjust_final(torch, R, dtype):
+    """solve!(...; adjust_final = false) (odesolvers.jl:15,23-24): the last
+    step keeps dt and ends past timeend.  Library loop, Python loop and
+    stage-by-stage stepping agree bit for bit; FP64 also matches the oracle's
+    fixed-step run over the same number of steps."""
+    coords = [PB.julia_range(0, 2, 5)] * 2
+    grid, dg, og, odg = make_pair(R, 2, 5, coords, (True, True), "euler", dtype=dtype)
+    q0 = initial_state(og, "euler", 2, seed=33)
+    dt, T = 0.003, 0.01
+    qa = to_dev(torch, dg, q0)
+    sa = R.LSRK54(dg, qa, dt)
+    na = R.solve_(qa, T, sa, adjust_final=False)
+    qb = to_dev(torch, dg, q0)
+    sb = R.LSRK54(dg, qb, dt)
+    nb = R.solve_(qb, T, sb, adjust_final=False, after_step=lambda n, t, q: None)
+    qc = to_dev(torch, dg, q0)
+    sc = R.LSRK54(dg, qc, dt)
+    tmp = torch.empty_like(qc)
+    for _ in range(4):
+        R.dostep_(qc, sc, tmp)
+    dg.synchronize()
+    assert na == nb == 4
+    assert sa.time == sb.time == sc.time and sa.time > T
+    assert sa.dt == sb.dt == sc.dt
+    assert torch.equal(qa, qb) and torch.equal(qa, qc)
+    if dtype == "f64":
+        qo, dqo = q0.copy(), np.zeros(q0.size)
+        for k in range(4):
+            odg.lsrk_step(qo, dqo, k * dt, dt)
+        assert rel_l2(qa.cpu().numpy().reshape(-1), qo) <= RTOL_F64
+
+
+def test_norms_propagate_nan(torch, R):
+    """A NaN anywhere in a state makes weightednorm NaN for p = 2 and p = inf,
+    as Julia's sum and maximum do (measures.jl:6-21)."""
+    coords = [PB.julia_range(0, 2, 4)] * 3
+    grid, dg, og, odg = make_pair(R, 3, 4, coords, (True,) * 3, "euler")
+    q = initial_state(og, "euler", 3, seed=42)
+    q[q.size // 3] = np.nan
+    qd = to_dev(torch, dg, q)
+    s = (q.size // 3) // og.Np % 5
+    for p in (2, float("inf")):
+        comp = R.weightednorm(dg, qd, p=p, componentwise=True)
+        assert np.isnan(comp[s]) and np.all(np.isfinite(np.delete(comp, s)))
+        assert np.isnan(R.weightednorm(dg, qd, p=p))
+
+
 @pytest.mark.parametrize("p", [1, 3, 2.5, float("inf")])
 def test_weightednorm_any_p(torch, R, p):
     """weightednorm(dg, q, p; componentwise) (measures.jl:6-21): a constant

@@ -587,3 +587,32 @@ def test_directions_split_the_right_hand_side(law, dim, vflux):
         assert np.linalg.norm(r - full) > 1e-3 * np.linalg.norm(full)
         tot += r
     assert np.linalg.norm(tot - full) <= 1e-12 * np.linalg.norm(full)
+
+
+@pytest.mark.parametrize("vary", [0, 1, 2])
+def test_directions_name_the_varying_coordinate(vary):
+    """`directions` counts Raven's directions from 1 = x (dgsem.jl:32-36): for
+    an Euler state that varies along one coordinate only, the right-hand side
+    restricted to that direction is the whole right-hand side, and the other
+    directions' part vanishes to rounding (uniform neighbours' face fluxes
+    cancel the weak-form volume term).  A reversed or rotated bit order fails
+    both checks, which the split identity alone would not notice."""
+    nq = 5
+    coords = [PB.julia_range(0, 2, 4)] * 3
+    og = O.BrickGrid(nq, coords, (True,) * 3)
+    x = og.point_fields()[:, vary, :]
+    rho = 2 + 0.3 * np.sin(np.pi * x)
+    mom = [rho * 0.4, rho * 0.2, rho * -0.1]
+    q = np.ascontiguousarray(np.stack([rho] + mom + [PB.euler_energy(rho, mom, 1.0, 1.4)],
+                                      axis=1)).reshape(-1)
+    others = tuple(d for d in (1, 2, 3) if d != vary + 1)
+    out = {}
+    for dirs in [None, (vary + 1,), others]:
+        r = np.zeros(q.size)
+        O.DG(og, "euler", directions=dirs).rhs(r, q.copy(), increment=False)
+        out[dirs] = r
+    full = out[None]
+    scale = np.linalg.norm(full)
+    assert scale > 1.0
+    assert np.linalg.norm(out[(vary + 1,)] - full) <= 1e-12 * scale
+    assert np.linalg.norm(out[others]) <= 1e-12 * scale
