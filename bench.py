@@ -139,13 +139,20 @@ def run_gpu(R, torch, gm, q0, dtype, steps, warmup, dt, copy_probe=False):
     r1.record()
     torch.cuda.synchronize()
     rhs_ms = r0.elapsed_time(r1) / 5
-    if copy_probe:
-        # a plain device copy of one state array (known bytes: read + write
-        # state_bytes); profilers use it to scale HBM traffic counters
+    # the measured copy peak: a plain device copy of one state array (read
+    # and write state_bytes), best of three; profilers also use this copy to
+    # scale HBM traffic counters
+    copy_gbs = 0.0
+    for _ in range(3 if copy_probe or dtype == "f64" else 0):
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
         bufs[1 - cur[0]].copy_(qf)
+        c1.record()
         torch.cuda.synchronize()
+        copy_gbs = max(copy_gbs, 2 * info.state_bytes / (c0.elapsed_time(c1) * 1e-3) / 1e9)
     ints = dg.integrals(qf)
     res = dict(dg=dg, elapsed=elapsed, wall=wall, durs=durs, info=info, rhs_ms=rhs_ms,
+               copy_gbs=copy_gbs,
                q=qf.double().cpu().numpy().reshape(-1), integrals=ints,
                integrals0=None)
     # conservation reference: integrals of the initial state
@@ -194,7 +201,20 @@ def cpu_baseline(q0, dt, budget_s):
                   "workload")
     return {"value": dofs * stages / t_total, "unit": "DOF-updates/s", "cores": cores,
             "kind": "port", "sample": sample, "seconds": round(t_total, 3),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "note": "C restatement of Raven's CPU path (oracle/), not Raven itself"}
+
+
+def cpu_model():
+    """The host CPU's model name (what lscpu reports), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 _ORACLE_GRID = {}
@@ -287,6 +307,21 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     t_seq = time.perf_counter() - t0
     info = batch.info
     batch.close()
+    # every member's batched state against its plain stage-by-stage run
+    # (rv_lsrk_stage per stage, `steps` whole steps): bit for bit
+    same = 0
+    for r, dg, q_init, dt, qb in zip(reqs, dgs, q0, dts, qs):
+        qp = q_init.clone()
+        sv = R.LSRK54(dg, qp, dt)
+        tmp = torch.empty_like(qp)
+        for _ in range(steps):
+            R.dostep_(qp, sv, tmp)
+        torch.cuda.synchronize()
+        same += int(torch.equal(qp, qb))
+    # algorithmic HBM bytes of one LSRK54 step of every member (stage 1 does
+    # not read dq), per second of the batched run
+    step_bytes = sum(dg.info.stage_bytes_first + 4 * dg.info.stage_bytes for dg in dgs)
+    gbs = step_bytes * steps / t_batch / 1e9
     return {
         "workload": f"{n_inst} independent 2-D periodic brick solves (advection and Euler, "
                     "degree 3-7, 8^2..64^2 cells, FP64 and FP32), LSRK54",
@@ -297,7 +332,10 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
         "sequential_ms_per_step": t_seq / steps * 1e3,
         "speedup_vs_sequential": t_seq / t_batch,
         "sequential_steps_match": nsteps == steps * n_inst,
+        "states_equal_to_plain_stepping": f"{same}/{n_inst}",
         "finite": finite,
+        "algorithmic_bytes_per_step": step_bytes,
+        "batched_achieved_GBs": gbs, "batched_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
         "work_items": info["items"], "persistent_waves": info["waves"],
         "launch": "per stage: one launch per kind large enough to fill the device, one "
                   "persistent work-list launch for the small kinds; one hipGraph per step",
@@ -451,7 +489,9 @@ def run_adaptive(R, torch, steps=200, nq=6, seed=99):
             torch.cuda.synchronize()
             sec = ev[0].elapsed_time(ev[1]) / 1e3
             i1 = dg.integrals(q)
+            info = dg.info
             res[dtype] = dict(q=q.double().cpu().numpy().reshape(-1), sec=sec, steps=n,
+                              step_bytes=info.stage_bytes_first + 4 * info.stage_bytes,
                               mass_drift=float((i1[0] - i0[0]) / i0[0]),
                               energy_drift=float((i1[4] - i0[4]) / i0[4]))
             del dg, q, solver
@@ -474,24 +514,52 @@ def run_adaptive(R, torch, steps=200, nq=6, seed=99):
             "energy_drift_fp32": num(res["f32"]["energy_drift"]),
             "steps_run": res["f64"]["steps"],
         }
+        # algorithmic HBM bytes per step (state, FP64 per-node geometry and
+        # per-face-node normals and weights; mortar work not counted) over
+        # the measured time of the run
+        for dtype in ("f64", "f32"):
+            gbs = res[dtype]["step_bytes"] * res[dtype]["steps"] / res[dtype]["sec"] / 1e9
+            out[key][f"{dtype}_bytes_per_step"] = res[dtype]["step_bytes"]
+            out[key][f"{dtype}_achieved_GBs"] = gbs
+            out[key][f"{dtype}_frac_of_hbm_peak"] = gbs / HBM_PEAK_GBS
     return out
 
 
 def run_served(R, torch, rate, n_req=32, seed=7, budget_gb=64):
     """A Poisson request stream (2-D / 3-D advection / Euler, degree 3-7,
     1e3..1e5 cells log-uniform, T in [0.1, 1], FP32 / FP64) served with
-    continuous batching, and the same stream served one request at a time."""
+    continuous batching, and the same stream served one request at a time.
+    Afterwards (outside both timed runs): each request's served result is
+    compared with its one-at-a-time result bit for bit, and each FP32
+    request's result with the same request solved in FP64 one at a time."""
+    import dataclasses
+
     import ravenhip.serve as S
     served = S.random_requests(n_req, rate, seed=seed)
     server = S.Server(hbm_budget=budget_gb << 30)
     t0 = time.perf_counter()
-    server.run(served)
+    server.run(served, keep_results=True)
     wall = time.perf_counter() - t0
     plain = S.random_requests(n_req, rate, seed=seed)
     t0 = time.perf_counter()
-    S.run_plain(plain)
+    S.run_plain(plain, keep_results=True)
     wall_plain = time.perf_counter() - t0
     a, b = S.summarize(served, wall), S.summarize(plain, wall_plain)
+    equal = sum(int(torch.equal(x.result, y.result)) for x, y in zip(served, plain))
+    errs = []
+    for x in served:
+        if x.dtype != "f32":
+            continue
+        ref = dataclasses.replace(x, dtype="f64", arrival=0.0, result=None, steps=0)
+        S.run_plain([ref], keep_results=True)
+        q32, q64 = x.result.double(), ref.result
+        errs.append(float(torch.linalg.norm(q32 - q64) / torch.linalg.norm(q64)))
+        ref.result = None
+    for x in served + plain:
+        x.result = None
+    # algorithmic HBM bytes of the served work: every step of every request,
+    # stage 1 reading q (writing dq, q), stages 2-5 also reading dq
+    work_bytes = sum(x.steps * x.state_bytes() * (3 + 4 * 4) for x in served)
     return {"rate_per_s": rate if math.isfinite(rate) else "burst", "requests": n_req,
             "hbm_budget_gb": budget_gb,
             "served": a, "one_at_a_time": b,
@@ -500,7 +568,12 @@ def run_served(R, torch, rate, n_req=32, seed=7, budget_gb=64):
             "peak_resident": server.peak_active,
             "server_host_s": {k: round(v, 3) for k, v in server.host_time.items()},
             "server_steps": server.steps_run,
-            "steps_match": all(x.steps == y.steps for x, y in zip(served, plain))}
+            "steps_match": all(x.steps == y.steps for x, y in zip(served, plain)),
+            "results_equal_to_one_at_a_time": f"{equal}/{n_req}",
+            "fp32_rel_l2_vs_fp64_solve": {"requests": len(errs),
+                                          "max": max(errs) if errs else None,
+                                          "median": float(np.median(errs)) if errs else None},
+            "served_algorithmic_GBs": work_bytes / wall / 1e9}
 
 
 def main():
@@ -596,6 +669,10 @@ def main():
             "single_rhs_frac": 2 * r64["info"].state_bytes / (r64["rhs_ms"] * 1e-3) / 1e9
             / HBM_PEAK_GBS,
             "algorithmic_bytes_per_launch": bytes_step / 5,
+            # a device copy of one state measured in the same run: the
+            # attainable streaming rate on this box
+            "measured_copy_GBs": r64["copy_gbs"],
+            "frac_of_measured_copy": achieved / r64["copy_gbs"] if r64["copy_gbs"] else None,
         },
     }
     if not args.no_fp32:

@@ -96,7 +96,8 @@ function HIPDGSEM(; law, grid, surface_numericalflux, volume_form = WeakForm(),
     vmapM = Array{Int64}(fm.vmapM)
     vmapP = Array{Int64}(fm.vmapP)
     bc = Array{Int64}(boundarycodes(grid))
-    dtoc = Array{Int64}(discontinuoustocontinuous(grid))
+    # not exported by Raven (grids.jl:68, 86)
+    dtoc = Array{Int64}(Raven.discontinuoustocontinuous(grid))
     u = law isa AdvectionLaw ? Tuple(Float64.(constants(law).u⃗)) : ()
     vel = ntuple(i -> i <= length(u) ? u[i] : 0.0, 3)
     γ = law isa EulerLaw ? Float64(constants(law).γ) : 1.4
@@ -109,7 +110,7 @@ function HIPDGSEM(; law, grid, surface_numericalflux, volume_form = WeakForm(),
                       dtype == Float64 ? RV_F64 : RV_F32, 0, dirmask, length(grid), γ, vel,
                       pointer(D), pointer(w), pointer(metrics), pointer(surf),
                       pointer(vmapM), pointer(vmapP), pointer(bc), pointer(dtoc),
-                      numcnodes(grid), stream, form, vflux)
+                      Raven.numcnodes(grid), stream, form, vflux)
         check(ccall((:rv_create, libravenhip), Cint, (Ref{RvDesc}, Ref{Ptr{Cvoid}}),
                     desc, handle), C_NULL)
     end
@@ -123,16 +124,44 @@ function (dg::HIPDGSEM)(dq, q, time; increment = true)
                 dg.handle, pointer(parent(dq)), pointer(parent(q)), time, increment), dg.handle)
 end
 
-# solve!(q, timeend, LSRK54(dg, q, dt)) (odesolvers.jl:9-30) in one call.
-function lsrk_solve!(dg::HIPDGSEM, q, dq, dt, timeend; t0 = 0.0, scheme = 54)
+# solve!(q, timeend, LSRK54(dg, q, dt); adjust_final) (odesolvers.jl:9-30) in
+# one call.
+function lsrk_solve!(dg::HIPDGSEM, q, dq, dt, timeend; t0 = 0.0, scheme = 54,
+                     adjust_final = true)
     tout = Ref{Cdouble}(0)
     nsteps = Ref{Int64}(0)
-    check(ccall((:rv_lsrk_run, libravenhip), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble,
+    check(ccall((:rv_lsrk_solve, libravenhip), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cint,
                  Ref{Cdouble}, Ref{Int64}),
                 dg.handle, pointer(parent(q)), pointer(parent(dq)), scheme, t0, dt, timeend,
-                tout, nsteps), dg.handle)
+                adjust_final ? 1 : 0, tout, nsteps), dg.handle)
     return tout[], nsteps[]
+end
+
+# Couple the hanging faces of an adapted grid through mortars, from Raven's own
+# non-conforming tables facemaps(grid).vmapNC / .nctoface
+# (lobattocells.jl:2121-2276): rv_mortars_from_nctables, then rv_enable_mortars.
+function enable_mortars!(dg::HIPDGSEM, grid)
+    cell = referencecell(grid)
+    dim = ndims(cell)
+    Nq = size(cell, 1)
+    K = Int64(numcells(grid))
+    fm = facemaps(grid)
+    vmapNC = [Array{Int64}(v) for v in fm.vmapNC]
+    nctoface = [Array{Int8}(v) for v in fm.nctoface]
+    nunique = Int64[size(v, ndims(v)) for v in vmapNC]
+    kind = zeros(Int32, 2dim, K)
+    partner = zeros(Int64, 4, 2dim, K)
+    GC.@preserve vmapNC nctoface begin
+        vptrs = Ptr{Int64}[pointer(v) for v in vmapNC]
+        fptrs = Ptr{Int8}[pointer(v) for v in nctoface]
+        check(ccall((:rv_mortars_from_nctables, libravenhip), Cint,
+                    (Int32, Int32, Int64, Ptr{Int64}, Ptr{Ptr{Int64}}, Ptr{Ptr{Int8}},
+                     Ptr{Int32}, Ptr{Int64}),
+                    dim, Nq, K, nunique, vptrs, fptrs, kind, partner), C_NULL)
+    end
+    check(ccall((:rv_enable_mortars, libravenhip), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int64}),
+                dg.handle, kind, partner), dg.handle)
 end
 
 # entropyintegral(dg, q) / entropyproduct(dg, p, q) (measures.jl:23-33)

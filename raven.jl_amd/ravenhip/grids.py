@@ -209,7 +209,7 @@ def _check_uniform(cg, desc):
             raise NotImplementedError("only uniformly spaced brick coordinates are supported")
 
 
-def generate(gm, warp=None, metrics=True):
+def generate(gm, warp=None, with_metrics=True):
     """generate(gm) / generate(warp, gm) (gridmanager.jl:1084-1208).
 
     warp(x, y[, z]) maps arrays of node coordinates to a tuple of arrays (the
@@ -219,11 +219,11 @@ def generate(gm, warp=None, metrics=True):
     (gm.adapt).  An adapted forest's grid also carries Raven's non-conforming
     tables (grid.vmapNC, .nctoface, .nctypes, .ncids, one entry per face
     group) and its continuous numbering with hanging nodes (rv_forest_dtoc).
-    metrics=False skips the device-side metrics of curved or adapted grids
+    with_metrics=False skips the device-side metrics of curved or adapted grids
     (grid.metrics and grid.surface are then None): the integer arrays and
     points alone, without a device."""
     if gm.adapted:
-        return _generate_forest(gm, warp, metrics)
+        return _generate_forest(gm, warp, with_metrics)
     desc = gm.brick_desc()
     _check_uniform(gm.coarsegrid, desc)
     lib = L.lib()
@@ -251,7 +251,7 @@ def generate(gm, warp=None, metrics=True):
         points = np.ascontiguousarray(
             np.stack([np.broadcast_to(np.asarray(x, dtype=np.float64), (K, Np)) for x in X],
                      axis=1))
-        metrics, surface = curved_metrics(gm.cell, points) if metrics else (None, None)
+        metrics, surface = curved_metrics(gm.cell, points) if with_metrics else (None, None)
     return Grid(gm, points, metrics, surface, vmapM, vmapP, bc, dtoc, nc.value)
 
 

@@ -59,7 +59,7 @@ def test_uniform_refinement_numbering_is_the_bricks(dim, nq, n, level, periodic)
     gm = R.GridManager(R.LobattoCell(*([nq] * dim)), R.brick(coords, per))
     for _ in range(level):
         gm.adapt(np.full(len(gm), R.AdaptRefine, dtype=np.uint8))
-    grid = R.generate(gm, metrics=False)
+    grid = R.generate(gm, with_metrics=False)
     assert grid.ncontinuous == ref.ncontinuous
     assert np.array_equal(grid.dtoc, ref.dtoc)
 
@@ -86,7 +86,7 @@ def test_numbering_matches_the_reference_test_drawing():
     order = [(0, 0, 0, 0, 0), (1, 0, 0, 0, 0), (0, 1, 0, 0, 0),
              (1, 1, 1, 0, 0), (1, 1, 1, 1, 0), (1, 1, 1, 0, 1), (1, 1, 1, 1, 1)]
     assert [tuple(tree[e]) + (level[e],) + tuple(quad[e]) for e in range(len(gm))] == order
-    grid = R.generate(gm, metrics=False)
+    grid = R.generate(gm, with_metrics=False)
     want = first_appearance(np.array(drawn).reshape(-1))
     assert grid.ncontinuous == 37
     assert np.array_equal(grid.dtoc.reshape(-1), want)
@@ -109,7 +109,7 @@ def hanging_face_targets(grid, e, fc):
                                                     (2, 5, 6, 2, 2.0)])
 def test_forest_numbering_invariants(dim, nq, n, rounds, radius):
     gm = forest_gm(dim, nq, n, rounds, radius)
-    grid = R.generate(gm, metrics=False)
+    grid = R.generate(gm, with_metrics=False)
     K, Np, nfp = grid.K, grid.Np, grid.nfp
     d = grid.dtoc.reshape(-1)
     # ids 1..n, numbered by first appearance
@@ -201,7 +201,7 @@ def test_forest_nctables_points_are_tohalves_of_the_parent():
     halves of c - 1 = s1 + 2 s2."""
     nq = 4
     gm = forest_gm(3, nq, 4, 2, 1.2)
-    grid = R.generate(gm, metrics=False)
+    grid = R.generate(gm, with_metrics=False)
     lower, upper = np.zeros(nq * nq), np.zeros(nq * nq)
     R._lib.check(R.lib().rv_lobatto_halves(nq, R._lib.ptr(lower, c.c_double),
                                            R._lib.ptr(upper, c.c_double)))
@@ -230,7 +230,7 @@ def test_forest_nctables_points_are_tohalves_of_the_parent():
 @pytest.mark.parametrize("dim,nq", [(3, 3), (2, 5)])
 def test_nctables_convert_to_the_forest_partners(dim, nq):
     gm = forest_gm(dim, nq, 6 if dim == 2 else 4, 2, 2.0 if dim == 2 else 1.2)
-    grid = R.generate(gm, metrics=False)
+    grid = R.generate(gm, with_metrics=False)
     kind, partner = G.mortars_from_nctables(dim, nq, grid.K, grid.vmapNC, grid.nctoface)
     assert np.array_equal(kind, grid.face_kind)
     assert np.array_equal(partner, grid.mortar_partner)
@@ -243,7 +243,7 @@ def test_nctables_convert_to_the_forest_partners(dim, nq):
 
 def test_nctables_with_rotated_nodes_are_refused():
     gm = forest_gm(3, 3, 4, 1, 1.5)
-    grid = R.generate(gm, metrics=False)
+    grid = R.generate(gm, with_metrics=False)
     vm = [v.copy() for v in grid.vmapNC]
     g = next(i for i in range(3) if vm[i].shape[0])
     vm[g][0, 0, [1, 3]] = vm[g][0, 0, [3, 1]]  # transpose two nodes of a parent face
