@@ -210,6 +210,18 @@ int rv_lsrk_solve(rv_handle *h, void *q, void *dq, int scheme, double t0, double
 int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
                 double timeend, double *t_out, int64_t *nsteps_out);
 
+/* rv_lsrk_run without any device allocation.  No counterpart in Raven: for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  The ping-pong state is a library-owned device
+ * array of RV_NOALLOC_SCRATCH_BYTES (32 MiB), one per GPU, held until the
+ * process ends; states larger than that are rejected with RV_E_ARG.  The
+ * caller passes q and dq as for rv_lsrk_run.  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (other streams or host
+ * threads) race and must be ordered by the caller. */
+#define RV_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int rv_lsrk_run_noalloc(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                        double timeend, double *t_out, int64_t *nsteps_out);
+
 /* The stage coefficients rka, rkb, rkc of a scheme (coefficients_lsrk54 /
  * coefficients_lsrk144, odesolvers.jl:163-244) in double precision, for
  * callers that step stage by stage through rv_lsrk_stage.  Any array pointer

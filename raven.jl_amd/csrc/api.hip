@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1496,6 +1497,67 @@ int rv_lsrk_run(rv_handle *h, void *q, void *dq, int scheme, double t0, double d
                 double timeend, double *t_out, int64_t *nsteps_out)
 {
     return rv_lsrk_solve(h, q, dq, scheme, t0, dt, timeend, 1, t_out, nsteps_out);
+}
+
+// The ping-pong state of rv_lsrk_run_noalloc: a file-scope device array of
+// RV_NOALLOC_SCRATCH_BYTES.  Every GPU the library runs on holds its own copy
+// at its own address until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory: kernels and copies get the
+// address hipGetSymbolAddress returns for the device.
+static __device__ __align__(256) unsigned char scratch[RV_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the handle's device, looked up on the first
+// call on that device with the device current, then kept.
+int scratch_address(rv_handle *h, void **out)
+{
+    if (h->device < 0 || h->device >= SCRATCH_MAX_DEVICES)
+        return fail(h, RV_E_UNSUPPORTED, "device index beyond the no-alloc scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        int cur = 0;
+        HIP_TRY(h, hipGetDevice(&cur));
+        if (cur != h->device)
+            HIP_TRY(h, hipSetDevice(h->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != h->device)
+            HIP_TRY(h, hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(h, RV_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        g_scratch_ptr[h->device] = p;
+    }
+    *out = g_scratch_ptr[h->device];
+    return RV_OK;
+}
+
+}  // namespace
+
+int rv_lsrk_run_noalloc(rv_handle *h, void *q, void *dq, int scheme, double t0, double dt,
+                        double timeend, double *t_out, int64_t *nsteps_out)
+{
+    if (!h || !q || !dq)
+        return fail(h, RV_E_ARG, "null argument");
+    if (!(dt > 0))
+        return fail(h, RV_E_ARG, "dt must be positive");
+    const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    if (bytes > (size_t)RV_NOALLOC_SCRATCH_BYTES)
+        return fail(h, RV_E_ARG,
+                    "rv_lsrk_run_noalloc: a state of " + std::to_string(bytes) +
+                        " bytes exceeds the " + std::to_string(RV_NOALLOC_SCRATCH_BYTES) +
+                        "-byte scratch; use rv_lsrk_run");
+    void *alt = nullptr;
+    if (int rc = scratch_address(h, &alt))
+        return rc;
+    // the scratch is written in full by the first stage before it is read,
+    // so nothing in it has to start at zero
+    return lsrk_run_impl(h, q, dq, scheme, t0, dt, timeend, 1, alt, t_out, nsteps_out);
 }
 
 int rv_entropy_integral(rv_handle *h, const void *q, double *out)

@@ -19,6 +19,8 @@ RV_F64, RV_F32 = 1, 2
 RV_GEOM_AUTO, RV_GEOM_AFFINE, RV_GEOM_GENERAL = 0, 1, 2
 RV_LSRK54, RV_LSRK144 = 54, 144
 RV_E_ARG, RV_E_UNSUPPORTED = -1, -4
+# device scratch of rv_lsrk_run_noalloc (RV_NOALLOC_SCRATCH_BYTES)
+RV_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 ERRORS = {-1: "RV_E_ARG", -2: "RV_E_HIP", -3: "RV_E_NOMEM", -4: "RV_E_UNSUPPORTED",
           -5: "RV_E_MAPS", -6: "RV_E_STATE"}
@@ -123,6 +125,9 @@ def lib():
                                          ctypes.c_double]),
         "rv_lsrk_run": (ctypes.c_int, [H, vp, vp, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, c_f64p, c_i64p]),
+        "rv_lsrk_run_noalloc": (ctypes.c_int, [H, vp, vp, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, c_f64p,
+                                               c_i64p]),
         "rv_lsrk_solve": (ctypes.c_int, [H, vp, vp, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          c_f64p, c_i64p]),
