@@ -596,6 +596,8 @@ def main():
                     help="skip the relaxation RK (RLSRK54) workload")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="skip the adaptive warped degree-5 FP32/FP64 workload")
+    ap.add_argument("--adaptive-steps", type=int, default=200,
+                    help="LSRK54 steps of the adaptive workload (200: the configuration)")
     ap.add_argument("--serve-rates", default="8,32,burst",
                     help="request arrival rates (per second) of the served stream; "
                          "'burst': every request arrives at once")
@@ -714,7 +716,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_adaptive:
         log("bench: adaptive warped 16^3 degree-5 workload")
-        out["adaptive"] = run_adaptive(R, torch)
+        out["adaptive"] = run_adaptive(R, torch, steps=args.adaptive_steps)
         torch.cuda.empty_cache()
     if not args.no_batched:
         log("bench: 256 batched 2-D solves")
