@@ -245,6 +245,44 @@ def test_full_c3_forest_stage_matches_oracle(torch, R, c3, mortar, dtype):
         assert e3 <= tol
 
 
+def test_self_mapped_forest_diverges_with_the_oracle(torch, R, c3):
+    """Raven leaves hanging faces self-mapped (mapP is the identity there,
+    lobattocells.jl:2072-2074), so the coarse and fine sides of a hanging face
+    never exchange fluxes.  On the adaptive workload (dt = h_min / 8 / N /
+    (1 + sqrt(gamma)), as the benchmark runs it) the oracle's LSRK54 run is
+    unstable: finite after 32 steps, not after 33.  The library, stepping the
+    same grid and state in FP64, follows it step for step (relative L2 within
+    1e-9 while both are finite) into the same divergence."""
+    grid, q = c3
+    nq = grid.nq
+    dt = 0.125 * (10 / 16 / 4) / (nq - 1) / (1.0 + np.sqrt(1.4))
+    dg = R.DGSEM(law=R.EulerLaw(3), grid=grid, surface_numericalflux=R.RusanovFlux())
+    odg = O.DG(_ArrayGrid(grid), "euler")
+    qo = np.ascontiguousarray(q).reshape(-1).copy()
+    dqo = np.zeros_like(qo)
+    qd = to_dev(torch, dg, q)
+    qtmp = torch.empty_like(qd)
+    solver = R.LSRK54(dg, qd, dt)
+    diverged = None
+    worst = 0.0
+    for n in range(1, 41):
+        odg.lsrk_step(qo, dqo, solver.time, dt)
+        R.dostep_(qd, solver, qtmp)
+        got = host(qd)
+        fo, fg = bool(np.isfinite(qo).all()), bool(np.isfinite(got).all())
+        if fo and fg:
+            err = rel_l2(got, qo)
+            worst = max(worst, err)
+            assert err <= 1e-9, f"step {n}: rel L2 {err:.2e}"
+            continue
+        assert fo == fg, f"step {n}: oracle finite {fo}, library finite {fg}"
+        diverged = n
+        break
+    say(f"   self-mapped forest: both diverge at step {diverged}; worst rel L2 before "
+        f"{worst:.2e}")
+    assert diverged == 33
+
+
 def test_lsrk144_run_matches_oracle(torch, R):
     """solve!(q, T, LSRK144(dg, q, dt)) through rv_lsrk_run (14 stages, the
     last step shortened to land on T) against the oracle's solve, FP64."""
