@@ -68,7 +68,9 @@ struct Plane3 {
     static constexpr int PAD = pick_pad();
     static constexpr int PLANE = NS * NQ * PAD;      // one flux component of one plane
     static constexpr int PBUF = 2 * PLANE;           // f1 and f2 of one plane
-    static constexpr int FACEBUF = 4 * NS * NQ2;     // q- / lift of the four x,y faces
+    // q- / lift of the four x,y faces, then a spare face (lanes off the x or
+    // y faces store to it and subtract it, zeroed, without branches)
+    static constexpr int FACEBUF = 5 * NS * NQ2;
     static constexpr int SLAB0 = (2 * PBUF > FACEBUF) ? 2 * PBUF : FACEBUF;
     static constexpr int XFB = 2 * NS * NQ2;         // x-face neighbour states, read early
     static constexpr int SLAB = SLAB0 + XFB;
@@ -333,32 +335,27 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
         for (int f = 2; f < 6; ++f)
             if ((P.dir_mask >> (f >> 1)) & 1)
                 load_neighbour<NQ, NS, R>(P, qbase, rec[f], l, qp[f]);
-        if (dx_on) {
-            const R *xf = slab + T::SLAB0;
-#pragma unroll
-            for (int f = 0; f < 2; ++f)
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    qp[f][s] = xf[(f * NS + s) * NQ2 + l];
-        }
     }
 
     // ---- x and y faces: owners publish q-, every lane evaluates one node of
-    // each of the four faces, owners collect the lift
-    R *fb = slab;  // [face 0..3][s][face node]
+    // each of the four faces, owners collect the lift.  A lane's x slot is
+    // face 0 or 1 when it sits on one (i == 0, i == Nq-1) and the spare face
+    // otherwise, likewise its y slot: every lane stores and later subtracts
+    // both slots without branches, the spare face being zeroed before the
+    // lifts are subtracted
+    R *fb = slab;  // [face 0..3, spare][s][face node]
+    constexpr int SPARE = 4 * NS * NQ2;
+    const int xslot = (dx_on && i == 0) ? 0 : (dx_on && i == NQ - 1) ? NS * NQ2 : SPARE;
+    const int yslot = (dy_on && j == 0) ? 2 * NS * NQ2 : (dy_on && j == NQ - 1) ? 3 * NS * NQ2
+                                                                               : SPARE;
+    R *xs = fb + xslot + j, *ys = fb + yslot + i;
     if (active) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                if (i == 0)
-                    fb[(0 * NS + s) * NQ2 + j + NQ * k] = q[s][k];
-                if (i == NQ - 1)
-                    fb[(1 * NS + s) * NQ2 + j + NQ * k] = q[s][k];
-                if (j == 0)
-                    fb[(2 * NS + s) * NQ2 + i + NQ * k] = q[s][k];
-                if (j == NQ - 1)
-                    fb[(3 * NS + s) * NQ2 + i + NQ * k] = q[s][k];
+                xs[s * NQ2 + NQ * k] = q[s][k];
+                ys[s * NQ2 + NQ * k] = q[s][k];
             }
         }
         // z faces, in the lane's own column
@@ -395,6 +392,10 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     }
     wave_lds_sync();
     if (active) {
+        // the spare face reads as zero when the lifts are subtracted
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            fb[SPARE + s * NQ2 + l] = R(0);
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int d = f >> 1;
@@ -404,6 +405,12 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
 #pragma unroll
             for (int s = 0; s < NS; ++s)
                 qm[s] = fb[(f * NS + s) * NQ2 + l];
+            if (f < 2) {
+                // x-face neighbour states parked in LDS at the top
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    qp[f][s] = slab[T::SLAB0 + (f * NS + s) * NQ2 + l];
+            }
             if (rec[f].x == FACE_SELF) {
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
@@ -428,23 +435,10 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     }
     wave_lds_sync();
     if (active) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                R v = acc[s][k];
-                if (dx_on && i == 0)
-                    v = v - fb[(0 * NS + s) * NQ2 + j + NQ * k];
-                if (dx_on && i == NQ - 1)
-                    v = v - fb[(1 * NS + s) * NQ2 + j + NQ * k];
-                if (dy_on && j == 0)
-                    v = v - fb[(2 * NS + s) * NQ2 + i + NQ * k];
-                if (dy_on && j == NQ - 1)
-                    v = v - fb[(3 * NS + s) * NQ2 + i + NQ * k];
-                acc[s][k] = v;
-            }
-        }
-        // ---- update and store
+        // ---- lifts, update and store, one plane at a time (the scheduler
+        // may not hoist later planes' LDS reads past the barrier: with the
+        // column, the accumulators and the reads of every plane in flight
+        // at once a wave would run out of registers)
         R *dqo = static_cast<R *>(P.dq) + e * (NS * NP) + l;
         R *qo = static_cast<R *>(P.q_out) + e * (NS * NP) + l;
 #pragma unroll
@@ -454,13 +448,17 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 mji = R(P.mji[e * NP + l + k * NQ2]);
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
+                // v - 0 is v, so lanes off a face keep their sum bit for bit
                 R v = acc[s][k];
+                v = v - xs[s * NQ2 + NQ * k];
+                v = v - ys[s * NQ2 + NQ * k];
                 if constexpr (GEO == GEO_GENERAL)
                     v = mji * v;
                 store_out<ST>(v, dqo + s * NP + k * NQ2);
                 if (P.write_q)
                     store_out<ST>(R(q[s][k] + bdt * v), qo + s * NP + k * NQ2);
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
