@@ -581,9 +581,11 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
         rk.relax = 1;
         h->relax_launcher = find_stage_launcher(rk);
     }
-    // waves per workgroup measured on MI355X for the 3-D column kernel at
-    // Nq = 8: two for FP64 (256 VGPRs, 2 waves/SIMD), four for FP32
-    h->wpb = h->dim == 3 && h->dtype == RV_F32 ? 2 : 1;
+    // waves per workgroup measured on MI355X for the 3-D column kernel's
+    // fused stage on the 32^3 Nq = 8 Euler brick: four for FP64 (0.50 ms
+    // against 0.52 ms with one or two), two for FP32 (one, two and four
+    // within 2 %)
+    h->wpb = h->dim == 3 ? (h->dtype == RV_F32 ? 2 : 4) : 1;
     if (!h->launcher)
         return fail(h, RV_E_UNSUPPORTED,
                     "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +
@@ -2810,6 +2812,25 @@ int rv_batch_step(rv_batch *b, void *const *q0, void *const *q1, void *const *dq
     BHIP_TRY(b, hipEventRecord(b->ring_ev[slot], b->stream));
     b->ring = (slot + 1) % rv_batch::RING;
     return batch_launch(b, dev, nst, b->stream, &active);
+}
+
+int rv_batch_steps(rv_batch *b, void *const *q0, void *const *q1, void *const *dq,
+                   const int32_t *cur, const double *dt, int scheme, int64_t nsteps)
+{
+    if (!b || !cur || nsteps < 0)
+        return bfail(b, RV_E_ARG, "invalid argument");
+    const double *A, *B, *C;
+    const int nst = lsrk_tables(scheme, &A, &B, &C);
+    if (nst == 0)
+        return bfail(b, RV_E_ARG, "unknown LSRK scheme");
+    std::vector<int32_t> c(cur, cur + b->members.size());
+    for (int64_t k = 0; k < nsteps; ++k) {
+        if (int rc = rv_batch_step(b, q0, q1, dq, c.data(), dt, scheme))
+            return rc;
+        for (auto &x : c)
+            x = (x + nst) & 1;
+    }
+    return RV_OK;
 }
 
 int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, const double *dt,

@@ -73,8 +73,20 @@ struct Plane3 {
     static constexpr int FACEBUF = 5 * NS * NQ2;
     static constexpr int SLAB0 = (2 * PBUF > FACEBUF) ? 2 * PBUF : FACEBUF;
     static constexpr int XFB = 2 * NS * NQ2;         // x-face neighbour states, read early
-    static constexpr int SLAB = SLAB0 + XFB;
+    // FP64 general cells keep their x / y derivative weights in LDS (the
+    // column, the accumulators and two planes of metric terms fill the
+    // registers); the others in registers
+    static constexpr bool WLDS = GEO == GEO_GENERAL && sizeof(R) == 8;
+    static constexpr int CTAB = WLDS ? 2 * NQ2 : 0;
+    static constexpr int SLAB = SLAB0 + XFB + CTAB;
     static_assert(EPW >= 1, "Nq^2 must not exceed the wave size");
+    // Waves that share their cells: Nq^2 of 25, 36 or 49 columns leave 14 to
+    // 44 % of a wave's lanes idle when cells do not straddle waves, so a
+    // workgroup of G waves packs CPG cells across its 64 G lanes instead
+    // (125 of 128, 252 of 256, 245 of 256 lanes busy) and synchronises its
+    // LDS slabs with workgroup barriers.
+    static constexpr int G = (NQ == 5) ? 2 : (NQ == 6 || NQ == 7) ? 4 : 1;
+    static constexpr int CPG = 64 * G / NQ2;
     using S = Stage<3, NQ, LAW, R, GEO, FLUX>;
 };
 
@@ -101,8 +113,10 @@ __device__ inline void load_neighbour(const StageParams &P, const R *qbase, cons
 }
 
 // The cells [cell0, cell0 + EPW) of one wave (cell0 is the same on every
-// lane), working in the wave's own LDS slabs `wlds` (EPW * SLAB values).
-template <int NQ, int LAW, typename R, int GEO, int FLUX, int ST = 0>
+// lane), working in the wave's own LDS slabs `wlds` (EPW * SLAB values); with
+// GW > 1, the cells [cell0, cell0 + CPG) of a workgroup of GW waves (the
+// workgroup's first 64 GW threads) and their CPG slabs.
+template <int NQ, int LAW, typename R, int GEO, int FLUX, int ST = 0, int GW = 1>
 __device__ __attribute__((always_inline)) inline void plane3_wave(const StageParams &P, int64_t cell0, R *wlds)
 {
     using T = Plane3<NQ, LAW, R, GEO, FLUX>;
@@ -110,15 +124,24 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     constexpr int NS = T::NS, NQ2 = T::NQ2, NP = T::NP, PAD = T::PAD;
     constexpr int PLANE = T::PLANE, PBUF = T::PBUF;
 
+    constexpr int CELLS = GW == 1 ? T::EPW : 64 * GW / NQ2;
     const int lane = threadIdx.x & 63;
-    const int slot = lane / NQ2;
-    const int l = lane - slot * NQ2;  // column index = i + NQ*j
+    const int gl = GW == 1 ? lane : (int)threadIdx.x;  // position in the wave / workgroup
+    const int slot = gl / NQ2;
+    const int l = gl - slot * NQ2;  // column index = i + NQ*j
     const int i = l % NQ, j = l / NQ;
     int64_t e = cell0 + slot;
-    if constexpr (T::EPW == 1)
+    if constexpr (T::EPW == 1 && GW == 1)
         e = wave_uniform(e);  // one cell per wave: per-cell data comes in scalar loads
-    const bool active = (slot < T::EPW) && (e < P.K);
-    R *slab = wlds + (slot < T::EPW ? slot : 0) * T::SLAB;
+    const bool active = (slot < CELLS) && (e < P.K);
+    R *slab = wlds + (slot < CELLS ? slot : 0) * T::SLAB;
+    // slabs shared by the waves of a workgroup need its barrier
+    auto lds_sync = [] {
+        if constexpr (GW == 1)
+            wave_lds_sync();
+        else
+            __syncthreads();
+    };
     const Law<3, LAW, R> law(P);
     const R *qbase = static_cast<const R *>(P.q_in);
     const bool dx_on = P.dir_mask & 1, dy_on = (P.dir_mask >> 1) & 1,
@@ -126,7 +149,8 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
 
     R q[NS][NQ], acc[NS][NQ];
     R xq[2][NS];       // x-face neighbour states until they are parked in LDS
-    R cx[NQ], cy[NQ];  // this lane's x / y derivative weights
+    R cx[T::WLDS ? 1 : NQ], cy[T::WLDS ? 1 : NQ];  // this lane's x / y derivative weights
+    R *ctab = slab + T::SLAB0 + T::XFB;  // with WLDS: [output node][n], x rows then y rows
     R sz = R(1);
     FaceRec rec[6];
     const R a = R(P.a), bdt = R(P.b_dt);
@@ -143,11 +167,13 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
         // every load of the cell is issued before the first wait: derivative
         // weights, geometry factors, face records, then q and dq
         const double *M = (GEO == GEO_AFFINE) ? P.Dw : P.Dt;
-        double mi[NQ], mj[NQ];
+        double mi[T::WLDS ? 1 : NQ], mj[T::WLDS ? 1 : NQ];
+        if constexpr (!T::WLDS) {
 #pragma unroll
-        for (int n = 0; n < NQ; ++n) {
-            mi[n] = M[i * NQ + n];
-            mj[n] = M[j * NQ + n];
+            for (int n = 0; n < NQ; ++n) {
+                mi[n] = M[i * NQ + n];
+                mj[n] = M[j * NQ + n];
+            }
         }
         R sx = R(1), sy = R(1);
         if constexpr (GEO == GEO_AFFINE) {
@@ -197,10 +223,18 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             load_neighbour<NQ, NS, R>(P, qbase, rec[1], l, xq[1]);
         }
         const R fx = dx_on ? sx : R(0), fy = dy_on ? sy : R(0);
+        if constexpr (T::WLDS) {
+            // column l stores entry l of the operator (ztab is entry
+            // lane % Nq^2, another one where cells straddle waves)
+            const double ml = M[l];
+            ctab[l] = fx * R(ml);
+            ctab[NQ2 + l] = fy * R(ml);
+        } else {
 #pragma unroll
-        for (int n = 0; n < NQ; ++n) {
-            cx[n] = fx * R(mi[n]);
-            cy[n] = fy * R(mj[n]);
+            for (int n = 0; n < NQ; ++n) {
+                cx[n] = fx * R(mi[n]);
+                cy[n] = fy * R(mj[n]);
+            }
         }
     }
 
@@ -209,8 +243,9 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     struct alignas(16) Vec {
         R v[T::VEC];
     };
-    // general cells: wJ*dRdX of the next plane, requested one plane ahead
-    double gnext[3][3];
+    // general cells: wJ*dRdX of plane k in gbuf[k & 1], requested two planes
+    // ahead (one plane's work is shorter than a load's latency)
+    double gbuf[2][3][3];
     auto load_geo = [&](const int k) {
         if constexpr (GEO == GEO_GENERAL) {
             const int node = l + k * NQ2;
@@ -218,7 +253,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    gnext[d][c] = P.vgeo[((e * 3 + d) * 3 + c) * NP + node];
+                    gbuf[k & 1][d][c] = P.vgeo[((e * 3 + d) * 3 + c) * NP + node];
         }
     };
     auto plane_flux = [&](const int k) {
@@ -243,9 +278,9 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    g[d][c] = R(gnext[d][c]);
-            if (k + 1 < NQ)
-                load_geo(k + 1);
+                    g[d][c] = R(gbuf[k & 1][d][c]);
+            if (k + 2 < NQ)
+                load_geo(k + 2);
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 f1[s] = g[0][0] * Fa[0][s] + g[0][1] * Fa[1][s] + g[0][2] * Fa[2][s];
@@ -275,6 +310,49 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     };
     auto gather = [&](const int k) {
         const R *buf = slab + (k & 1) * PBUF;
+        if constexpr (T::WLDS) {
+            // weights from LDS, read once for the five states; per state the
+            // same order of terms as below (x then y terms of each VEC block)
+            const R *cxr = ctab + i * NQ, *cyr = ctab + NQ2 + j * NQ;
+            if constexpr (T::VECREAD) {
+#pragma unroll
+                for (int t = 0; t < NQ / T::VEC; ++t) {
+                    const Vec c1 = reinterpret_cast<const Vec *>(cxr)[t];
+                    const Vec c2 = reinterpret_cast<const Vec *>(cyr)[t];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const Vec x1 = reinterpret_cast<const Vec *>(buf + s * NQ * PAD + j * PAD)[t];
+                        const Vec x2 =
+                            reinterpret_cast<const Vec *>(buf + PLANE + s * NQ * PAD + i * PAD)[t];
+                        R v = acc[s][k];
+#pragma unroll
+                        for (int c = 0; c < T::VEC; ++c)
+                            v = v + c1.v[c] * x1.v[c];
+#pragma unroll
+                        for (int c = 0; c < T::VEC; ++c)
+                            v = v + c2.v[c] * x2.v[c];
+                        acc[s][k] = v;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < NQ; ++n) {
+                    const R c1 = cxr[n];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        acc[s][k] = acc[s][k] + c1 * buf[s * NQ * PAD + j * PAD + n];
+                }
+#pragma unroll
+                for (int n = 0; n < NQ; ++n) {
+                    const R c2 = cyr[n];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        acc[s][k] = acc[s][k] + c2 * buf[PLANE + s * NQ * PAD + i * PAD + n];
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const R *r1 = buf + s * NQ * PAD + j * PAD;
@@ -305,7 +383,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     };
     if (active) {
         load_geo(0);
+        // FP64 general cells request plane 1 once plane 0 is done (with the
+        // column, the register and the x-face states all in flight at the
+        // top, a second plane of metric terms would not fit)
+        if (NQ > 1 && !T::WLDS)
+            load_geo(1);
         plane_flux(0);
+        if (NQ > 1 && T::WLDS)
+            load_geo(1);
     }
     if (active && dx_on) {
         R *xf = slab + T::SLAB0;
@@ -315,7 +400,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int s = 0; s < NS; ++s)
                 xf[(f * NS + s) * NQ2 + l] = xq[f][s];
     }
-    wave_lds_sync();
+    lds_sync();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         if (active) {
@@ -323,7 +408,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 plane_flux(k + 1);
             gather(k);
         }
-        wave_lds_sync();
+        lds_sync();
     }
 
     // ---- neighbour states of the y and z faces, issued together: node l of
@@ -390,7 +475,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             }
         }
     }
-    wave_lds_sync();
+    lds_sync();
     if (active) {
         // the spare face reads as zero when the lifts are subtracted
 #pragma unroll
@@ -433,7 +518,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 fb[(f * NS + s) * NQ2 + l] = lift * nf[s];
         }
     }
-    wave_lds_sync();
+    lds_sync();
     if (active) {
         // ---- lifts, update and store, one plane at a time (the scheduler
         // may not hoist later planes' LDS reads past the barrier: with the
@@ -441,11 +526,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
         // at once a wave would run out of registers)
         R *dqo = static_cast<R *>(P.dq) + e * (NS * NP) + l;
         R *qo = static_cast<R *>(P.q_out) + e * (NS * NP) + l;
+        // general cells: 1 / (w J) of the whole column, requested at once
+        R mjiv[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+            mjiv[k] = GEO == GEO_GENERAL ? R(P.mji[e * NP + l + k * NQ2]) : R(1);
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            R mji = R(1);
-            if constexpr (GEO == GEO_GENERAL)
-                mji = R(P.mji[e * NP + l + k * NQ2]);
+            const R mji = mjiv[k];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 // v - 0 is v, so lanes off a face keep their sum bit for bit
@@ -469,11 +557,17 @@ dg_plane3_kernel(const StageParams P)
 {
     using T = Plane3<NQ, LAW, R, GEO, FLUX>;
     extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
-    const int wave = threadIdx.x >> 6;
-    const int wpb = blockDim.x >> 6;
     const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    R *wlds = reinterpret_cast<R *>(dg_lds) + wave * T::EPW * T::SLAB;
-    plane3_wave<NQ, LAW, R, GEO, FLUX, ST>(P, (blk * wpb + wave) * T::EPW, wlds);
+    if constexpr (T::G > 1) {
+        // one workgroup of G waves packs CPG cells
+        plane3_wave<NQ, LAW, R, GEO, FLUX, ST, T::G>(P, blk * T::CPG,
+                                                     reinterpret_cast<R *>(dg_lds));
+    } else {
+        const int wave = threadIdx.x >> 6;
+        const int wpb = blockDim.x >> 6;
+        R *wlds = reinterpret_cast<R *>(dg_lds) + wave * T::EPW * T::SLAB;
+        plane3_wave<NQ, LAW, R, GEO, FLUX, ST>(P, (blk * wpb + wave) * T::EPW, wlds);
+    }
 }
 
 template <int NQ, int LAW, typename R, int GEO, int FLUX>
@@ -484,9 +578,18 @@ int launch_plane3(const StageParams &P, void *stream, int wpb)
         return 0;
     if (wpb != 1 && wpb != 2 && wpb != 4)
         wpb = 1;
-    const int64_t waves = (P.K + T::EPW - 1) / T::EPW;
-    const int64_t blocks = (waves + wpb - 1) / wpb;
-    const size_t lds = (size_t)wpb * T::EPW * T::SLAB * sizeof(R);
+    int64_t blocks;
+    size_t lds;
+    if constexpr (T::G > 1) {
+        // packed cells: the workgroup is the G waves that share them
+        wpb = T::G;
+        blocks = (P.K + T::CPG - 1) / T::CPG;
+        lds = (size_t)T::CPG * T::SLAB * sizeof(R);
+    } else {
+        const int64_t waves = (P.K + T::EPW - 1) / T::EPW;
+        blocks = (waves + wpb - 1) / wpb;
+        lds = (size_t)wpb * T::EPW * T::SLAB * sizeof(R);
+    }
     if (P.streaming)
         hipLaunchKernelGGL((dg_plane3_kernel<NQ, LAW, R, GEO, FLUX, 1>), dim3((unsigned)blocks),
                            dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);

@@ -37,6 +37,17 @@
 
 namespace rv {
 
+// s + c += s2 + c2 with the rounding error of s + s2 (Knuth's TwoSum, exact
+// and independent of the operands' order) added to the compensation
+__device__ inline void comp_add(double &s, double &c, double s2, double c2)
+{
+    const double t = s + s2;
+    const double bb = t - s;
+    const double err = (s - (t - bb)) + (s2 - bb);
+    s = t;
+    c = (c + c2) + err;
+}
+
 namespace {
 
 template <int DIM, int NS, int LAW, int VF, typename R>
@@ -216,7 +227,7 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
     // (and the surface pass's face terms after it)
     constexpr int HS_BYTES = T::NPAIR * NS * T::HLP * (int)sizeof(R);
     constexpr int FS_BYTES = T::CPB * NF * NFP * NS * (int)sizeof(R);
-    constexpr int RED_BYTES = RX ? T::THREADS * (int)sizeof(double) : 0;
+    constexpr int RED_BYTES = RX ? 2 * T::THREADS * (int)sizeof(double) : 0;
     constexpr int POOL0 = HS_BYTES > FS_BYTES ? HS_BYTES : FS_BYTES;
     __shared__ __align__(16) char pool[POOL0 > RED_BYTES ? POOL0 : RED_BYTES];
     R *hs = reinterpret_cast<R *>(pool);
@@ -457,35 +468,47 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
                 acc[s] = a * dq[(e * NS + s) * NP + node] + acc[s];
         }
         double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
-        // the workgroup's sum: a fixed tree over the threads (in the pair
-        // buffer, once every thread has read its face terms)
+        double pc = 0.0;  // compensation of prod
+        // the workgroup's sum, compensated: a fixed tree over the threads of
+        // (sum, compensation) pairs, each addition's rounding error (TwoSum,
+        // exact and symmetric in its operands) carried in the compensation;
+        // in the pair buffer, once every thread has read its face terms
         double *red = reinterpret_cast<double *>(pool);
         __syncthreads();
         if constexpr (T::THREADS % 64 == 0) {
-            // whole waves: a butterfly inside each wave, then the wave sums
-            // in wave order
+            // whole waves: a butterfly inside each wave (both lanes of a pair
+            // form the same sum), then the wave sums in wave order
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-                prod += __shfl_xor(prod, off);
-            if ((t & 63) == 0)
-                red[t >> 6] = prod;
+            for (int off = 32; off > 0; off >>= 1) {
+                const double os = __shfl_xor(prod, off), oc = __shfl_xor(pc, off);
+                comp_add(prod, pc, os, oc);
+            }
+            if ((t & 63) == 0) {
+                red[2 * (t >> 6)] = prod;
+                red[2 * (t >> 6) + 1] = pc;
+            }
             __syncthreads();
             if (t == 0) {
-                double sum = red[0];
+                double sum = red[0], c = red[1];
                 for (int w = 1; w < T::THREADS / 64; ++w)
-                    sum += red[w];
-                P.eprod[blockIdx.x] = sum;
+                    comp_add(sum, c, red[2 * w], red[2 * w + 1]);
+                P.eprod[blockIdx.x] = sum + c;
             }
         } else {
-            red[t] = prod;
+            red[2 * t] = prod;
+            red[2 * t + 1] = pc;
             __syncthreads();
             for (int w = 256; w > 0; w >>= 1) {
-                if (t < w && t + w < T::THREADS)
-                    red[t] += red[t + w];
+                if (t < w && t + w < T::THREADS) {
+                    double sum = red[2 * t], c = red[2 * t + 1];
+                    comp_add(sum, c, red[2 * (t + w)], red[2 * (t + w) + 1]);
+                    red[2 * t] = sum;
+                    red[2 * t + 1] = c;
+                }
                 __syncthreads();
             }
             if (t == 0)
-                P.eprod[blockIdx.x] = red[0];
+                P.eprod[blockIdx.x] = red[0] + red[1];
         }
     }
     if (!active)
