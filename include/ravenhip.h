@@ -379,6 +379,12 @@ int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, voi
  * launches once (2-D) or once per kind of the members stepping (3-D). */
 int rv_batch_step(rv_batch *b, void *const *q0, void *const *q1, void *const *dq,
                   const int32_t *cur, const double *dt, int scheme);
+/* nsteps calls of rv_batch_step with the same dt[m], the parities advancing
+ * by the scheme's stage count after each step (cur itself is not written):
+ * a serving loop issues the steps before the next final, shortened step of
+ * any member in one call. */
+int rv_batch_steps(rv_batch *b, void *const *q0, void *const *q1, void *const *dq,
+                   const int32_t *cur, const double *dt, int scheme, int64_t nsteps);
 /* nsteps whole LSRK steps of every member with its own dt[m]: dq[m] is zeroed,
  * each step replays a captured hipGraph of the scheme's stages, and q[m] holds
  * the result (the library ping-pongs through buffers of its own).  The graphs

@@ -182,6 +182,9 @@ def lib():
                                           ctypes.POINTER(vp), c_f64p, c_f64p]),
         "rv_batch_step": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                          ctypes.POINTER(vp), c_i32p, c_f64p, ctypes.c_int]),
+        "rv_batch_steps": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.POINTER(vp), c_i32p, c_f64p, ctypes.c_int,
+                                          ctypes.c_int64]),
         "rv_batch_lsrk_run": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                              ctypes.c_int, c_f64p, ctypes.c_int64]),
     }

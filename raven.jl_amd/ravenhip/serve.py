@@ -4,7 +4,7 @@ A request is one brick solve (2-D or 3-D, advection or Euler, degree 3-7,
 FP32 or FP64) to a final time T.  The server keeps every admitted solve's
 state resident in device memory, admits waiting requests in arrival order as
 soon as their device footprint fits the budget, and advances all admitted
-solves of one dimension together: each LSRK54 step is one rv_batch_step call
+solves of one dimension together: LSRK54 steps are rv_batch_step(s) calls
 over the current mix, with every member's own dt and its final step shortened
 to land on its T exactly as ``solve!`` does (odesolvers.jl:9-30).  Requests
 that arrive while others run join at the next step; finished ones leave and
@@ -173,6 +173,20 @@ class _Active:
             return self.rnd(self.T - self.time), True
         return self.dt, False
 
+    def full_steps(self, limit):
+        """How many of the coming steps, at most limit, are whole steps of dt
+        before the final one (the clock arithmetic of next_step)."""
+        t, n = self.time, 0
+        while n < limit:
+            if self.f32:
+                tdt = float(np.float32(t) + np.float32(self.dt))
+            else:
+                tdt = t + self.dt
+            if tdt >= self.T:
+                break
+            t, n = tdt, n + 1
+        return n
+
     def bdt(self, s, step):
         if self.f32:
             return float(np.float32(self.b[s]) * np.float32(step))
@@ -202,10 +216,13 @@ class _Group:
         self.q1 = vp(*[m.bufs[1].data_ptr() for m in self.members])
         self.dq = vp(*[m.dq.data_ptr() for m in self.members])
 
-    def step(self, scheme, nstages):
-        """One LSRK step of every unfinished member: one C call, one launch
-        per stage.  Returns the members that finished with it."""
+    def step(self, scheme, nstages, max_steps=8):
+        """LSRK steps of every unfinished member in one C call: as many whole
+        steps (at most max_steps) as come before any member's final, shortened
+        step, else that one step.  Returns (steps taken, members that
+        finished with them)."""
         plan = []
+        run = max_steps
         for k, m in enumerate(self.members):
             if m.done:
                 self.dt[k] = 0.0
@@ -215,21 +232,31 @@ class _Group:
             self.dt[k] = st
             self.cur[k] = m.cur
             plan.append((st, last))
-        L.check_batch(L.lib().rv_batch_step(self.batch._b, self.q0, self.q1, self.dq,
-                                            L.ptr(self.cur, L.ctypes.c_int32),
-                                            L.ptr(self.dt, L.ctypes.c_double), scheme),
-                      self.batch._b)
+            run = min(run, 0 if last else m.full_steps(run))
+        if run >= 1:
+            # whole steps of every member's dt, no final step among them
+            L.check_batch(L.lib().rv_batch_steps(self.batch._b, self.q0, self.q1, self.dq,
+                                                 L.ptr(self.cur, L.ctypes.c_int32),
+                                                 L.ptr(self.dt, L.ctypes.c_double), scheme,
+                                                 run), self.batch._b)
+        else:
+            run = 1
+            L.check_batch(L.lib().rv_batch_step(self.batch._b, self.q0, self.q1, self.dq,
+                                                L.ptr(self.cur, L.ctypes.c_int32),
+                                                L.ptr(self.dt, L.ctypes.c_double), scheme),
+                          self.batch._b)
         finished = []
         for m, p in zip(self.members, plan):
             if p is None:
                 continue
             st, last = p
-            m.cur = (m.cur + nstages) % 2
-            m.advance_clock(st, last)
-            m.req.steps += 1
+            m.cur = (m.cur + run * nstages) % 2
+            for _ in range(run):
+                m.advance_clock(st, last)
+            m.req.steps += run
             if last:
                 finished.append(m)
-        return finished
+        return run, finished
 
 
 class Server:
@@ -243,8 +270,10 @@ class Server:
     states are prepared on the default stream, which does not wait for it, so
     admission overlaps the device's work on the solves already running.  Each
     pass of the loop admits arrived requests in arrival order while their
-    footprint fits (for at most admit_slice seconds), then issues one LSRK step
-    of every resident solve: one rv_batch_step call per dimension present.  The
+    footprint fits (for at most admit_slice seconds), then issues LSRK steps of
+    every resident solve: one call per dimension present, of as many whole
+    steps (at most max_steps) as come before any member's final, shortened
+    step, or that one step (one step per pass while requests wait).  The
     host never waits for the device between steps (the batch's parameter ring
     keeps it at most a few steps ahead); a solve's completion is seen through an
     event recorded after its last step, and its states go back to the
@@ -253,11 +282,12 @@ class Server:
     waits for the device.
     """
 
-    def __init__(self, hbm_budget, scheme=L.RV_LSRK54, admit_slice=0.004):
+    def __init__(self, hbm_budget, scheme=L.RV_LSRK54, admit_slice=0.004, max_steps=8):
         from .odesolvers import LSRK
         self.hbm_budget = int(hbm_budget)
         self.scheme = scheme
         self.admit_slice = float(admit_slice)
+        self.max_steps = int(max_steps)  # steps per call when no request waits
 
         class _NoDG:
             def GridArray(self, fill):
@@ -351,11 +381,15 @@ class Server:
             for dim in sorted(groups):
                 g = groups[dim]
                 if any(not m.done for m in g.members):
-                    for m in g.step(self.scheme, len(self.a)):
+                    # no admission waits on these steps when nothing is
+                    # waiting; else one step per pass
+                    nrun, done = g.step(self.scheme, len(self.a),
+                                        max_steps=1 if waiting else self.max_steps)
+                    for m in done:
                         ev = torch.cuda.Event()
                         ev.record(compute)
                         inflight.append((m, ev))
-                    self.steps_run += 1
+                    self.steps_run += nrun
                     issued = True
             stepping = [m for m in stepping if not m.done]
             for dim in [d for d, g in groups.items() if all(m.done for m in g.members)]:
