@@ -598,7 +598,7 @@ def main():
                     help="skip the adaptive warped degree-5 FP32/FP64 workload")
     ap.add_argument("--adaptive-steps", type=int, default=200,
                     help="LSRK54 steps of the adaptive workload (200: the configuration)")
-    ap.add_argument("--serve-rates", default="8,32,burst",
+    ap.add_argument("--serve-rates", default="8,32,64,burst",
                     help="request arrival rates (per second) of the served stream; "
                          "'burst': every request arrives at once")
     args = ap.parse_args()

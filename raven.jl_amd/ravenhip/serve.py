@@ -263,17 +263,21 @@ class Server:
     """Continuous-batching server over one device.
 
     hbm_budget  -- device bytes the resident solves may hold together
-    admit_slice -- host seconds one pass of the loop may spend admitting
-                   (at least one request is admitted when none is resident)
+    admit_slice -- host seconds one pass of the loop may wait for set-ups
+                   under way before it steps (0: solves join as soon as
+                   their set-up is done)
+    max_steps   -- most LSRK steps issued in one call
 
-    Solves step on a stream of the server's own; operator set-up and initial
-    states are prepared on the default stream, which does not wait for it, so
-    admission overlaps the device's work on the solves already running.  Each
-    pass of the loop admits arrived requests in arrival order while their
-    footprint fits (for at most admit_slice seconds), then issues LSRK steps of
-    every resident solve: one call per dimension present, of as many whole
-    steps (at most max_steps) as come before any member's final, shortened
-    step, or that one step (one step per pass while requests wait).  The
+    Solves step on a stream of the server's own.  Each pass of the loop
+    admits arrived requests in arrival order while their footprint fits
+    (reserving it), and a worker thread builds their operators and initial
+    states on the default stream, which the solves' stream does not wait for,
+    so set-up overlaps the device's work and the loop's step issue.  Admitted
+    solves join the mix in order as their set-up completes.  The loop then
+    issues LSRK steps of every resident solve: one call per dimension present,
+    of as many whole steps (at most max_steps) as come before any member's
+    final, shortened step, or that one step (one step per pass while requests
+    wait or are being set up).  The
     host never waits for the device between steps (the batch's parameter ring
     keeps it at most a few steps ahead); a solve's completion is seen through an
     event recorded after its last step, and its states go back to the
@@ -282,7 +286,7 @@ class Server:
     waits for the device.
     """
 
-    def __init__(self, hbm_budget, scheme=L.RV_LSRK54, admit_slice=0.004, max_steps=8):
+    def __init__(self, hbm_budget, scheme=L.RV_LSRK54, admit_slice=0.0, max_steps=8):
         from .odesolvers import LSRK
         self.hbm_budget = int(hbm_budget)
         self.scheme = scheme
@@ -305,12 +309,23 @@ class Server:
         """Serve the requests (arrival times relative to the call); returns
         the requests with admitted / finished times filled in.  stream: the
         stream the solves step on (default: a new stream)."""
+        import concurrent.futures as cf
+
         import torch
         compute = stream if stream is not None else torch.cuda.Stream()
         setup = torch.cuda.current_stream()
         handle = L.ctypes.c_void_p(compute.cuda_stream)
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting = []
+        # admitted solves whose operator and initial state a worker thread is
+        # building (arrival order): set-up overlaps the device's work and the
+        # loop's step issue instead of holding the loop
+        preparing = []
+        pool = cf.ThreadPoolExecutor(max_workers=1, thread_name_prefix="rv-admit")
+
+        def prepare(r):
+            with torch.cuda.stream(setup):
+                return _Active(r, self.a, self.b, handle)
         groups = {}               # dim -> _Group of the solves still stepping
         stepping = []             # admitted solves whose last step is not issued
         inflight = []             # (solve, event after its last step)
@@ -329,7 +344,7 @@ class Server:
                     obj.dg.close()
             graveyard.clear()
 
-        while k < len(pending) or waiting or stepping or inflight:
+        while k < len(pending) or waiting or preparing or stepping or inflight:
             now = time.perf_counter() - t0
             while k < len(pending) and pending[k].arrival <= now:
                 waiting.append(pending[k])
@@ -346,23 +361,26 @@ class Server:
                 else:
                     still.append((m, ev))
             inflight = still
-            # admission in arrival order while the footprint fits
+            # admission in arrival order while the footprint fits: the
+            # memory is reserved and the set-up handed to the worker
             ta = time.perf_counter()
-            admitted = []
             while waiting and (used + waiting[0].footprint() <= self.hbm_budget
-                               or not (stepping or inflight)):
-                if admitted and time.perf_counter() - ta > self.admit_slice:
-                    break
-                if not (stepping or inflight) and graveyard:
+                               or not (stepping or inflight or preparing)):
+                if not (stepping or inflight or preparing) and graveyard:
                     bury()
                 r = waiting.pop(0)
                 r.admitted = time.perf_counter() - t0
                 used += r.footprint()
-                with torch.cuda.stream(setup):
-                    m = _Active(r, self.a, self.b, handle)
+                preparing.append((r, pool.submit(prepare, r)))
+                self.peak_bytes = max(self.peak_bytes, used)
+            # the solves whose set-up is done join, in order (waiting for the
+            # first only when nothing else could run)
+            admitted = []
+            while preparing and (preparing[0][1].done() or not (stepping or inflight)
+                                 or time.perf_counter() - ta < self.admit_slice):
+                m = preparing.pop(0)[1].result()
                 stepping.append(m)
                 admitted.append(m)
-                self.peak_bytes = max(self.peak_bytes, used)
                 self.peak_active = max(self.peak_active, len(stepping) + len(inflight))
             if admitted:
                 # the first steps wait for the initial states
@@ -384,7 +402,8 @@ class Server:
                     # no admission waits on these steps when nothing is
                     # waiting; else one step per pass
                     nrun, done = g.step(self.scheme, len(self.a),
-                                        max_steps=1 if waiting else self.max_steps)
+                                        max_steps=1 if waiting or preparing
+                                        else self.max_steps)
                     for m in done:
                         ev = torch.cuda.Event()
                         ev.record(compute)
@@ -398,7 +417,11 @@ class Server:
             clock["step"] += tc - tb
             if issued:
                 continue
-            if inflight:
+            if preparing:
+                # nothing to issue until a set-up is done
+                cf.wait([preparing[0][1]], timeout=0.002)
+                clock["wait"] += time.perf_counter() - tc
+            elif inflight:
                 # nothing to issue: wait for the earliest completion, or
                 # until the next arrival could be admitted
                 inflight[0][1].synchronize()
@@ -408,6 +431,7 @@ class Server:
                 time.sleep(max(0.0, min(pending[k].arrival - (time.perf_counter() - t0), 0.01)))
                 clock["idle"] += time.perf_counter() - tc
         compute.synchronize()
+        pool.shutdown(wait=True)
         bury()
         return requests
 
