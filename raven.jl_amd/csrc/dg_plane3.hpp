@@ -76,7 +76,7 @@ struct Plane3 {
     // FP64 general cells keep their x / y derivative weights in LDS (the
     // column, the accumulators and two planes of metric terms fill the
     // registers); the others in registers
-    static constexpr bool WLDS = GEO == GEO_GENERAL && sizeof(R) == 8;
+    static constexpr bool WLDS = GEO == GEO_GENERAL;
     static constexpr int CTAB = WLDS ? 2 * NQ2 : 0;
     static constexpr int SLAB = SLAB0 + XFB + CTAB;
     static_assert(EPW >= 1, "Nq^2 must not exceed the wave size");
@@ -400,27 +400,41 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int s = 0; s < NS; ++s)
                 xf[(f * NS + s) * NQ2 + l] = xq[f][s];
     }
+    // ---- neighbour states of the y and z faces, issued together: node l of
+    // each y face (one node per lane) and the ends of the lane's own column
+    // for the two z faces (the x faces come from LDS).  General cells request
+    // them before the last plane's gather, in the registers the metric terms
+    // no longer need, together with the z faces' normals and weights; affine
+    // cells after the plane loop
+    R qp[6][NS];
+    double zsg[2][4];  // general cells: n and wsJ of this lane's z-face nodes
+    auto face_loads = [&] {
+#pragma unroll
+        for (int f = 2; f < 6; ++f)
+            if ((P.dir_mask >> (f >> 1)) & 1)
+                load_neighbour<NQ, NS, R>(P, qbase, rec[f], l, qp[f]);
+        if constexpr (GEO == GEO_GENERAL) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    zsg[side][c] = P.sgeo[((e * 6 + 4 + side) * 4 + c) * NQ2 + l];
+        }
+    };
     lds_sync();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         if (active) {
             if (k + 1 < NQ)
                 plane_flux(k + 1);
+            if (GEO == GEO_GENERAL && k == NQ - 1)
+                face_loads();
             gather(k);
         }
         lds_sync();
     }
-
-    // ---- neighbour states of the y and z faces, issued together: node l of
-    // each y face (one node per lane) and the ends of the lane's own column
-    // for the two z faces; the x faces come from LDS
-    R qp[6][NS];
-    if (active) {
-#pragma unroll
-        for (int f = 2; f < 6; ++f)
-            if ((P.dir_mask >> (f >> 1)) & 1)
-                load_neighbour<NQ, NS, R>(P, qbase, rec[f], l, qp[f]);
-    }
+    if (GEO == GEO_AFFINE && active)
+        face_loads();
 
     // ---- x and y faces: owners publish q-, every lane evaluates one node of
     // each of the four faces, owners collect the lift.  A lane's x slot is
@@ -434,7 +448,19 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     const int yslot = (dy_on && j == 0) ? 2 * NS * NQ2 : (dy_on && j == NQ - 1) ? 3 * NS * NQ2
                                                                                : SPARE;
     R *xs = fb + xslot + j, *ys = fb + yslot + i;
+    // FP32 general cells: n and wsJ of this lane's node on each x / y face,
+    // requested before the z faces are evaluated (FP64 has no registers to
+    // spare for them and reads them where the faces are evaluated)
+    constexpr bool XYPRE = GEO == GEO_GENERAL && sizeof(R) == 4;
+    double xysg[XYPRE ? 4 : 1][4];
     if (active) {
+        if constexpr (XYPRE) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    xysg[f][c] = P.sgeo[((e * 6 + f) * 4 + c) * NQ2 + l];
+        }
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
 #pragma unroll
@@ -464,9 +490,8 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                     S::nflux_axis(law, 2, side ? R(1) : R(-1), fm, qm, qp[4 + side], nf);
                     lift = sz * R(P.winv[kend]);
                 } else {
-                    const double *sg = P.sgeo + ((e * 6 + 4 + side) * 4) * NQ2 + l;
-                    R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
-                    lift = R(sg[3 * NQ2]);
+                    R nv[3] = {R(zsg[side][0]), R(zsg[side][1]), R(zsg[side][2])};
+                    lift = R(zsg[side][3]);
                     S::nflux_general(law, nv, qm, qp[4 + side], nf);
                 }
 #pragma unroll
@@ -507,6 +532,10 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 law.flux_row(d, qm, fm);
                 S::nflux_axis(law, d, (f & 1) ? R(1) : R(-1), fm, qm, qp[f], nf);
                 lift = R(P.affine[e * 3 + d]) * R(P.winv[(f & 1) ? NQ - 1 : 0]);
+            } else if constexpr (XYPRE) {
+                R nv[3] = {R(xysg[f][0]), R(xysg[f][1]), R(xysg[f][2])};
+                lift = R(xysg[f][3]);
+                S::nflux_general(law, nv, qm, qp[f], nf);
             } else {
                 const double *sg = P.sgeo + ((e * 6 + f) * 4) * NQ2 + l;
                 R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};

@@ -95,83 +95,140 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
             __syncthreads();  // t1 is reused by the next interpolation
         }
     };
-    // sub-face flux as the fine cell sees it: q- = fine state, q+ = the
-    // interpolated coarse trace, the fine face's normal and weight
-    auto subflux = [&](int64_t fine, int ffine, const R *qi, R *wnf) {
-        R qf[NS], nf[NS], n[DIM];
+    // what a sub-face flux reads of the fine side, as the fine cell sees it:
+    // q- = the fine state at its face node, the fine face's normal and weight
+    struct Fine {
+        R q[NS], n[DIM], wsJ;
+    };
+    auto load_fine = [&](int64_t fine, int ffine, Fine &F) {
         const int node = face_node<DIM, NQ>(ffine, m);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-            qf[s] = qin[fine * NS * NP + s * NP + node];
+            F.q[s] = qin[fine * NS * NP + s * NP + node];
         const double *sg = M.sgeo + ((fine * NF + ffine) * (DIM + 1)) * NFP + m;
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-            n[c] = R(sg[c * NFP]);
-        const R wsJ = R(sg[DIM * NFP]);
-        R qp[NS];
+            F.n[c] = R(sg[c * NFP]);
+        F.wsJ = R(sg[DIM * NFP]);
+    };
+    // the sub-face flux with q+ = the interpolated coarse trace qi
+    auto subflux = [&](const Fine &F, const R *qi, R *wnf) {
+        R qf[NS], nf[NS], n[DIM], qp[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s)
+        for (int s = 0; s < NS; ++s) {
+            qf[s] = F.q[s];
             qp[s] = qi[s];
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+            n[c] = F.n[c];
         S::nflux_general(law, n, qf, qp, nf);
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-            wnf[s] = wsJ * nf[s];
+            wnf[s] = F.wsJ * nf[s];
     };
-    auto apply = [&](int node, const R *delta) {
+    // dq += delta and, in stage mode, q_out += b_dt delta at this cell's node,
+    // from the values read with the face's other inputs
+    auto load_old = [&](int node, R *dq0, R *qo0) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int64_t idx = e * NS * NP + s * NP + node;
-            dq[idx] = dq[idx] + delta[s];
+            dq0[s] = dq[idx];
+            qo0[s] = P.write_q ? qo[idx] : R(0);
+        }
+    };
+    auto apply = [&](int node, const R *delta, const R *dq0, const R *qo0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int64_t idx = e * NS * NP + s * NP + node;
+            dq[idx] = dq0[s] + delta[s];
             if (P.write_q)
-                qo[idx] = qo[idx] + R(P.b_dt) * delta[s];
+                qo[idx] = qo0[s] + R(P.b_dt) * delta[s];
         }
     };
 
+    // faces in order: each face issues all of its loads (trace, fine states,
+    // geometry and the values it updates) before its first wait, and its
+    // updates land before the next face reads (edge nodes are shared),
+    // behind the workgroup barrier; the next face's kind and partners
+    // (scalar: the cell is the workgroup's) are requested a face ahead
+    int kind_next = M.kind[e * NF];
+    int64_t part_next[NSUB];
+#pragma unroll
+    for (int t = 0; t < NSUB; ++t)
+        part_next[t] = M.partner[4 * (e * NF) + t];
     for (int f = 0; f < NF; ++f) {
-        const int kind = M.kind[e * NF + f];
+        const int kind = kind_next;
+        int64_t part_f[NSUB];
+#pragma unroll
+        for (int t = 0; t < NSUB; ++t)
+            part_f[t] = part_next[t];
+        if (f + 1 < NF) {
+            kind_next = M.kind[e * NF + f + 1];
+#pragma unroll
+            for (int t = 0; t < NSUB; ++t)
+                part_next[t] = M.partner[4 * (e * NF + f + 1) + t];
+        }
         if (kind == 0)
             continue;
-        const int64_t *pt = M.partner + 4 * (e * NF + f);
+        const int node = face_node<DIM, NQ>(f, m);
+        R mji = R(0), dq0[NS], qo0[NS];
         if (kind == 1) {
             // this cell is the fine side: the coarse trace on its face f^1
-            const int64_t c = pt[0];
-            const int sub = (int)pt[1];
+            const int64_t c = part_f[0];
+            const int sub = (int)part_f[1];
+            Fine F;
             if (on) {
                 const int cn = face_node<DIM, NQ>(f ^ 1, m);
+                R trv[NS];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    tr[s][m] = qin[c * NS * NP + s * NP + cn];
+                    trv[s] = qin[c * NS * NP + s * NP + cn];
+                load_fine(e, f, F);
+                mji = R(M.stage.mji[e * NP + node]);
+                load_old(node, dq0, qo0);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    tr[s][m] = trv[s];
             }
             __syncthreads();
             R qi[NS];
             interp(sub, qi);
             if (on) {
                 R wnf[NS], delta[NS];
-                subflux(e, f, qi, wnf);
-                const int node = face_node<DIM, NQ>(f, m);
-                const R mji = R(M.stage.mji[e * NP + node]);
+                subflux(F, qi, wnf);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     delta[s] = -(mji * wnf[s]);
-                apply(node, delta);
+                apply(node, delta, dq0, qo0);
             }
             __syncthreads();
         } else {
             // this cell is the coarse side: its own trace, the fine fluxes of
             // every sub-face, then their projection onto its face nodes
-            const int node = face_node<DIM, NQ>(f, m);
+            Fine F[NSUB];
             if (on) {
+                R trv[NS];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    tr[s][m] = qin[e * NS * NP + s * NP + node];
+                    trv[s] = qin[e * NS * NP + s * NP + node];
+#pragma unroll
+                for (int sub = 0; sub < NSUB; ++sub)
+                    load_fine(part_f[sub], f ^ 1, F[sub]);
+                mji = R(M.stage.mji[e * NP + node]);
+                load_old(node, dq0, qo0);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    tr[s][m] = trv[s];
             }
             __syncthreads();
+#pragma unroll
             for (int sub = 0; sub < NSUB; ++sub) {
                 R qi[NS];
                 interp(sub, qi);
                 if (on) {
                     R wnf[NS];
-                    subflux(pt[sub], f ^ 1, qi, wnf);
+                    subflux(F[sub], qi, wnf);
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
                         g[sub][s][m] = wnf[s];
@@ -215,9 +272,9 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
                                 v = v + H[s2][bb][j] * u[sub][s][i + NQ * bb];
                         }
                     }
-                    delta[s] = R(M.stage.mji[e * NP + node]) * v;
+                    delta[s] = mji * v;
                 }
-                apply(node, delta);
+                apply(node, delta, dq0, qo0);
             }
             __syncthreads();
         }
