@@ -108,13 +108,17 @@ struct TwoPoint {
             uavg[d] = (P1.u[d] + P2.u[d]) / R(2);
         const R bavg = (P1.b + P2.b) / R(2);
         const R u2avg = (P1.uu + P2.uu) / R(2);
-        const R rlog = logavg(q1[0], q2[0]);
-        const R blog = logavg(P1.b, P2.b);
+        // the logarithmic means as (a + b) / 2F: rho's taken whole, b's only
+        // through 1 / (2 gm1 blog) = F_b / (2 bavg gm1), which shares
+        // 1 / (2 bavg) with the pressure average
+        const R rlog = ravg / logavg_factor(q1[0], q2[0]);
+        const R Fb = logavg_factor(P1.b, P2.b);
+        const R inv2b = R(1) / (R(2) * bavg);
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
             fr[d] = uavg[d] * rlog;
-        const R pav = ravg / (R(2) * bavg);
-        const R c = R(1) / (R(2) * law.gm1 * blog) - u2avg / R(2);
+        const R pav = ravg * inv2b;
+        const R c = Fb * inv2b * (R(1) / law.gm1) - u2avg / R(2);
         for (int r = 0; r < nrows; ++r) {
             const int d = row0 + r;
             F[r][0] = fr[d];
@@ -129,7 +133,7 @@ struct TwoPoint {
         }
     }
 
-    // logavg (numericalfluxes.jl:9-22): the logarithmic mean (a + b) / (2F),
+    // logavg (numericalfluxes.jl:9-22): the logarithmic mean is (a + b) / (2F),
     // F = log(zeta) / (2f) = atanh(f) / f with zeta = a / b and
     // f = (zeta - 1) / (zeta + 1) = (a - b) / (a + b).  Raven takes the series
     // of F in u = f^2 below the type's epsilon and the logarithm above; here
@@ -138,8 +142,8 @@ struct TwoPoint {
     // f comes from a - b: as accurate as Raven's form (within 2.2 eps of the
     // exact mean on 20,000 random pairs, against 2.4 eps), and neighbouring
     // nodes of a resolved state never reach the logarithm, two divisions
-    // and a log fewer per mean
-    __device__ static inline R logavg(R a, R b)
+    // and a log fewer per mean; returns F
+    __device__ static inline R logavg_factor(R a, R b)
     {
         const R f = (a - b) / (a + b);
         const R u = f * f;
@@ -159,7 +163,7 @@ struct TwoPoint {
         } else {
             F = log(a / b) / (R(2) * f);
         }
-        return (a + b) / (R(2) * F);
+        return F;
     }
 };
 
