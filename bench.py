@@ -278,8 +278,8 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     dqs = [torch.empty_like(q) for q in q0]
     # launch layout chosen by timing (each kind's waves per block, the mixed
     # launch's waves per block and the own-launch threshold)
-    tuned = batch.autotune(qs, dts)
-    layout = min(tuned, key=tuned.get)
+    tuned = batch.autotune(qs, dts)  # {layout: (median, min, max) ms per step}
+    layout = min(tuned, key=lambda k: tuned[k][0])
     batch.lsrk_run(qs, dqs, dts, steps)  # builds and caches the step graphs
     for q, q_init in zip(qs, q0):
         q.copy_(q_init)
@@ -341,8 +341,12 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
                   "persistent work-list launch for the small kinds; one hipGraph per step",
         "autotuned_layout": {"mixed_waves_per_block": layout[0],
                              "own_launch_items": layout[1],
-                             "step_ms_by_layout": {f"{w},{t}": round(v, 4)
-                                                    for (w, t), v in tuned.items()}},
+                             "step_ms_by_layout": {f"{w},{t}": round(v[0], 4)
+                                                    for (w, t), v in tuned.items()},
+                             "step_ms_spread_by_layout": {
+                                 f"{w},{t}": [round(v[1], 4), round(v[2], 4)]
+                                 for (w, t), v in tuned.items()},
+                             "timed_runs_per_layout": 3},
     }
 
 

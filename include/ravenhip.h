@@ -162,7 +162,10 @@ int rv_state_download(rv_handle *h, void *host, const void *state);
 
 /* Launch shape of the stage kernels: waves per workgroup (1, 2 or 4; each wave
  * keeps its own cells and LDS slab, so results do not depend on it).  The
- * library picks a default per configuration; an autotuner that times the
+ * library picks a default per configuration, for fused stages and for
+ * right-hand sides alone; setting it sets both, and get returns the stage
+ * value.  3-D cells of Nq 5 to 7, which the kernel packs across the waves of
+ * a workgroup, keep their own workgroup.  An autotuner that times the
  * choices on the caller's grid may override it.  No counterpart in Raven
  * (KernelAbstractions picks its workgroup size at launch). */
 int rv_set_waves_per_block(rv_handle *h, int waves_per_block);

@@ -70,7 +70,8 @@ struct rv_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    int wpb = 1;
+    int wpb = 1;      // waves per workgroup of a fused stage
+    int wpb_rhs = 1;  // ... of a right-hand side alone (no q_out written)
     StageLauncher launcher = nullptr;
     StageLauncher relax_launcher = nullptr;  // fused relaxation stage, when built
     // device buffers
@@ -581,11 +582,13 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
         rk.relax = 1;
         h->relax_launcher = find_stage_launcher(rk);
     }
-    // waves per workgroup measured on MI355X for the 3-D column kernel's
-    // fused stage on the 32^3 Nq = 8 Euler brick: four for FP64 (0.50 ms
-    // against 0.52 ms with one or two), two for FP32 (one, two and four
-    // within 2 %)
+    // waves per workgroup measured on MI355X for the 3-D column kernel on the
+    // 32^3 Nq = 8 Euler brick: a fused stage is fastest with four for FP64
+    // (0.50 ms against 0.52 ms with one or two) and two for FP32 (one, two
+    // and four within 2 %); a right-hand side alone with one for FP64
+    // (0.36 ms against 0.37 and 0.40 ms) and two for FP32
     h->wpb = h->dim == 3 ? (h->dtype == RV_F32 ? 2 : 4) : 1;
+    h->wpb_rhs = h->dim == 3 && h->dtype == RV_F32 ? 2 : 1;
     if (!h->launcher)
         return fail(h, RV_E_UNSUPPORTED,
                     "no kernel for dim=" + std::to_string(h->dim) + " nq=" + std::to_string(nq) +
@@ -838,7 +841,7 @@ StageParams stage_params(rv_handle *h)
 
 int launch(rv_handle *h, const StageParams &P, StageLauncher fn = nullptr)
 {
-    if ((fn ? fn : h->launcher)(P, h->stream, h->wpb) != 0) {
+    if ((fn ? fn : h->launcher)(P, h->stream, P.write_q ? h->wpb : h->wpb_rhs) != 0) {
         hipError_t e = hipGetLastError();
         return fail(h, RV_E_HIP, std::string("stage launch failed: ") + hipGetErrorString(e));
     }
@@ -1244,6 +1247,7 @@ int rv_set_waves_per_block(rv_handle *h, int waves_per_block)
     if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
         return fail(h, RV_E_ARG, "waves per block must be 1, 2 or 4");
     h->wpb = waves_per_block;
+    h->wpb_rhs = waves_per_block;
     return RV_OK;
 }
 

@@ -104,7 +104,7 @@ class Batch:
                                                  int(own_launch_items)), self._b)
 
     def autotune(self, q, dt, mixed_waves=(1, 2, 4), own_items=(512, 1024, 4096, 1 << 40),
-                 member_waves=(1, 2), steps=2, scheme=L.RV_LSRK54):
+                 member_waves=(1, 2), steps=2, scheme=L.RV_LSRK54, repeats=3):
         """Pick the launch layout by timing what a run does: whole LSRK steps
         replayed as hipGraphs (lsrk_run), on scratch copies of the states q
         with step sizes dt (q is not modified).
@@ -112,9 +112,11 @@ class Batch:
         First each kind's waves per block (the single operator's launch shape,
         timed on the kind's largest member with DGSEM.autotune and applied to
         every member of the kind), then the mixed launch's waves per block and
-        the own-launch threshold: for each layout, one run of `steps` steps
-        builds its graphs and a second is timed.  Returns {(mixed_waves,
-        own_items): ms per step}."""
+        the own-launch threshold: `repeats` rounds over the layouts in turn
+        (so a drift of the clock touches them alike), each setting the layout,
+        building its graphs with one run of `steps` steps and timing a second;
+        the layout with the lowest median is kept.  Returns {(mixed_waves, own_items): (median,
+        min, max) ms per step}."""
         import torch
         by_kind = {}
         for k, dg in enumerate(self.dgs):
@@ -129,17 +131,21 @@ class Batch:
                     dg.waves_per_block = w
         qs = [t.clone() for t in q]
         dqs = [torch.empty_like(t) for t in q]
-        times = {}
-        for w in mixed_waves:
-            for t in own_items:
-                self.configure(w, t)
-                self.lsrk_run(qs, dqs, dt, steps, scheme)  # captures the step graphs
+        layouts = [(w, t) for w in mixed_waves for t in own_items]
+        runs = {k: [] for k in layouts}
+        for _ in range(repeats):
+            for k in layouts:
+                # a new layout drops the captured graphs: one run captures
+                # them, the next is timed
+                self.configure(*k)
+                self.lsrk_run(qs, dqs, dt, steps, scheme)
                 ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
                 ev[0].record()
                 self.lsrk_run(qs, dqs, dt, steps, scheme)
                 ev[1].record()
                 ev[1].synchronize()
-                times[(w, t)] = ev[0].elapsed_time(ev[1]) / steps
-        best = min(times, key=times.get)
+                runs[k].append(ev[0].elapsed_time(ev[1]) / steps)
+        times = {k: (float(np.median(v)), min(v), max(v)) for k, v in runs.items()}
+        best = min(times, key=lambda k: times[k][0])
         self.configure(*best)
         return times
