@@ -824,6 +824,21 @@ void orc_dg_set_surface_flux(orc_dg *dg, int flux) { dg->surface_flux = flux; }
 void orc_dg_set_directions(orc_dg *dg, int mask) { dg->dir_mask = mask; }
 void orc_dg_set_threads(orc_dg *dg, int nthreads) { dg->nthreads = nthreads; }
 
+#ifdef _OPENMP
+/* threads of a parallel loop over the operator: the caller's choice, else
+ * about one per 32768 state values up to the available ones (small grids run
+ * thousands of steps, where thread start-up would dominate); every loop is
+ * over independent cells or values, so the results do not depend on it */
+static int dg_threads(const orc_dg *dg)
+{
+    if (dg->nthreads > 0)
+        return dg->nthreads;
+    const int64_t n = 1 + dg->K * dg->Np * dg->ns / 32768;
+    const int mx = omp_get_max_threads();
+    return n < mx ? (int)n : mx;
+}
+#endif
+
 /* flux(law, q) as a dim x Ns matrix F[dd + dim*s]
  * Euler: euler.jl:50-61; advection: advection.jl:18 */
 static void law_flux(const orc_dg *dg, const double *q, double *F)
@@ -1049,7 +1064,7 @@ void orc_dg_rhs(orc_dg *dg, double *dq, const double *q, double t, int increment
     const int64_t nfpt = nfaces * nfp;
     const double *D = dg->D;
 #ifdef _OPENMP
-    int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
+    int nth = dg_threads(dg);
 #endif
 #ifndef _OPENMP
     int nth = 1;
@@ -1264,7 +1279,7 @@ void orc_dg_lsrk_step(orc_dg *dg, int scheme, double *q, double *dq, double time
     int nst = orc_lsrk_coefficients(scheme, a, b, c);
     const int64_t n = dg->Np * dg->ns * dg->K;
 #ifdef _OPENMP
-    int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
+    int nth = dg_threads(dg);
 #endif
     for (int s = 0; s < nst; ++s) {
         double stagetime = time + c[s] * dt;
@@ -1353,7 +1368,7 @@ double orc_dg_entropy_integral(const orc_dg *dg, const double *q)
     const int64_t Np = dg->Np;
     double sum = 0;
 #ifdef _OPENMP
-    const int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
+    const int nth = dg_threads(dg);
 #endif
 #pragma omp parallel for schedule(static) reduction(+ : sum) num_threads(nth)
     for (int64_t e = 0; e < dg->K; ++e)
@@ -1373,7 +1388,7 @@ double orc_dg_entropy_product(const orc_dg *dg, const double *p, const double *q
     const int64_t Np = dg->Np;
     double sum = 0;
 #ifdef _OPENMP
-    const int nth = dg->nthreads > 0 ? dg->nthreads : omp_get_max_threads();
+    const int nth = dg_threads(dg);
 #endif
 #pragma omp parallel for schedule(static) reduction(+ : sum) num_threads(nth)
     for (int64_t e = 0; e < dg->K; ++e)
