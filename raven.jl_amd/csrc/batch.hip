@@ -11,13 +11,16 @@
 // (stage_wave / plane3_wave), with that member's StageParams read from a
 // device table.  Members are restricted to affine geometry and the Rusanov
 // flux (the brick solves batched and served here); 2-D members take Nq 2..8,
-// 3-D members Nq 4..8.
+// 3-D members Nq 4..8.  2-D items run the node kernel's group routine
+// (quad2_group) with one wave per group: the same arithmetic as the single
+// operator's packed groups, so results are bit-identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "common.hpp"
 #include "dg_plane3.hpp"
+#include "dg_quad2.hpp"
 
 namespace rv {
 
@@ -39,8 +42,7 @@ constexpr int kind_slab_elems()
         using T = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
         return T::EPW * T::SLAB;
     } else {
-        using S = Stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
-        return S::EPW * S::NS * S::NP;
+        return Quad2<NQ, LAW, R, FLUX_RUSANOV, 1>::SLAB;
     }
 }
 
@@ -56,9 +58,14 @@ __device__ __attribute__((always_inline)) inline void run_item(const StageParams
 {
     if constexpr (DIM == 3)
         plane3_wave<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>(P, cell0, reinterpret_cast<R *>(wl));
-    else
-        stage_wave<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>(P, cell0,
-                                                              reinterpret_cast<R *>(wl));
+    else {
+        // the lane index is made opaque here, so that what each kind derives
+        // from it stays inside that kind's code: hoisted out of the work-list
+        // loop for every kind at once, it would stay live across the loop
+        int lane = (int)(threadIdx.x & 63);
+        asm volatile("" : "+v"(lane));
+        quad2_group<NQ, LAW, R, FLUX_RUSANOV, 0, 1>(P, cell0, reinterpret_cast<R *>(wl), lane);
+    }
 }
 
 template <int DIM, int NQ>
@@ -87,7 +94,7 @@ __device__ __attribute__((always_inline)) inline void run_kind(int kind, const S
 
 // Every wave exits after its last item: the loop bound is the list length.
 template <int DIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIM == 2 ? 4 : 2)))
 dg_batch_kernel(const StageParams *__restrict__ members, const BatchItem *__restrict__ items,
                 int64_t nitems, int64_t nwaves, int slab_bytes)
 {
@@ -101,6 +108,8 @@ dg_batch_kernel(const StageParams *__restrict__ members, const BatchItem *__rest
     // the same time share kinds and neighbouring cells, and every XCD sees
     // the whole mix of kinds (cheap and costly items alike)
     const int64_t first = wave_uniform(xcd_remap(blockIdx.x, gridDim.x) * wpb + wave);
+    if constexpr (DIM == 2)
+        quad2_clear_spare(reinterpret_cast<double *>(wl), (int)(threadIdx.x & 63));
     for (int64_t it = first; it < nitems; it += nwaves) {
         const BatchItem w = items[it];
         run_kind<DIM, BatchRange<DIM>::NQ_MIN>(w.kind, members[w.member], w.cell0, wl);
@@ -128,10 +137,10 @@ dg_batch_kind_kernel(const StageParams *__restrict__ members, const BatchItem *_
     // known alignment gives the same LDS access widths (16-byte reads where
     // that kernel has them; for FP32 those measured slower)
     (void)slab_bytes;
-    run_item<DIM, NQ, LAW, R>(members[w.member], w.cell0,
-                              reinterpret_cast<unsigned char *>(
-                                  reinterpret_cast<R *>(dg_lds) +
-                                  wave * kind_slab_elems<DIM, NQ, LAW, R>()));
+    R *wl = reinterpret_cast<R *>(dg_lds) + wave * kind_slab_elems<DIM, NQ, LAW, R>();
+    if constexpr (DIM == 2)
+        quad2_clear_spare(wl, (int)(threadIdx.x & 63));
+    run_item<DIM, NQ, LAW, R>(members[w.member], w.cell0, reinterpret_cast<unsigned char *>(wl));
 }
 
 template <int DIM, int NQ>
@@ -182,7 +191,7 @@ void layout_of(int *epw, int *slab_bytes)
     if constexpr (DIM == 3)
         *epw = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
     else
-        *epw = Stage<DIM, NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
+        *epw = Quad2<NQ, LAW, R, FLUX_RUSANOV, 1>::CPG;
     *slab_bytes = kind_slab_bytes<DIM, NQ, LAW, R>();
 }
 

@@ -2,8 +2,10 @@
 // law) triple; the Makefile compiles this file once per triple with
 //   -DINST_DIM=<1|2|3> -DINST_REAL=<double|float> -DINST_LAW=<1|2> -DINST_FN=<name>
 // and api.hip asks every unit for a launcher.  3-D uses the column kernel
-// (dg_plane3.hpp), 1-D and 2-D the pencil kernel (dg_kernels.hpp).
+// (dg_plane3.hpp), 2-D affine cells the node kernel (dg_quad2.hpp), 1-D and
+// 2-D general cells the pencil kernel (dg_kernels.hpp).
 #include "dg_plane3.hpp"
+#include "dg_quad2.hpp"
 
 #ifndef INST_DIM
 #error "INST_DIM must be defined"
@@ -20,6 +22,8 @@ StageLauncher pick_one()
     using R = INST_REAL;
     if constexpr (DIM == 3)
         return &launch_plane3<NQ, LAW, R, GEO, FLUX>;
+    else if constexpr (DIM == 2 && GEO == GEO_AFFINE)
+        return &launch_quad2<NQ, LAW, R, FLUX>;
     else
         return &launch_stage<DIM, NQ, LAW, R, GEO, FLUX>;
 }
