@@ -42,7 +42,7 @@ constexpr int kind_slab_elems()
         using T = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>;
         return T::EPW * T::SLAB;
     } else {
-        return Quad2<NQ, LAW, R, FLUX_RUSANOV, 1>::SLAB;
+        return Quad2<NQ, LAW, R, FLUX_RUSANOV, 1, quad2_nodes_per_lane<LAW, R>()>::SLAB;
     }
 }
 
@@ -64,7 +64,8 @@ __device__ __attribute__((always_inline)) inline void run_item(const StageParams
         // loop for every kind at once, it would stay live across the loop
         int lane = (int)(threadIdx.x & 63);
         asm volatile("" : "+v"(lane));
-        quad2_group<NQ, LAW, R, FLUX_RUSANOV, 0, 1>(P, cell0, reinterpret_cast<R *>(wl), lane);
+        quad2_group<NQ, LAW, R, FLUX_RUSANOV, 0, 1, quad2_nodes_per_lane<LAW, R>()>(
+            P, cell0, reinterpret_cast<R *>(wl), lane);
     }
 }
 
@@ -191,7 +192,7 @@ void layout_of(int *epw, int *slab_bytes)
     if constexpr (DIM == 3)
         *epw = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
     else
-        *epw = Quad2<NQ, LAW, R, FLUX_RUSANOV, 1>::CPG;
+        *epw = Quad2<NQ, LAW, R, FLUX_RUSANOV, 1, quad2_nodes_per_lane<LAW, R>()>::CELLS;
     *slab_bytes = kind_slab_bytes<DIM, NQ, LAW, R>();
 }
 

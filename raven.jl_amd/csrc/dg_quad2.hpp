@@ -19,33 +19,39 @@
 //    spare, which leaves their sums unchanged);
 //  * Nq^2 = 25, 36, 49 leave lanes of a wave idle, so G = 2, 4, 4 waves
 //    share one workgroup's cells (5, 7, 5 cells over 128, 256, 256 lanes) and
-//    synchronise with workgroup barriers; the work-list kernels use G = 1.
+//    synchronise with workgroup barriers; the work-list kernels use G = 1;
+//  * light cells (advection, FP32 Euler) give a lane V = 4 or 2 nodes, one in
+//    each of V sets of cells, all requested before the first wait: a wave
+//    then has V times the bytes in flight.
 #pragma once
 
 #include "dg_kernels.hpp"
 
 namespace rv {
 
-template <int NQ, int LAW, typename R, int FLUX, int GW>
+template <int NQ, int LAW, typename R, int FLUX, int GW, int VW>
 struct Quad2 {
     static constexpr int NS = LawInfo<2, LAW>::NS;
     static constexpr int NP = NQ * NQ;
     static constexpr int G = GW;                 // waves sharing the cells
+    static constexpr int V = VW;                 // nodes per lane (sets of cells)
     static constexpr int LANES = 64 * G;
-    static constexpr int CPG = LANES / NP;       // cells per group of G waves
+    static constexpr int CPS = LANES / NP;       // cells per set
+    static constexpr int CELLS = V * CPS;        // cells per group
     static constexpr int NFU = 4 * NQ;           // face nodes per cell
-    static constexpr int FPASS = (CPG * NFU + LANES - 1) / LANES;
+    static constexpr int FPASS = (CPS * NFU + LANES - 1) / LANES;
     // LDS of a group, in elements of R.  The spare sits first, at the same
     // place for every (Nq, law), so that kinds sharing a slab in the work-list
-    // kernel never overwrite it.
-    static constexpr int SPARE = 0;                    // SPARE_LEN zeros: 32 bytes in
+    // kernel never overwrite it; then one part per set of cells.
+    static constexpr int SPARE = 0;                        // SPARE_LEN zeros: 32 bytes in
     static constexpr int SPARE_LEN = 32 / (int)sizeof(R);  // either precision
-    static constexpr int FXO = SPARE_LEN;              // [c][s][j][n] = f_x(n, j)
-    static constexpr int FYO = FXO + CPG * NS * NP;    // [c][s][i][n] = f_y(i, n)
-    static constexpr int NFO = FYO + CPG * NS * NP;    // [c][f][p][s] numerical flux
-    static constexpr int SLAB = NFO + CPG * NFU * NS;
+    static constexpr int FXO = 0;                          // [c][s][j][n] = f_x(n, j)
+    static constexpr int FYO = CPS * NS * NP;              // [c][s][i][n] = f_y(i, n)
+    static constexpr int NFO = 2 * CPS * NS * NP;          // [c][f][p][s] numerical flux
+    static constexpr int PART = NFO + CPS * NFU * NS;
+    static constexpr int SLAB = SPARE_LEN + V * PART;
+    static_assert(CPS >= 1, "Nq^2 must not exceed the group's lanes");
     static_assert(NS <= SPARE_LEN, "the spare must cover every state");
-    static_assert(CPG >= 1, "Nq^2 must not exceed the group's lanes");
 };
 
 // Waves per group of the single-operator kernel: pack cells across waves
@@ -56,14 +62,22 @@ constexpr int quad2_group_waves()
     return NQ == 5 ? 2 : (NQ == 6 || NQ == 7) ? 4 : 1;
 }
 
-// The cells [cell0, cell0 + CPG) of one group; gl is the lane's index in the
-// group (0 .. 64 G - 1) and glds the group's slab.  The spare must hold zeros
+// Nodes per lane: as many as the registers allow for the law and precision.
+template <int LAW, typename R>
+constexpr int quad2_nodes_per_lane()
+{
+    return LAW == LAW_ADVECTION ? 4 : sizeof(R) == 4 ? 2 : 1;
+}
+
+// The cells [cell0, cell0 + CELLS) of one group: set v holds the cells
+// [cell0 + v CPS, cell0 + (v + 1) CPS).  gl is the lane's index in the group
+// (0 .. 64 G - 1) and glds the group's slab, whose spare must hold zeros
 // (quad2_clear_spare) when the group starts.
-template <int NQ, int LAW, typename R, int FLUX, int ST, int G>
+template <int NQ, int LAW, typename R, int FLUX, int ST, int G, int V>
 __device__ __attribute__((always_inline)) inline void quad2_group(const StageParams &P,
                                                                   int64_t cell0, R *glds, int gl)
 {
-    using Q = Quad2<NQ, LAW, R, FLUX, G>;
+    using Q = Quad2<NQ, LAW, R, FLUX, G, V>;
     using S = Stage<2, NQ, LAW, R, GEO_AFFINE, FLUX>;
     constexpr int NS = Q::NS, NP = Q::NP;
     const Law<2, LAW, R> law(P);
@@ -75,22 +89,37 @@ __device__ __attribute__((always_inline)) inline void quad2_group(const StagePar
     const int c = gl / NP;
     const int node = gl - c * NP;
     const int i = node % NQ, j = node / NQ;
-    const int64_t e = cell0 + c;
-    const bool active = c < Q::CPG && e < P.K;
-    const int64_t off = e * (NS * NP) + node;
-
-    // the node's state and dq_old, and the operator rows, lift weights and
-    // cell scales of the node (requested before the first wait)
-    R q[NS], dqo[NS], mx[NQ], my[NQ], wx = R(0), wy = R(0), sx = R(0), sy = R(0);
-    if (active) {
+    bool active[V];
+    int64_t off[V];
 #pragma unroll
-        for (int s = 0; s < NS; ++s)
-            q[s] = qin[off + s * NP];
-        if (P.read_dq) {
+    for (int v = 0; v < V; ++v) {
+        const int64_t e = cell0 + v * Q::CPS + c;
+        active[v] = c < Q::CPS && e < P.K;
+        off[v] = e * (NS * NP) + node;
+    }
+
+    // the node's state and dq_old, the cell scales, and the operator rows and
+    // lift weights of the node (requested before the first wait)
+    R q[V][NS], dqo[V][NS], sx[V], sy[V], mx[NQ], my[NQ];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        sx[v] = sy[v] = R(0);
+        if (active[v]) {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-                dqo[s] = load_dq<ST>(dqp + off + s * NP);
+                q[v][s] = qin[off[v] + s * NP];
+            if (P.read_dq) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    dqo[v][s] = load_dq<ST>(dqp + off[v] + s * NP);
+            }
+            const int64_t e = cell0 + v * Q::CPS + c;
+            sx[v] = R(P.affine[e * 2 + 0]);
+            sy[v] = R(P.affine[e * 2 + 1]);
         }
+    }
+    R wx = R(0), wy = R(0);
+    if (active[0]) {
         const double *M = P.Dw;
 #pragma unroll
         for (int n = 0; n < NQ; ++n) {
@@ -99,81 +128,89 @@ __device__ __attribute__((always_inline)) inline void quad2_group(const StagePar
         }
         wx = R(P.winv[i == 0 ? 0 : NQ - 1]);
         wy = R(P.winv[j == 0 ? 0 : NQ - 1]);
-        sx = R(P.affine[e * 2 + 0]);
-        sy = R(P.affine[e * 2 + 1]);
     }
     // face nodes: own and neighbour state (face record first)
-    R qm[Q::FPASS][NS], qp[Q::FPASS][NS];
+    R qm[V][Q::FPASS][NS], qp[V][Q::FPASS][NS];
 #pragma unroll
-    for (int ps = 0; ps < Q::FPASS; ++ps) {
-        const int u = gl + ps * Q::LANES;
-        const int fc = u / Q::NFU;
-        const int r = u - fc * Q::NFU;
-        const int f = r / NQ, p = r - f * NQ;
-        const int64_t fe = cell0 + fc;
-        const bool d_on = (f >> 1) == 0 ? onx : ony;
-        if (fc < Q::CPG && fe < P.K && d_on) {
-            const R *qe = qin + fe * (NS * NP) + face_node<2, NQ>(f, p);
+    for (int v = 0; v < V; ++v)
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
-                qm[ps][s] = qe[s * NP];
-            S::neighbour_state(P, qin, fe, f, p, qm[ps], qp[ps]);
+        for (int ps = 0; ps < Q::FPASS; ++ps) {
+            const int u = gl + ps * Q::LANES;
+            const int fc = u / Q::NFU;
+            const int r = u - fc * Q::NFU;
+            const int f = r / NQ, p = r - f * NQ;
+            const int64_t fe = cell0 + v * Q::CPS + fc;
+            const bool d_on = (f >> 1) == 0 ? onx : ony;
+            if (fc < Q::CPS && fe < P.K && d_on) {
+                const R *qe = qin + fe * (NS * NP) + face_node<2, NQ>(f, p);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    qm[v][ps][s] = qe[s * NP];
+                S::neighbour_state(P, qin, fe, f, p, qm[v][ps], qp[v][ps]);
+            }
         }
-    }
     // flux rows at the node, published for the derivative sums
-    if (active) {
-        R fx[NS], fy[NS];
-        law.flux_row(0, q, fx);
-        law.flux_row(1, q, fy);
-        R *cl = glds + c * (NS * NP);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            cl[Q::FXO + s * NP + node] = fx[s];
-            cl[Q::FYO + s * NP + i * NQ + j] = fy[s];
+    for (int v = 0; v < V; ++v)
+        if (active[v]) {
+            R fx[NS], fy[NS];
+            law.flux_row(0, q[v], fx);
+            law.flux_row(1, q[v], fy);
+            R *cl = glds + Q::SPARE_LEN + v * Q::PART + c * (NS * NP);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                cl[Q::FXO + s * NP + node] = fx[s];
+                cl[Q::FYO + s * NP + i * NQ + j] = fy[s];
+            }
         }
-    }
     // numerical fluxes of the face nodes
 #pragma unroll
-    for (int ps = 0; ps < Q::FPASS; ++ps) {
-        const int u = gl + ps * Q::LANES;
-        const int fc = u / Q::NFU;
-        const int r = u - fc * Q::NFU;
-        const int f = r / NQ;
-        const int64_t fe = cell0 + fc;
-        const int d = f >> 1;
-        const bool d_on = d == 0 ? onx : ony;
-        if (fc < Q::CPG && fe < P.K && d_on) {
-            // the direction as a constant: a run-time one would index the
-            // law's private arrays (scratch memory)
-            R fm[NS], nf[NS];
-            const R sgn = (f & 1) ? R(1) : R(-1);
-            if (d == 0) {
-                law.flux_row(0, qm[ps], fm);
-                S::nflux_axis(law, 0, sgn, fm, qm[ps], qp[ps], nf);
-            } else {
-                law.flux_row(1, qm[ps], fm);
-                S::nflux_axis(law, 1, sgn, fm, qm[ps], qp[ps], nf);
-            }
+    for (int v = 0; v < V; ++v)
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
-                glds[Q::NFO + u * NS + s] = nf[s];
+        for (int ps = 0; ps < Q::FPASS; ++ps) {
+            const int u = gl + ps * Q::LANES;
+            const int fc = u / Q::NFU;
+            const int r = u - fc * Q::NFU;
+            const int f = r / NQ;
+            const int64_t fe = cell0 + v * Q::CPS + fc;
+            const int d = f >> 1;
+            const bool d_on = d == 0 ? onx : ony;
+            if (fc < Q::CPS && fe < P.K && d_on) {
+                // the direction as a constant: a run-time one would index the
+                // law's private arrays (scratch memory)
+                R fm[NS], nf[NS];
+                const R sgn = (f & 1) ? R(1) : R(-1);
+                if (d == 0) {
+                    law.flux_row(0, qm[v][ps], fm);
+                    S::nflux_axis(law, 0, sgn, fm, qm[v][ps], qp[v][ps], nf);
+                } else {
+                    law.flux_row(1, qm[v][ps], fm);
+                    S::nflux_axis(law, 1, sgn, fm, qm[v][ps], qp[v][ps], nf);
+                }
+                R *nb = glds + Q::SPARE_LEN + v * Q::PART + Q::NFO;
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    nb[u * NS + s] = nf[s];
+            }
         }
-    }
     if constexpr (G == 1)
         wave_lds_sync();
     else
         __syncthreads();
 
-    if (active) {
+    const R a = R(P.a), bdt = R(P.b_dt);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        if (!active[v])
+            continue;
         // numerical flux of the face the node lies on (the spare's zeros for
         // interior nodes)
-        const int fcell = Q::NFO + c * Q::NFU * NS;
+        const int fcell = Q::SPARE_LEN + v * Q::PART + Q::NFO + c * Q::NFU * NS;
         const int bx = i == 0 ? fcell + j * NS
                               : (i == NQ - 1 ? fcell + (NQ + j) * NS : Q::SPARE);
         const int by = j == 0 ? fcell + (2 * NQ + i) * NS
                               : (j == NQ - 1 ? fcell + (3 * NQ + i) * NS : Q::SPARE);
-        const R a = R(P.a), bdt = R(P.b_dt);
-        const R *cl = glds + c * (NS * NP);
+        const R *cl = glds + Q::SPARE_LEN + v * Q::PART + c * (NS * NP);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             R vx = R(0), vy = R(0);
@@ -184,7 +221,7 @@ __device__ __attribute__((always_inline)) inline void quad2_group(const StagePar
                 for (int n = 1; n < NQ; ++n)
                     vx = vx + mx[n] * row[n];
                 vx = vx - wx * glds[bx + s];
-                vx = sx * vx;
+                vx = sx[v] * vx;
             }
             if (ony) {
                 const R *row = cl + Q::FYO + s * NP + i * NQ;
@@ -193,15 +230,15 @@ __device__ __attribute__((always_inline)) inline void quad2_group(const StagePar
                 for (int n = 1; n < NQ; ++n)
                     vy = vy + my[n] * row[n];
                 vy = vy - wy * glds[by + s];
-                vy = sy * vy;
+                vy = sy[v] * vy;
             }
             const R tot = vx + vy;
-            R v = tot;
+            R val = tot;
             if (P.read_dq)
-                v = a * dqo[s] + tot;
-            store_out<ST>(v, dqp + off + s * NP);
+                val = a * dqo[v][s] + tot;
+            store_out<ST>(val, dqp + off[v] + s * NP);
             if (P.write_q)
-                store_out<ST>(R(q[s] + bdt * v), qo + off + s * NP);
+                store_out<ST>(R(q[v][s] + bdt * val), qo + off[v] + s * NP);
         }
     }
 }
@@ -217,10 +254,10 @@ __device__ inline void quad2_clear_spare(R *glds, int gl)
 
 // Single operator: a workgroup of wpb waves holds wpb / G groups (G > 1:
 // wpb = G, one group).
-template <int NQ, int LAW, typename R, int FLUX, int ST, int G>
+template <int NQ, int LAW, typename R, int FLUX, int ST, int G, int V>
 __global__ void __launch_bounds__(256) dg_quad2_kernel(const StageParams P)
 {
-    using Q = Quad2<NQ, LAW, R, FLUX, G>;
+    using Q = Quad2<NQ, LAW, R, FLUX, G, V>;
     extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
     const int grp = (int)(threadIdx.x / Q::LANES);
     const int gl = (int)(threadIdx.x - grp * Q::LANES);
@@ -228,14 +265,15 @@ __global__ void __launch_bounds__(256) dg_quad2_kernel(const StageParams P)
     const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
     R *glds = reinterpret_cast<R *>(dg_lds) + grp * Q::SLAB;
     quad2_clear_spare(glds, gl);
-    quad2_group<NQ, LAW, R, FLUX, ST, G>(P, (blk * gpb + grp) * Q::CPG, glds, gl);
+    quad2_group<NQ, LAW, R, FLUX, ST, G, V>(P, (blk * gpb + grp) * Q::CELLS, glds, gl);
 }
 
 template <int NQ, int LAW, typename R, int FLUX>
 int launch_quad2(const StageParams &P, void *stream, int wpb)
 {
     constexpr int G = quad2_group_waves<NQ>();
-    using Q = Quad2<NQ, LAW, R, FLUX, G>;
+    constexpr int V = quad2_nodes_per_lane<LAW, R>();
+    using Q = Quad2<NQ, LAW, R, FLUX, G, V>;
     if (P.K <= 0)
         return 0;
     if (G > 1)
@@ -243,15 +281,15 @@ int launch_quad2(const StageParams &P, void *stream, int wpb)
     else if (wpb != 1 && wpb != 2 && wpb != 4)
         wpb = 1;
     const int gpb = wpb / G;
-    const int64_t groups = (P.K + Q::CPG - 1) / Q::CPG;
+    const int64_t groups = (P.K + Q::CELLS - 1) / Q::CELLS;
     const int64_t blocks = (groups + gpb - 1) / gpb;
     const size_t lds = (size_t)gpb * Q::SLAB * sizeof(R);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (P.streaming)
-        hipLaunchKernelGGL((dg_quad2_kernel<NQ, LAW, R, FLUX, 1, G>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((dg_quad2_kernel<NQ, LAW, R, FLUX, 1, G, V>), dim3((unsigned)blocks),
                            dim3(64 * wpb), lds, st, P);
     else
-        hipLaunchKernelGGL((dg_quad2_kernel<NQ, LAW, R, FLUX, 0, G>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((dg_quad2_kernel<NQ, LAW, R, FLUX, 0, G, V>), dim3((unsigned)blocks),
                            dim3(64 * wpb), lds, st, P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
