@@ -98,8 +98,9 @@ struct rv_handle {
     MortarLauncher mortar = nullptr;
     int64_t mortar_cells = 0;
     int64_t *m_cells = nullptr, *m_partner = nullptr;
-    int32_t *m_kind = nullptr;
+    int32_t *m_kind = nullptr, *m_slot = nullptr;
     double *m_sgeo = nullptr, *m_halves = nullptr;
+    unsigned char *m_delta = nullptr;  // per (mortar cell, face) updates of one stage
 };
 
 namespace {
@@ -199,7 +200,7 @@ void free_handle(rv_handle *h)
                     h->sgeo,   h->Dt,           h->Dw,       h->winv,     h->MJ,
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
                     h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
-                    h->cellJ,  h->relax_partials, h->eprod};
+                    h->m_delta, h->m_slot, h->cellJ,  h->relax_partials, h->eprod};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -839,14 +840,14 @@ StageParams stage_params(rv_handle *h)
     return P;
 }
 
-int launch(rv_handle *h, const StageParams &P, StageLauncher fn = nullptr)
+int launch(rv_handle *h, const StageParams &P0, StageLauncher fn = nullptr)
 {
-    if ((fn ? fn : h->launcher)(P, h->stream, P.write_q ? h->wpb : h->wpb_rhs) != 0) {
-        hipError_t e = hipGetLastError();
-        return fail(h, RV_E_HIP, std::string("stage launch failed: ") + hipGetErrorString(e));
-    }
+    StageParams P = P0;
+    MortarParams M{};
+    // the 3-D weak-form column kernel adds the mortar updates after its own
+    // stores; the other stage kernels leave that to mortar_apply_kernel
+    const bool fused = h->mortar && !fn && h->dim == 3 && h->vflux == 0;
     if (h->mortar) {
-        MortarParams M{};
         M.stage = P;
         M.ncells = h->mortar_cells;
         M.cells = as_global(h->m_cells);
@@ -854,10 +855,24 @@ int launch(rv_handle *h, const StageParams &P, StageLauncher fn = nullptr)
         M.partner = as_global(h->m_partner);
         M.sgeo = as_global(h->m_sgeo);
         M.halves = as_global(h->m_halves);
-        if (h->mortar(M, h->stream) != 0) {
+        M.delta = as_global(static_cast<void *>(h->m_delta));
+        if (h->mortar(M, h->stream, 0) != 0) {
             hipError_t e = hipGetLastError();
             return fail(h, RV_E_HIP, std::string("mortar launch failed: ") + hipGetErrorString(e));
         }
+        if (fused) {
+            P.mslot = as_global(h->m_slot);
+            P.mkind = as_global(h->m_kind);
+            P.mdelta = M.delta;
+        }
+    }
+    if ((fn ? fn : h->launcher)(P, h->stream, P.write_q ? h->wpb : h->wpb_rhs) != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(h, RV_E_HIP, std::string("stage launch failed: ") + hipGetErrorString(e));
+    }
+    if (h->mortar && !fused && h->mortar(M, h->stream, 1) != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(h, RV_E_HIP, std::string("mortar launch failed: ") + hipGetErrorString(e));
     }
     return RV_OK;
 }
@@ -1289,13 +1304,19 @@ int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *par
                     " Rusanov or central");
     std::vector<double> halves(2 * h->nq * h->nq);
     lobatto_halves(h->nq, halves.data(), halves.data() + h->nq * h->nq);
+    std::vector<int32_t> slot(h->K, -1);
+    for (size_t li = 0; li < cells.size(); ++li)
+        slot[cells[li]] = (int32_t)li;
     const size_t nsg = (size_t)h->K * nf * (h->dim + 1) * h->nfp;
     int rc;
     if ((rc = upload(h, &h->m_cells, cells.data(), std::max<size_t>(cells.size(), 1))) ||
         (rc = upload(h, &h->m_kind, face_kind, (size_t)h->K * nf)) ||
         (rc = upload(h, &h->m_partner, partner, (size_t)h->K * nf * 4)) ||
         (rc = upload(h, &h->m_halves, halves.data(), halves.size())) ||
-        (rc = dalloc(h, &h->m_sgeo, nsg)))
+        (rc = upload(h, &h->m_slot, slot.data(), slot.size())) ||
+        (rc = dalloc(h, &h->m_sgeo, nsg)) ||
+        (rc = dalloc(h, &h->m_delta, cells.size() * nf * h->ns * h->nfp *
+                                         (h->dtype == RV_F32 ? 4 : 8))))
         return rc;
     HIP_TRY(h, hipMemcpyAsync(h->m_sgeo, h->sgeo, nsg * sizeof(double), hipMemcpyDeviceToDevice,
                               h->stream));

@@ -73,6 +73,12 @@ struct StageParams {
     // with k = L(q_in) accumulated alone, and each workgroup stores
     // sum MJ v(q_in) . k over its nodes at eprod[workgroup]
     double RV_GLOBAL *eprod;
+    // mortar updates the 3-D column kernel adds after its stores (nullptr:
+    // none): per cell its slot in mdelta or -1, per (cell, face) the face
+    // kind, and the per-(slot, face) updates of mortar_face_kernel
+    const int32_t RV_GLOBAL *mslot;
+    const int32_t RV_GLOBAL *mkind;
+    const void RV_GLOBAL *mdelta;
 };
 
 // Launch the fused stage for one kernel configuration; returns false when no
@@ -94,8 +100,11 @@ struct MortarParams {
     const int64_t RV_GLOBAL *partner;  // (K, 2dim, 4), see forest_mortars
     const double RV_GLOBAL *sgeo;      // surface geometry with the lift weights kept
     const double RV_GLOBAL *halves;    // tohalves_1d, (2, Nq, Nq) row-major
+    void RV_GLOBAL *delta;             // (ncells, 2dim, Ns, Nfp) per-face updates (R)
 };
-using MortarLauncher = int (*)(const MortarParams &, void *stream);
+// phase 0: the per-face updates (before the stage kernel); phase 1: adding
+// them to dq and q_out (after a stage kernel that does not add them itself)
+using MortarLauncher = int (*)(const MortarParams &, void *stream, int phase);
 MortarLauncher find_mortar_launcher(int dim, int nq, int law, int real_bytes, int flux);
 
 // Flux-differencing kernels (fluxdiff.hip): whole cells per workgroup, a

@@ -153,6 +153,10 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     R *ctab = slab + T::SLAB0 + T::XFB;  // with WLDS: [output node][n], x rows then y rows
     R sz = R(1);
     FaceRec rec[6];
+    // cells with non-conforming faces (mortar_face_kernel ran before this
+    // stage): their slot in the mortar lift buffer and which faces those are
+    int mli = -1;
+    unsigned mmask = 0;
     const R a = R(P.a), bdt = R(P.b_dt);
     // a*dq_old is formed at the first z scatter so that the dq loads need not
     // complete before the first plane's flux; general cells accumulate
@@ -184,6 +188,16 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
 #pragma unroll
         for (int f = 0; f < 6; ++f)
             rec[f] = P.faces[e * 6 + f];
+        if constexpr (GEO == GEO_GENERAL) {
+            if (P.mslot) {
+                mli = P.mslot[e];
+                unsigned mk = 0;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                    mk |= (P.mkind[e * 6 + f] != 0 ? 1u : 0u) << f;
+                mmask = mli >= 0 ? mk : 0u;
+            }
+        }
         const R *qe = qbase + e * (NS * NP) + l;
 #pragma unroll
         for (int k = 0; k < NQ; ++k)
@@ -494,9 +508,23 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                     lift = R(zsg[side][3]);
                     S::nflux_general(law, nv, qm, qp[4 + side], nf);
                 }
+                R lv[NS];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    acc[s][kend] = acc[s][kend] - lift * nf[s];
+                    lv[s] = lift * nf[s];
+                if constexpr (GEO == GEO_GENERAL) {
+                    // a non-conforming face: its mortar lift instead
+                    if ((mmask >> (4 + side)) & 1u) {
+                        const R *L = static_cast<const R *>(P.mdelta) +
+                                     ((int64_t)mli * 6 + 4 + side) * (NS * NQ2) + l;
+#pragma unroll
+                        for (int s = 0; s < NS; ++s)
+                            lv[s] = L[s * NQ2];
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    acc[s][kend] = acc[s][kend] - lv[s];
             }
         }
     }
@@ -542,9 +570,23 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 lift = R(sg[3 * NQ2]);
                 S::nflux_general(law, nv, qm, qp[f], nf);
             }
+            R lv[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s)
-                fb[(f * NS + s) * NQ2 + l] = lift * nf[s];
+                lv[s] = lift * nf[s];
+            if constexpr (GEO == GEO_GENERAL) {
+                // a non-conforming face: its mortar lift instead
+                if ((mmask >> f) & 1u) {
+                    const R *L = static_cast<const R *>(P.mdelta) +
+                                 ((int64_t)mli * 6 + f) * (NS * NQ2) + l;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        lv[s] = L[s * NQ2];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                fb[(f * NS + s) * NQ2 + l] = lv[s];
         }
     }
     lds_sync();

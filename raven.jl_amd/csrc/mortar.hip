@@ -13,10 +13,19 @@
 //     +MJI(i, j) sum_s sum_b H[s2][b][j] sum_a H[s1][a][i] wsJ_f nf_s(a, b),
 // the transpose of the interpolation, so the fluxes through the two sides of
 // the mortar cancel exactly up to rounding (discrete conservation).  The stage
-// kernel leaves these faces out (their lift weight is zero on the device), and
-// this kernel runs after it on the same stream: one wave per cell with
-// non-conforming faces, the faces in order, so every update is deterministic.
-// dq += delta and, in stage mode, q_out += b_dt * delta.
+// kernel leaves these faces out (their lift weight is zero on the device).
+//  * mortar_face_kernel, one wave per (cell with non-conforming faces, face),
+//    runs before the stage kernel on the same stream: the face's lift L at
+//    its face nodes (the update is delta = -MJI L), into a per-face buffer
+//    (the faces only read q_in, so they are independent of each other and of
+//    the stage): L = wsJ_f nf_s on the fine side, minus the projection on the
+//    coarse side;
+//  * the 3-D column kernel takes L as the lift of these faces, as it takes
+//    lift x flux on a conforming face; the other stage kernels leave them to
+//    mortar_apply_kernel (a thread per node of those cells), which adds
+//    dq += delta and, in stage mode, q_out += b_dt * delta after the stage,
+//    faces in ascending order, so every update is deterministic (and the
+//    same as applying the faces one after the other).
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -27,7 +36,7 @@ namespace rv {
 namespace {
 
 template <int DIM, int NQ, int LAW, typename R, int FLUX>
-__global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
+__global__ void __launch_bounds__(64) mortar_face_kernel(const MortarParams M)
 {
     using S = Stage<DIM, NQ, LAW, R, GEO_GENERAL, FLUX>;
     constexpr int NS = S::NS, NP = S::NP, NFP = S::NP2, NF = 2 * DIM;
@@ -43,13 +52,17 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
     __shared__ R u[NSUB][NS][NFP];
     const StageParams &P = M.stage;
     const Law<DIM, LAW, R> law(P);
-    const int64_t e = M.cells[blockIdx.x];
+    const int64_t li = blockIdx.x / NF;
+    const int f = (int)(blockIdx.x - li * NF);
+    const int64_t e = M.cells[li];
+    const int kind = M.kind[e * NF + f];
+    if (kind == 0)
+        return;  // the whole workgroup: a conforming face
     const int m = threadIdx.x;
     const bool on = m < NFP;
     const int a = m % NQ, b = m / NQ;  // face node m = a + NQ*b (b = 0 in 2-D)
     const R *qin = static_cast<const R *>(P.q_in);
-    R *dq = static_cast<R *>(P.dq);
-    R *qo = static_cast<R *>(P.q_out);
+    R *out = static_cast<R *>(M.delta) + (li * NF + f) * (NS * NFP);
     for (int t = m; t < 2 * NQ * NQ; t += 64)
         (&H[0][0][0])[t] = R(M.halves[t]);
     __syncthreads();
@@ -127,167 +140,200 @@ __global__ void __launch_bounds__(64) mortar_kernel(const MortarParams M)
         for (int s = 0; s < NS; ++s)
             wnf[s] = F.wsJ * nf[s];
     };
-    // dq += delta and, in stage mode, q_out += b_dt delta at this cell's node,
-    // from the values read with the face's other inputs
-    auto load_old = [&](int node, R *dq0, R *qo0) {
+    // the face's lift at face node m, to the per-face buffer
+    auto store_lift = [&](const R *lift) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int64_t idx = e * NS * NP + s * NP + node;
-            dq0[s] = dq[idx];
-            qo0[s] = P.write_q ? qo[idx] : R(0);
-        }
-    };
-    auto apply = [&](int node, const R *delta, const R *dq0, const R *qo0) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int64_t idx = e * NS * NP + s * NP + node;
-            dq[idx] = dq0[s] + delta[s];
-            if (P.write_q)
-                qo[idx] = qo0[s] + R(P.b_dt) * delta[s];
-        }
+        for (int s = 0; s < NS; ++s)
+            out[s * NFP + m] = lift[s];
     };
 
-    // faces in order: each face issues all of its loads (trace, fine states,
-    // geometry and the values it updates) before its first wait, and its
-    // updates land before the next face reads (edge nodes are shared),
-    // behind the workgroup barrier; the next face's kind and partners
-    // (scalar: the cell is the workgroup's) are requested a face ahead
-    int kind_next = M.kind[e * NF];
-    int64_t part_next[NSUB];
+    int64_t part[NSUB];
 #pragma unroll
     for (int t = 0; t < NSUB; ++t)
-        part_next[t] = M.partner[4 * (e * NF) + t];
-    for (int f = 0; f < NF; ++f) {
-        const int kind = kind_next;
-        int64_t part_f[NSUB];
+        part[t] = M.partner[4 * (e * NF + f) + t];
+    const int node = face_node<DIM, NQ>(f, m);
+    if (kind == 1) {
+        // this cell is the fine side: the coarse trace on its face f^1
+        const int64_t c = part[0];
+        const int sub = (int)part[1];
+        Fine F;
+        if (on) {
+            const int cn = face_node<DIM, NQ>(f ^ 1, m);
+            R trv[NS];
 #pragma unroll
-        for (int t = 0; t < NSUB; ++t)
-            part_f[t] = part_next[t];
-        if (f + 1 < NF) {
-            kind_next = M.kind[e * NF + f + 1];
+            for (int s = 0; s < NS; ++s)
+                trv[s] = qin[c * NS * NP + s * NP + cn];
+            load_fine(e, f, F);
 #pragma unroll
-            for (int t = 0; t < NSUB; ++t)
-                part_next[t] = M.partner[4 * (e * NF + f + 1) + t];
+            for (int s = 0; s < NS; ++s)
+                tr[s][m] = trv[s];
         }
-        if (kind == 0)
-            continue;
-        const int node = face_node<DIM, NQ>(f, m);
-        R mji = R(0), dq0[NS], qo0[NS];
-        if (kind == 1) {
-            // this cell is the fine side: the coarse trace on its face f^1
-            const int64_t c = part_f[0];
-            const int sub = (int)part_f[1];
-            Fine F;
-            if (on) {
-                const int cn = face_node<DIM, NQ>(f ^ 1, m);
-                R trv[NS];
+        __syncthreads();
+        R qi[NS];
+        interp(sub, qi);
+        if (on) {
+            R wnf[NS];
+            subflux(F, qi, wnf);
+            store_lift(wnf);
+        }
+    } else {
+        // this cell is the coarse side: its own trace, the fine fluxes of
+        // every sub-face, then their projection onto its face nodes
+        Fine F[NSUB];
+        if (on) {
+            R trv[NS];
 #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    trv[s] = qin[c * NS * NP + s * NP + cn];
-                load_fine(e, f, F);
-                mji = R(M.stage.mji[e * NP + node]);
-                load_old(node, dq0, qo0);
+            for (int s = 0; s < NS; ++s)
+                trv[s] = qin[e * NS * NP + s * NP + node];
 #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    tr[s][m] = trv[s];
-            }
-            __syncthreads();
+            for (int sub = 0; sub < NSUB; ++sub)
+                load_fine(part[sub], f ^ 1, F[sub]);
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                tr[s][m] = trv[s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int sub = 0; sub < NSUB; ++sub) {
             R qi[NS];
             interp(sub, qi);
             if (on) {
-                R wnf[NS], delta[NS];
-                subflux(F, qi, wnf);
+                R wnf[NS];
+                subflux(F[sub], qi, wnf);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    delta[s] = -(mji * wnf[s]);
-                apply(node, delta, dq0, qo0);
+                    g[sub][s][m] = wnf[s];
             }
-            __syncthreads();
-        } else {
-            // this cell is the coarse side: its own trace, the fine fluxes of
-            // every sub-face, then their projection onto its face nodes
-            Fine F[NSUB];
+        }
+        __syncthreads();
+        // projection: sum_sub sum_b H[s2][b][j] (sum_a H[s1][a][i] g(a, b)),
+        // the inner sums once per (i, b) by lane i + NQ*b
+        if constexpr (DIM == 3) {
             if (on) {
-                R trv[NS];
+                const int i = a, bb = b;
+                for (int sub = 0; sub < NSUB; ++sub) {
+                    const int s1 = sub & 1;
 #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    trv[s] = qin[e * NS * NP + s * NP + node];
+                    for (int s = 0; s < NS; ++s) {
+                        R inner = R(0);
 #pragma unroll
-                for (int sub = 0; sub < NSUB; ++sub)
-                    load_fine(part_f[sub], f ^ 1, F[sub]);
-                mji = R(M.stage.mji[e * NP + node]);
-                load_old(node, dq0, qo0);
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    tr[s][m] = trv[s];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int sub = 0; sub < NSUB; ++sub) {
-                R qi[NS];
-                interp(sub, qi);
-                if (on) {
-                    R wnf[NS];
-                    subflux(F[sub], qi, wnf);
-#pragma unroll
-                    for (int s = 0; s < NS; ++s)
-                        g[sub][s][m] = wnf[s];
-                }
-            }
-            __syncthreads();
-            // projection: sum_sub sum_b H[s2][b][j] (sum_a H[s1][a][i] g(a, b)),
-            // the inner sums once per (i, b) by lane i + NQ*b
-            if constexpr (DIM == 3) {
-                if (on) {
-                    const int i = a, bb = b;
-                    for (int sub = 0; sub < NSUB; ++sub) {
-                        const int s1 = sub & 1;
-#pragma unroll
-                        for (int s = 0; s < NS; ++s) {
-                            R inner = R(0);
-#pragma unroll
-                            for (int aa = 0; aa < NQ; ++aa)
-                                inner = inner + H[s1][aa][i] * g[sub][s][aa + NQ * bb];
-                            u[sub][s][m] = inner;
-                        }
+                        for (int aa = 0; aa < NQ; ++aa)
+                            inner = inner + H[s1][aa][i] * g[sub][s][aa + NQ * bb];
+                        u[sub][s][m] = inner;
                     }
                 }
-                __syncthreads();
-            }
-            if (on) {
-                const int i = a, j = b;
-                R delta[NS];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    R v = R(0);
-                    for (int sub = 0; sub < NSUB; ++sub) {
-                        const int s1 = sub & 1, s2 = (sub >> 1) & 1;
-                        if constexpr (DIM == 2) {
-#pragma unroll
-                            for (int aa = 0; aa < NQ; ++aa)
-                                v = v + H[s1][aa][i] * g[sub][s][aa];
-                        } else {
-#pragma unroll
-                            for (int bb = 0; bb < NQ; ++bb)
-                                v = v + H[s2][bb][j] * u[sub][s][i + NQ * bb];
-                        }
-                    }
-                    delta[s] = mji * v;
-                }
-                apply(node, delta, dq0, qo0);
             }
             __syncthreads();
+        }
+        if (on) {
+            const int i = a, j = b;
+            R lift[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                R v = R(0);
+                for (int sub = 0; sub < NSUB; ++sub) {
+                    const int s1 = sub & 1, s2 = (sub >> 1) & 1;
+                    if constexpr (DIM == 2) {
+#pragma unroll
+                        for (int aa = 0; aa < NQ; ++aa)
+                            v = v + H[s1][aa][i] * g[sub][s][aa];
+                    } else {
+#pragma unroll
+                        for (int bb = 0; bb < NQ; ++bb)
+                            v = v + H[s2][bb][j] * u[sub][s][i + NQ * bb];
+                    }
+                }
+                lift[s] = -v;
+            }
+            store_lift(lift);
         }
     }
 }
 
-template <int DIM, int NQ, int LAW, typename R, int FLUX>
-int launch_mortar(const MortarParams &M, void *stream)
+// Index of node `node` on face f (the inverse of face_node), or -1 when the
+// node is not on that face.
+template <int DIM, int NQ>
+__device__ inline int node_on_face(int f, int node)
 {
+    const int x = node % NQ, y = (node / NQ) % NQ, z = DIM == 3 ? node / (NQ * NQ) : 0;
+    const int d = f >> 1;
+    const int fixed = (f & 1) ? NQ - 1 : 0;
+    if constexpr (DIM == 2) {
+        if (d == 0)
+            return x == fixed ? y : -1;
+        return y == fixed ? x : -1;
+    } else {
+        if (d == 0)
+            return x == fixed ? y + NQ * z : -1;
+        if (d == 1)
+            return y == fixed ? x + NQ * z : -1;
+        return z == fixed ? x + NQ * y : -1;
+    }
+}
+
+// dq += delta (and q_out += b_dt delta) at node `node` of mortar cell li, for
+// its non-conforming faces in ascending order; delta = -MJI L.
+template <int DIM, int NQ, int LAW, typename R>
+__global__ void __launch_bounds__(256) mortar_apply_kernel(const MortarParams M)
+{
+    constexpr int NS = LawInfo<DIM, LAW>::NS, NP = IPow<NQ, DIM>::v, NFP = NP / NQ;
+    constexpr int NF = 2 * DIM;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t li = t / NP;
+    if (li >= M.ncells)
+        return;
+    const int node = (int)(t - li * NP);
+    const int64_t e = M.cells[li];
+    int mf[NF];
+    bool any = false;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        mf[f] = node_on_face<DIM, NQ>(f, node);
+        if (mf[f] >= 0 && M.kind[e * NF + f] == 0)
+            mf[f] = -1;
+        any = any || mf[f] >= 0;
+    }
+    if (!any)
+        return;
+    const StageParams &P = M.stage;
+    R *dq = static_cast<R *>(P.dq);
+    R *qo = static_cast<R *>(P.q_out);
+    const R *lift = static_cast<const R *>(M.delta) + li * NF * (NS * NFP);
+    const R bdt = R(P.b_dt);
+    const R mji = R(P.mji[e * NP + node]);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int64_t idx = e * NS * NP + s * NP + node;
+        R v = dq[idx];
+        R w = P.write_q ? qo[idx] : R(0);
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            if (mf[f] >= 0) {
+                const R d = -(mji * lift[(f * NS + s) * NFP + mf[f]]);
+                v = v + d;
+                w = w + bdt * d;
+            }
+        dq[idx] = v;
+        if (P.write_q)
+            qo[idx] = w;
+    }
+}
+
+template <int DIM, int NQ, int LAW, typename R, int FLUX>
+int launch_mortar(const MortarParams &M, void *stream, int phase)
+{
+    constexpr int NP = IPow<NQ, DIM>::v;
     if (M.ncells <= 0)
         return 0;
-    hipLaunchKernelGGL((mortar_kernel<DIM, NQ, LAW, R, FLUX>), dim3((unsigned)M.ncells), dim3(64),
-                       0, static_cast<hipStream_t>(stream), M);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (phase == 0) {
+        hipLaunchKernelGGL((mortar_face_kernel<DIM, NQ, LAW, R, FLUX>),
+                           dim3((unsigned)(M.ncells * 2 * DIM)), dim3(64), 0, st, M);
+    } else {
+        const int64_t nodes = M.ncells * NP;
+        hipLaunchKernelGGL((mortar_apply_kernel<DIM, NQ, LAW, R>),
+                           dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, st, M);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
