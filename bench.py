@@ -666,7 +666,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic("f64"),
+            "traffic": measured_traffic("fp64"),
             "kernel": KERNEL_NAME["f64"],
             "avg_kernel_ms": kavg * 1e3,
             "single_rhs_ms": r64["rhs_ms"],
@@ -698,7 +698,7 @@ def main():
             "single_rhs_GBs": 2 * i32.state_bytes / (r32["rhs_ms"] * 1e-3) / 1e9,
             "avg_kernel_ms": k32 * 1e3, "achieved_GBs": b32 / k32 / 1e9,
             "frac": b32 / k32 / 1e9 / HBM_PEAK_GBS,
-            "kernel": KERNEL_NAME["f32"], "traffic": measured_traffic("f32"),
+            "kernel": KERNEL_NAME["f32"], "traffic": measured_traffic("fp32"),
             "rel_l2_vs_fp64_after_steps": err,
             "mass_drift_fp64": float((m64[0] - m0[0]) / m0[0]),
             "mass_drift_fp32": float((m32[0] - m0[0]) / m0[0]),
