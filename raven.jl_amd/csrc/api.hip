@@ -83,7 +83,9 @@ struct rv_handle {
     double *cellJ = nullptr;     // brick handles: per-cell J that MJ is filled from
     double *partials = nullptr;  // reduction scratch
     double *relax_partials = nullptr;  // entropy passes: RELAX_SLOTS slots of 4*RELAX_BLOCKS
-    double *eprod = nullptr;           // fused relaxation stages: 14 slots of relax_partials_count
+    double *relax_sums = nullptr;       // device sums of the partials (relax_read)
+    double *relax_sums_host = nullptr;  // pinned copy of them
+    double *eprod = nullptr;           // fused relaxation stages: 14 slots of relax_partials_cap
     void *qtmp = nullptr;        // ping-pong state of rv_lsrk_run
     int64_t *dtoc = nullptr;     // 0-based
     int64_t *ctod_ptr = nullptr;
@@ -200,10 +202,13 @@ void free_handle(rv_handle *h)
                     h->sgeo,   h->Dt,           h->Dw,       h->winv,     h->MJ,
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
                     h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
-                    h->m_delta, h->m_slot, h->cellJ,  h->relax_partials, h->eprod};
+                    h->m_delta, h->m_slot, h->cellJ,  h->relax_partials, h->eprod,
+                    h->relax_sums};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
+    if (h->relax_sums_host)
+        (void)hipHostFree(h->relax_sums_host);
     if (h->own_stream && h->stream)
         (void)hipStreamDestroy(h->stream);
     if (h->forest)
@@ -578,7 +583,9 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
     KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
     key.vflux = h->vflux;
     h->launcher = find_stage_launcher(key);
-    if (h->vflux != 0) {
+    {
+        // the fused relaxation stage: flux-differencing kernels, and the 3-D
+        // column kernel of the weak form
         KernelKey rk = key;
         rk.relax = 1;
         h->relax_launcher = find_stage_launcher(rk);
@@ -921,6 +928,8 @@ int reduce(rv_handle *h, const void *q, int power, double *out, double pexp = 2.
 // Entropy passes (entropy.hip): slot 0 holds Newton evaluations and the
 // diagnostics, slot 1 the entropy integral of q0, slots 2.. the stages
 constexpr int RELAX_SLOTS = 16;
+// relax_sums: 2 per slot, then one per fused relaxation stage (at most 14)
+constexpr int RELAX_SUMS = 2 * RELAX_SLOTS + 16;
 // Newton iterations of the relaxation root per step at most
 constexpr int RELAX_MAX_NEWTON = 50;
 
@@ -930,15 +939,31 @@ int ensure_relax(rv_handle *h)
         return fail(h, RV_E_UNSUPPORTED, "no entropy for this law");
     if (h->relax_partials)
         return RV_OK;
+    if (int rc = dalloc(h, &h->relax_sums, (size_t)RELAX_SUMS))
+        return rc;
+    HIP_TRY(h, hipHostMalloc((void **)&h->relax_sums_host, RELAX_SUMS * sizeof(double)));
     return dalloc(h, &h->relax_partials, (size_t)4 * RELAX_BLOCKS * RELAX_SLOTS);
 }
 
 // partial sums one fused relaxation stage leaves: one per workgroup of the
 // flux-differencing launch
+// Product slots a fused relaxation stage writes (one per workgroup of the
+// flux-differencing kernel, one per wave of the 3-D column kernel) ...
 int64_t relax_partials_count(const rv_handle *h)
 {
+    if (h->vflux == 0)
+        return plane3_relax_slots(h->nq, h->K, h->wpb);
     const int cpb = fd_cells_per_workgroup((int)h->Np);
     return (h->K + cpb - 1) / cpb;
+}
+
+// ... and the most it can write whatever the waves per workgroup (the stride
+// of the per-stage slots in eprod, fixed when eprod is allocated)
+int64_t relax_partials_cap(const rv_handle *h)
+{
+    if (h->vflux == 0)
+        return plane3_relax_slots(h->nq, h->K, 4);
+    return relax_partials_count(h);
 }
 
 RelaxParams relax_params(rv_handle *h, int mode, int slot)
@@ -965,26 +990,29 @@ int relax(rv_handle *h, const RelaxParams &P)
     return RV_OK;
 }
 
-// Neumaier sums of slots [0, nslots) in workgroup order: out[2*slot] the
-// entropy integral, out[2*slot + 1] the entropy product
-int relax_read(rv_handle *h, int nslots, double *out)
+// Compensated sums of slots [0, nslots), formed on the device in a fixed
+// order: out[2*slot] the entropy integral, out[2*slot + 1] the entropy
+// product.  With nstage > 0 also the fused stages' product sums (launched
+// into relax_sums + 2 RELAX_SLOTS by the caller) into stage_out.  One pinned
+// read-back, one synchronisation.
+int relax_read(rv_handle *h, int nslots, double *out, int nstage = 0, double *stage_out = nullptr)
 {
-    std::vector<double> part((size_t)4 * RELAX_BLOCKS * nslots);
-    HIP_TRY(h, hipMemcpyAsync(part.data(), h->relax_partials, part.size() * sizeof(double),
+    if (launch_slot_sums(h->relax_partials, 2 * RELAX_BLOCKS, nslots, 2,
+                         (int64_t)4 * RELAX_BLOCKS, 2, 2, 4, h->relax_sums, h->stream) != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(h, RV_E_HIP, std::string("entropy sum launch failed: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(h, hipMemcpyAsync(h->relax_sums_host, h->relax_sums, 2 * nslots * sizeof(double),
                               hipMemcpyDeviceToHost, h->stream));
+    if (nstage > 0)
+        HIP_TRY(h, hipMemcpyAsync(h->relax_sums_host + 2 * RELAX_SLOTS,
+                                  h->relax_sums + 2 * RELAX_SLOTS, nstage * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    for (int slot = 0; slot < nslots; ++slot)
-        for (int which = 0; which < 2; ++which) {
-            double sum = 0, comp = 0;
-            for (int b = 0; b < RELAX_BLOCKS; ++b)
-                for (int k = 0; k < 2; ++k) {
-                    const double x = part[(size_t)4 * (RELAX_BLOCKS * slot + b) + 2 * which + k];
-                    const double t = sum + x;
-                    comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
-                    sum = t;
-                }
-            out[2 * slot + which] = sum + comp;
-        }
+    for (int i = 0; i < 2 * nslots; ++i)
+        out[i] = h->relax_sums_host[i];
+    for (int i = 0; i < nstage; ++i)
+        stage_out[i] = h->relax_sums_host[2 * RELAX_SLOTS + i];
     return RV_OK;
 }
 
@@ -1661,7 +1689,7 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
         // sums of MJ v(q) . k; the stages ping-pong between q and k (k is
         // needed as an array only after the stages)
         const int64_t nwg = relax_partials_count(h);
-        const int64_t cap = nwg;
+        const int64_t cap = relax_partials_cap(h);
         if (!h->eprod)
             if (int rc = dalloc(h, &h->eprod, (size_t)14 * cap))
                 return rc;
@@ -1681,23 +1709,17 @@ int rv_rlsrk_step(rv_handle *h, void *q, void *dq, void *k, void *q0, int scheme
             std::swap(cur, alt);
         }
         qfinal = cur;
-        std::vector<double> part((size_t)nst * cap);
-        HIP_TRY(h, hipMemcpyAsync(part.data(), h->eprod, part.size() * sizeof(double),
-                                  hipMemcpyDeviceToHost, h->stream));
+        // each stage's product: its slots summed on the device
+        if (launch_slot_sums(h->eprod, nwg, nst, 1, cap, 0, 1, 1, h->relax_sums + 2 * RELAX_SLOTS,
+                             h->stream) != 0) {
+            hipError_t e = hipGetLastError();
+            return fail(h, RV_E_HIP, std::string("entropy sum launch failed: ") +
+                                         hipGetErrorString(e));
+        }
         double e2[4];
-        if (int rc = relax_read(h, 2, e2))  // synchronises the stream
+        if (int rc = relax_read(h, 2, e2, nst, stage_prod))  // synchronises the stream
             return rc;
         eta0 = e2[2];
-        for (int st = 0; st < nst; ++st) {
-            double sum = 0, comp = 0;
-            for (int64_t b = 0; b < nwg; ++b) {
-                const double x = part[(size_t)st * cap + b];
-                const double t = sum + x;
-                comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
-                sum = t;
-            }
-            stage_prod[st] = sum + comp;
-        }
     } else {
         // stages: k = L(q); dq = a dq + k; product v(q).k into slot 2 + s; q += b dt dq
         StageParams S = stage_params(h);

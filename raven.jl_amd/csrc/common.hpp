@@ -81,6 +81,25 @@ struct StageParams {
     const void RV_GLOBAL *mdelta;
 };
 
+// Product slots (StageParams::eprod) the 3-D column kernel's relaxation stage
+// writes, one per wave launched: Nq = 5, 6, 7 pack 64 G / Nq^2 cells into a
+// workgroup of G = 2, 4, 4 waves, the others give a wave 64 / Nq^2 cells and a
+// workgroup wpb waves.
+inline int64_t plane3_relax_slots(int nq, int64_t K, int wpb)
+{
+    const int nq2 = nq * nq;
+    const int g = nq == 5 ? 2 : (nq == 6 || nq == 7) ? 4 : 1;
+    if (g > 1) {
+        const int cpg = 64 * g / nq2;
+        return (K + cpg - 1) / cpg * g;
+    }
+    if (wpb != 1 && wpb != 2 && wpb != 4)
+        wpb = 1;
+    const int epw = 64 / nq2;
+    const int64_t waves = (K + epw - 1) / epw;
+    return (waves + wpb - 1) / wpb * wpb;
+}
+
 // Launch the fused stage for one kernel configuration; returns false when no
 // kernel is instantiated for it.
 using StageLauncher = int (*)(const StageParams &, void *stream, int waves_per_block);
@@ -139,6 +158,14 @@ struct RelaxParams {
     double *partial;      // 4 * RELAX_BLOCKS doubles
 };
 int launch_relax(const RelaxParams &P, void *stream);
+
+// Compensated sums of per-workgroup partials on the device, in a fixed order
+// (deterministic): for slot a in [0, nslots) and b in [0, nwhich),
+//   out[a * nwhich + b] = sum over i in [0, n) of
+//       src[a * slot_stride + b * which_stride + (i / group) * group_stride + i % group].
+int launch_slot_sums(const double *src, int64_t n, int nslots, int nwhich, int64_t slot_stride,
+                     int64_t which_stride, int group, int64_t group_stride, double *out,
+                     void *stream);
 
 // Batched stages (batch.hip): one work-list item per wave's worth of cells of
 // one member; `kind` selects the member's (Nq, law, precision) routine.

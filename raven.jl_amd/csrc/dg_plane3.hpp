@@ -29,6 +29,7 @@
 #pragma once
 
 #include "dg_kernels.hpp"
+#include "entropy.hpp"
 
 namespace rv {
 
@@ -116,8 +117,13 @@ __device__ inline void load_neighbour(const StageParams &P, const R *qbase, cons
 // lane), working in the wave's own LDS slabs `wlds` (EPW * SLAB values); with
 // GW > 1, the cells [cell0, cell0 + CPG) of a workgroup of GW waves (the
 // workgroup's first 64 GW threads) and their CPG slabs.
-template <int NQ, int LAW, typename R, int GEO, int FLUX, int ST = 0, int GW = 1>
-__device__ __attribute__((always_inline)) inline void plane3_wave(const StageParams &P, int64_t cell0, R *wlds)
+// RX = 1: the fused relaxation stage (rv_rlsrk_step).  The accumulators form
+// k = L(q_in) alone; the store loop then makes dq = a dq_old + k and
+// q_out = q_in + b_dt dq as usual, and the wave's sum of MJ v(q_in) . k
+// (TwoSum-compensated, v the entropy variables) goes to P.eprod[eslot].
+template <int NQ, int LAW, typename R, int GEO, int FLUX, int ST = 0, int GW = 1, int RX = 0>
+__device__ __attribute__((always_inline)) inline void plane3_wave(const StageParams &P, int64_t cell0, R *wlds,
+                                                                  int64_t eslot = 0)
 {
     using T = Plane3<NQ, LAW, R, GEO, FLUX>;
     using S = typename T::S;
@@ -157,11 +163,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     // stage): their slot in the mortar lift buffer and which faces those are
     int mli = -1;
     unsigned mmask = 0;
+    double prod = 0.0, pc = 0.0;  // RX: the lane's MJ v(q) . k and its compensation
     const R a = R(P.a), bdt = R(P.b_dt);
     // a*dq_old is formed at the first z scatter so that the dq loads need not
     // complete before the first plane's flux; general cells accumulate
     // MJ * dq (the final update multiplies by MJI), so their factor is a*MJ
-    const bool late_scale = P.read_dq && dz_on;
+    // the relaxation stage accumulates k alone and reads dq_old at the store
+    const bool rdq = RX ? false : P.read_dq != 0;
+    const bool late_scale = rdq && dz_on;
     R zscale[NQ];  // general cells: a * MJ of the column's nodes
 
     // the z-derivative weights M[kk*Nq + k], one per lane, broadcast with
@@ -205,7 +214,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int s = 0; s < NS; ++s)
                 q[s][k] = qe[s * NP + k * NQ2];
         const R *dqe = static_cast<const R *>(P.dq) + e * (NS * NP) + l;
-        if (P.read_dq) {
+        if (rdq) {
 #pragma unroll
             for (int k = 0; k < NQ; ++k)
 #pragma unroll
@@ -602,6 +611,45 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
 #pragma unroll
         for (int k = 0; k < NQ; ++k)
             mjiv[k] = GEO == GEO_GENERAL ? R(P.mji[e * NP + l + k * NQ2]) : R(1);
+        if constexpr (RX) {
+            const R *dqe = static_cast<const R *>(P.dq) + e * (NS * NP) + l;
+            const double *mje = P.mj + e * NP + l;
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) {
+                const R mji = mjiv[k];
+                // dq_old and MJ of the plane, requested before its sums
+                R dcur[NS];
+                if (P.read_dq) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        dcur[s] = load_dq<ST>(dqe + s * NP + k * NQ2);
+                }
+                const double mjk = mje[k * NQ2];
+                R kv[NS];
+                double x[NS], kk[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    R v = acc[s][k];
+                    v = v - xs[s * NQ2 + NQ * k];
+                    v = v - ys[s * NQ2 + NQ * k];
+                    if constexpr (GEO == GEO_GENERAL)
+                        v = mji * v;
+                    kv[s] = v;
+                    x[s] = (double)q[s][k];
+                    kk[s] = (double)v;
+                }
+                comp_add(prod, pc, mjk * law_entropy_dot_3d<LAW>(x, kk, P.gamma), 0.0);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    R v = kv[s];
+                    if (P.read_dq)
+                        v = a * dcur[s] + v;
+                    store_out<ST>(v, dqo + s * NP + k * NQ2);
+                    store_out<ST>(R(q[s][k] + bdt * v), qo + s * NP + k * NQ2);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
             const R mji = mjiv[k];
@@ -619,30 +667,45 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             }
             __builtin_amdgcn_sched_barrier(0);
         }
+        }
+    }
+    if constexpr (RX) {
+        // the wave's sum: a butterfly of (sum, compensation) pairs, every
+        // lane forming the same total
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double os = __shfl_xor(prod, off), oc = __shfl_xor(pc, off);
+            comp_add(prod, pc, os, oc);
+        }
+        if ((threadIdx.x & 63) == 0)
+            P.eprod[eslot] = prod + pc;
     }
 }
 
-template <int NQ, int LAW, typename R, int GEO, int FLUX, int ST>
+template <int NQ, int LAW, typename R, int GEO, int FLUX, int ST, int RX = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 dg_plane3_kernel(const StageParams P)
 {
     using T = Plane3<NQ, LAW, R, GEO, FLUX>;
     extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
     const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int wave = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    // RX: one product slot per wave of the launch
+    const int64_t eslot = blk * wpb + wave;
     if constexpr (T::G > 1) {
         // one workgroup of G waves packs CPG cells
-        plane3_wave<NQ, LAW, R, GEO, FLUX, ST, T::G>(P, blk * T::CPG,
-                                                     reinterpret_cast<R *>(dg_lds));
+        plane3_wave<NQ, LAW, R, GEO, FLUX, ST, T::G, RX>(P, blk * T::CPG,
+                                                         reinterpret_cast<R *>(dg_lds), eslot);
     } else {
-        const int wave = threadIdx.x >> 6;
-        const int wpb = blockDim.x >> 6;
         R *wlds = reinterpret_cast<R *>(dg_lds) + wave * T::EPW * T::SLAB;
-        plane3_wave<NQ, LAW, R, GEO, FLUX, ST>(P, (blk * wpb + wave) * T::EPW, wlds);
+        plane3_wave<NQ, LAW, R, GEO, FLUX, ST, 1, RX>(P, (blk * wpb + wave) * T::EPW, wlds,
+                                                      eslot);
     }
 }
 
-template <int NQ, int LAW, typename R, int GEO, int FLUX>
-int launch_plane3(const StageParams &P, void *stream, int wpb)
+template <int NQ, int LAW, typename R, int GEO, int FLUX, int RX>
+int launch_plane3_rx(const StageParams &P, void *stream, int wpb)
 {
     using T = Plane3<NQ, LAW, R, GEO, FLUX>;
     if (P.K <= 0)
@@ -662,12 +725,25 @@ int launch_plane3(const StageParams &P, void *stream, int wpb)
         lds = (size_t)wpb * T::EPW * T::SLAB * sizeof(R);
     }
     if (P.streaming)
-        hipLaunchKernelGGL((dg_plane3_kernel<NQ, LAW, R, GEO, FLUX, 1>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((dg_plane3_kernel<NQ, LAW, R, GEO, FLUX, 1, RX>), dim3((unsigned)blocks),
                            dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);
     else
-        hipLaunchKernelGGL((dg_plane3_kernel<NQ, LAW, R, GEO, FLUX, 0>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((dg_plane3_kernel<NQ, LAW, R, GEO, FLUX, 0, RX>), dim3((unsigned)blocks),
                            dim3(64 * wpb), lds, static_cast<hipStream_t>(stream), P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+template <int NQ, int LAW, typename R, int GEO, int FLUX>
+int launch_plane3(const StageParams &P, void *stream, int wpb)
+{
+    return launch_plane3_rx<NQ, LAW, R, GEO, FLUX, 0>(P, stream, wpb);
+}
+
+template <int NQ, int LAW, typename R, int GEO, int FLUX>
+int launch_plane3_relax(const StageParams &P, void *stream, int wpb)
+{
+    return launch_plane3_rx<NQ, LAW, R, GEO, FLUX, 1>(P, stream, wpb);
+}
+
 
 }  // namespace rv

@@ -184,6 +184,48 @@ int launch_real(const RelaxParams &P, hipStream_t st)
 
 }  // namespace
 
+// One workgroup per (slot, which): each thread sums its strided share in
+// index order, then a fixed tree over the threads; (sum, compensation) pairs
+// combined by TwoSum throughout.
+__global__ void __launch_bounds__(256) slot_sums_kernel(const double *__restrict__ src, int64_t n,
+                                                        int64_t slot_stride, int64_t which_stride,
+                                                        int group, int64_t group_stride,
+                                                        double *__restrict__ out)
+{
+    __shared__ double red[2 * 256];
+    const int t = threadIdx.x;
+    const double *base = src + blockIdx.x * slot_stride + blockIdx.y * which_stride;
+    double s = 0.0, c = 0.0;
+    for (int64_t i = t; i < n; i += 256)
+        comp_add(s, c, base[(i / group) * group_stride + i % group], 0.0);
+    red[2 * t] = s;
+    red[2 * t + 1] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            double a = red[2 * t], ac = red[2 * t + 1];
+            comp_add(a, ac, red[2 * (t + w)], red[2 * (t + w) + 1]);
+            red[2 * t] = a;
+            red[2 * t + 1] = ac;
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        out[blockIdx.x * gridDim.y + blockIdx.y] = red[0] + red[1];
+}
+
+int launch_slot_sums(const double *src, int64_t n, int nslots, int nwhich, int64_t slot_stride,
+                     int64_t which_stride, int group, int64_t group_stride, double *out,
+                     void *stream)
+{
+    if (nslots <= 0 || nwhich <= 0)
+        return 0;
+    hipLaunchKernelGGL(slot_sums_kernel, dim3((unsigned)nslots, (unsigned)nwhich), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), src, n, slot_stride, which_stride,
+                       group < 1 ? 1 : group, group_stride, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_relax(const RelaxParams &P, void *stream)
 {
     if (P.K <= 0 && P.mode == RELAX_AXPY)

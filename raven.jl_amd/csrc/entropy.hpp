@@ -9,6 +9,17 @@
 
 namespace rv {
 
+// s + c += s2 + c2 with the rounding error of s + s2 (Knuth's TwoSum, exact
+// and independent of the operands' order) added to the compensation
+__device__ inline void comp_add(double &s, double &c, double s2, double c2)
+{
+    const double t = s + s2;
+    const double bb = t - s;
+    const double err = (s - (t - bb)) + (s2 - bb);
+    s = t;
+    c = (c + c2) + err;
+}
+
 // entropy(law, q): advection q^2 / 2; Euler -rho log(p / rho^gamma) / (gamma - 1)
 template <int LAW, int DIM>
 __device__ inline double law_entropy(const double *q, double g)
@@ -54,6 +65,26 @@ __device__ inline double law_entropy_dot(const double *q, const double *k, doubl
         for (int d = 0; d < DIM; ++d)
             r = r + (2 * b * u[d]) * k[1 + d];
         return r + (-2 * b) * k[DIM + 1];
+    }
+}
+
+// law_entropy_dot for 3-D states with s = log p - gamma log rho (two
+// logarithms, no power): the same quantity to rounding, in fewer registers,
+// for the column kernel's fused relaxation stage
+template <int LAW>
+__device__ inline double law_entropy_dot_3d(const double *q, const double *k, double g)
+{
+    if constexpr (LAW == LAW_ADVECTION) {
+        return q[0] * k[0];
+    } else {
+        const double rinv = 1.0 / q[0];
+        const double u0 = q[1] * rinv, u1 = q[2] * rinv, u2 = q[3] * rinv;
+        const double uu = u0 * u0 + u1 * u1 + u2 * u2;
+        const double p = (g - 1) * (q[4] - 0.5 * q[0] * uu);
+        const double s = log(p) - g * log(q[0]);
+        const double b = 0.5 * q[0] / p;
+        return ((g - s) / (g - 1) - uu * b) * k[0] + 2 * b * (u0 * k[1] + u1 * k[2] + u2 * k[3]) -
+               2 * b * k[4];
     }
 }
 

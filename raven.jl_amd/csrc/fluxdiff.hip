@@ -37,17 +37,6 @@
 
 namespace rv {
 
-// s + c += s2 + c2 with the rounding error of s + s2 (Knuth's TwoSum, exact
-// and independent of the operands' order) added to the compensation
-__device__ inline void comp_add(double &s, double &c, double s2, double c2)
-{
-    const double t = s + s2;
-    const double bb = t - s;
-    const double err = (s - (t - bb)) + (s2 - bb);
-    s = t;
-    c = (c + c2) + err;
-}
-
 namespace {
 
 template <int DIM, int NS, int LAW, int VF, typename R>

@@ -16,16 +16,21 @@ namespace rv {
 namespace {
 
 template <int NQ, int GEO, int FLUX>
-StageLauncher pick_one()
+StageLauncher pick_one(bool relax)
 {
     constexpr int DIM = INST_DIM, LAW = INST_LAW;
     using R = INST_REAL;
-    if constexpr (DIM == 3)
-        return &launch_plane3<NQ, LAW, R, GEO, FLUX>;
-    else if constexpr (DIM == 2 && GEO == GEO_AFFINE)
-        return &launch_quad2<NQ, LAW, R, FLUX>;
-    else
-        return &launch_stage<DIM, NQ, LAW, R, GEO, FLUX>;
+    if constexpr (DIM == 3) {
+        return relax ? &launch_plane3_relax<NQ, LAW, R, GEO, FLUX>
+                     : &launch_plane3<NQ, LAW, R, GEO, FLUX>;
+    } else {
+        if (relax)
+            return nullptr;  // the fused relaxation stage is built for 3-D
+        if constexpr (DIM == 2 && GEO == GEO_AFFINE)
+            return &launch_quad2<NQ, LAW, R, FLUX>;
+        else
+            return &launch_stage<DIM, NQ, LAW, R, GEO, FLUX>;
+    }
 }
 
 template <int NQ>
@@ -36,11 +41,12 @@ StageLauncher pick(const KernelKey &k)
     } else {
         if (k.flux != FLUX_RUSANOV && k.flux != FLUX_CENTRAL)
             return nullptr;
+        const bool rx = k.relax != 0;
         if (k.geo == GEO_AFFINE)
-            return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_AFFINE, FLUX_RUSANOV>()
-                                          : pick_one<NQ, GEO_AFFINE, FLUX_CENTRAL>();
-        return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_GENERAL, FLUX_RUSANOV>()
-                                      : pick_one<NQ, GEO_GENERAL, FLUX_CENTRAL>();
+            return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_AFFINE, FLUX_RUSANOV>(rx)
+                                          : pick_one<NQ, GEO_AFFINE, FLUX_CENTRAL>(rx);
+        return k.flux == FLUX_RUSANOV ? pick_one<NQ, GEO_GENERAL, FLUX_RUSANOV>(rx)
+                                      : pick_one<NQ, GEO_GENERAL, FLUX_CENTRAL>(rx);
     }
 }
 
@@ -49,7 +55,7 @@ StageLauncher pick(const KernelKey &k)
 StageLauncher INST_FN(const KernelKey &k)
 {
     if (k.dim != INST_DIM || k.law != INST_LAW || k.real_bytes != (int)sizeof(INST_REAL) ||
-        k.relax)
+        k.vflux != 0)
         return nullptr;
     switch (k.nq) {
     case 2: return pick<2>(k);
