@@ -477,7 +477,11 @@ __device__ __forceinline__ void fd_stage(const StageParams &P)
             if (P.read_dq)
                 acc[s] = a * dq[(e * NS + s) * NP + node] + acc[s];
         }
-        double prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
+        double prod = 0.0;
+        if constexpr (DIM == 3)
+            prod = active ? MJ1 * law_entropy_dot_3d<LAW>(x, kk, P.gamma) : 0.0;
+        else
+            prod = active ? MJ1 * law_entropy_dot<LAW, DIM>(x, kk, P.gamma) : 0.0;
         double pc = 0.0;  // compensation of prod
         // the workgroup's sum, compensated: a fixed tree over the threads of
         // (sum, compensation) pairs, each addition's rounding error (TwoSum,
