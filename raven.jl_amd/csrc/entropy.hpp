@@ -32,7 +32,8 @@ __device__ inline double law_entropy(const double *q, double g)
         for (int d = 1; d < DIM; ++d)
             dot = dot + q[1 + d] * q[1 + d];
         const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
-        const double s = log(p / pow(q[0], g));
+        // log(p / rho^gamma) as two logarithms (no power)
+        const double s = log(p) - g * log(q[0]);
         return -q[0] * s / (g - 1);
     }
 }
@@ -50,7 +51,7 @@ __device__ inline double law_entropy_dot(const double *q, const double *k, doubl
         for (int d = 1; d < DIM; ++d)
             dot = dot + q[1 + d] * q[1 + d];
         const double p = (g - 1) * (q[DIM + 1] - dot / (2 * q[0]));
-        const double s = log(p / pow(q[0], g));
+        const double s = log(p) - g * log(q[0]);
         const double b = q[0] / (2 * p);
         double u[DIM], uu;
 #pragma unroll
@@ -68,9 +69,8 @@ __device__ inline double law_entropy_dot(const double *q, const double *k, doubl
     }
 }
 
-// law_entropy_dot for 3-D states with s = log p - gamma log rho (two
-// logarithms, no power): the same quantity to rounding, in fewer registers,
-// for the column kernel's fused relaxation stage
+// law_entropy_dot for 3-D states, written for fewer registers (the column
+// kernel's and the 3-D flux-differencing kernel's fused relaxation stages)
 template <int LAW>
 __device__ inline double law_entropy_dot_3d(const double *q, const double *k, double g)
 {
