@@ -682,6 +682,62 @@ def test_mortar_stage_with_flux_differencing_matches_oracle(torch, R):
     assert e1 <= RTOL_F64 and e2 <= RTOL_F64
 
 
+def _warp10_2d(x, y):
+    """A smooth warp of [0,10]^2 (periodic in each direction)."""
+    k = 2 * np.pi / 10
+    return x + 0.3 * np.sin(k * y), y + 0.25 * np.sin(k * x)
+
+
+@pytest.mark.parametrize("nq", [5, 7])
+def test_mortar_stage_2d_matches_oracle(torch, R, nq):
+    """2-D warped 2:1 forest with mortar-coupled hanging faces: the 2-D stage
+    kernel, then the mortar face kernel's lifts added by mortar_apply_kernel
+    (the path of every stage kernel other than the 3-D column kernel), against
+    the oracle's right-hand side plus its mortar restatement, FP64; and
+    discrete conservation of the coupled right-hand side."""
+    grid = _adapted_grid(R, 2, nq, 6, 2, 2.1, _warp10_2d)
+    hanging = int(np.sum(grid.face_kind == 1))
+    say(f"   2-D adapted grid: K={grid.K}, hanging faces {hanging}")
+    assert hanging > 0
+    dg = R.DGSEM(law=R.EulerLaw(2), grid=grid, surface_numericalflux=R.RusanovFlux())
+    dg.enable_mortars(grid)
+    K, nfp = grid.K, nq
+    ag = _ArrayGrid(grid)
+    ag.points = grid.points
+    surf = np.array(grid.surface).reshape(K, 4, nfp, 4)
+    surf_full = surf.reshape(-1).copy()
+    surf[grid.face_kind != 0, :, 3] = 0.0
+    ag.surf = surf.reshape(-1)
+    odg = O.DG(ag, "euler")
+    lower, upper = np.zeros(nq * nq), np.zeros(nq * nq)
+    R._lib.check(R.lib().rv_lobatto_halves(nq, R._lib.ptr(lower, R._lib.ctypes.c_double),
+                                           R._lib.ptr(upper, R._lib.ctypes.c_double)))
+    rng = np.random.default_rng(14)
+    q = PB.isentropic_vortex_2d(grid.points, 0.0)
+    q = q * (1 + 1e-3 * rng.uniform(-1, 1, q.size))
+    dq0 = 1e-3 * rng.standard_normal(q.size)
+    a, bdt = -0.6, 2e-3
+    L_o = np.zeros(q.size)
+    odg.rhs(L_o, q.copy(), 0.0, increment=False)
+    odg.mortar_add(grid.face_kind, grid.mortar_partner, lower, upper, surf_full, q, L_o)
+    want_dq = a * dq0 + L_o
+    want_q = q + bdt * want_dq
+    qd, dqd = to_dev(torch, dg, q), to_dev(torch, dg, dq0)
+    qo = torch.empty_like(qd)
+    dg.lsrk_stage(qo, qd, dqd, a, bdt)
+    Ld = dg.GridArray(0.0)
+    dg(Ld, qd, 0.0, increment=False)
+    dg.synchronize()
+    e1 = rel_l2(dqd.cpu().numpy().reshape(-1), want_dq)
+    e2 = rel_l2(qo.cpu().numpy().reshape(-1), want_q)
+    e3 = rel_l2(Ld.cpu().numpy().reshape(-1), L_o)
+    say(f"   2-D mortar Nq={nq}: dq {e1:.2e}, q_out {e2:.2e}, rhs {e3:.2e}")
+    assert e1 <= RTOL_F64 and e2 <= RTOL_F64 and e3 <= RTOL_F64
+    tot = dg.integrals(Ld)
+    scale = dg.integrals(torch.abs(Ld).contiguous())
+    assert np.all(np.abs(tot) <= 1e-11 * scale)
+
+
 @pytest.mark.parametrize("dim,nq", [(2, 5), (3, 5), (3, 8), (1, 6)])
 def test_min_node_distance_matches_oracle(torch, R, dim, nq):
     """min_node_distance(grid; dims) (grids.jl:94-145) on the warped bricks of
