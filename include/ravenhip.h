@@ -177,9 +177,13 @@ int rv_get_waves_per_block(const rv_handle *h);
  * is interpolated with tohalves_1d (lobattocells.jl:15-16), the numerical flux
  * is evaluated with the fine face's normal and weight, and the coarse cell
  * receives the transpose projection, so the scheme stays conservative.  Every
- * rv_rhs / rv_lsrk_stage / rv_lsrk_run then runs one more kernel after the
- * stage kernel.  Needs general geometry.  No counterpart in Raven's RHS, which
- * builds these structures but does not read them. */
+ * rv_rhs / rv_lsrk_stage / rv_lsrk_run then runs one more kernel before the
+ * stage kernel (the non-conforming faces' lifts, into a device buffer of
+ * (cells with such faces) x 2dim x Ns x Nq^(dim-1) values this call allocates)
+ * and, except with the 3-D weak form, whose stage kernel takes those lifts
+ * itself, one after it that adds them.  Needs general geometry.  No
+ * counterpart in Raven's RHS, which builds these structures but does not read
+ * them. */
 int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *partner);
 
 /* tohalves_1d (lobattocells.jl:15-16): row-major [a][b] = l_b((x_a -+ 1)/2),
