@@ -2496,7 +2496,9 @@ struct rv_batch {
     int tail_mbegin = 0;                   // position in `order` of its first member
     std::vector<int> order;                // members sorted by (small kind, kind)
     std::vector<int64_t> mfirst, mcount;   // each member's items
-    std::vector<int> kind, epw, slab;      // each member's kind, cells per item, LDS slab
+    // each member's kind, and cells and LDS bytes per item in the work-list
+    // launch (epw, slab) and when its kind launches on its own (epw_own, slab_own)
+    std::vector<int> kind, epw, slab, epw_own, slab_own;
 };
 
 namespace {
@@ -2556,10 +2558,10 @@ void free_batch(rv_batch *b)
 void batch_layout(rv_batch *b, std::vector<BatchItem> &items)
 {
     const int n = (int)b->members.size();
-    const std::vector<int> &kind = b->kind, &epw = b->epw, &slab = b->slab;
+    const std::vector<int> &kind = b->kind, &slab = b->slab;
     std::map<int, int64_t> kind_items;
     for (int m = 0; m < n; ++m)
-        kind_items[kind[m]] += (b->members[m]->K + epw[m] - 1) / epw[m];
+        kind_items[kind[m]] += (b->members[m]->K + b->epw[m] - 1) / b->epw[m];
     std::vector<char> small(n);
     for (int m = 0; m < n; ++m)
         small[m] = b->dim != 3 && kind_items[kind[m]] < b->own_items;
@@ -2586,7 +2588,10 @@ void batch_layout(rv_batch *b, std::vector<BatchItem> &items)
         if (small[m])
             b->slab_bytes = std::max(b->slab_bytes, slab[m]);
         b->mfirst[m] = (int64_t)items.size();
-        for (int64_t c = 0; c < b->members[m]->K; c += epw[m])
+        // a kind launching on its own takes its own item size (packed
+        // workgroups for 2-D Nq = 5, 6, 7)
+        const int ep = small[m] ? b->epw[m] : b->epw_own[m];
+        for (int64_t c = 0; c < b->members[m]->K; c += ep)
             items.push_back(BatchItem{m, kind[m], c});
         b->mcount[m] = (int64_t)items.size() - b->mfirst[m];
     }
@@ -2598,7 +2603,7 @@ void batch_layout(rv_batch *b, std::vector<BatchItem> &items)
     for (int p = 0; p < b->tail_mbegin; ++p) {
         const int m = order[p];
         if (p == 0 || kind[m] != kind[order[p - 1]])
-            b->kind_ranges.push_back({kind[m], slab[m], b->members[m]->wpb, p, p + 1});
+            b->kind_ranges.push_back({kind[m], b->slab_own[m], b->members[m]->wpb, p, p + 1});
         else
             b->kind_ranges.back().mend = p + 1;
     }
@@ -2676,7 +2681,7 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
         return bfail(nullptr, RV_E_ARG, "a batch needs at least one member");
     rv_batch *b = new rv_batch();
     b->members.assign(members, members + n);
-    std::vector<int> kind(n), epw(n), slab(n);
+    std::vector<int> kind(n), epw(n), slab(n), epw_own(n), slab_own(n);
     for (int m = 0; m < n; ++m) {
         rv_handle *h = members[m];
         if (!h) {
@@ -2698,15 +2703,20 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
                          "batch members need affine geometry, the weak form, the Rusanov flux "
                          "and all directions");
         }
-        int sb = 0;
-        if (!batch_kind(h->dim, h->nq, h->law, h->dtype == RV_F32 ? 4 : 8, &kind[m], &epw[m], &sb)) {
+        int ep[2] = {0, 0}, sb[2] = {0, 0};
+        if (!batch_kind(h->dim, h->nq, h->law, h->dtype == RV_F32 ? 4 : 8, &kind[m], ep, sb)) {
             free_batch(b);
             return bfail(nullptr, RV_E_UNSUPPORTED, "no batch kernel for this (dim, Nq)");
         }
-        slab[m] = (sb + 15) / 16 * 16;
+        epw[m] = ep[0];
+        slab[m] = (sb[0] + 15) / 16 * 16;
+        epw_own[m] = ep[1];
+        slab_own[m] = (sb[1] + 15) / 16 * 16;
     }
     b->kind = kind;
     b->epw = epw;
+    b->epw_own = epw_own;
+    b->slab_own = slab_own;
     b->slab = slab;
     std::vector<BatchItem> items;
     batch_layout(b, items);
@@ -2715,7 +2725,12 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
         return bfail(nullptr, RV_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
     };
     hipError_t e;
-    if ((e = hipMalloc(&b->items, std::max<size_t>(1, items.size()) * sizeof(BatchItem))))
+    // room for the longest work list any layout gives: every member in
+    // work-list items (the smallest items)
+    size_t cap = 1;
+    for (int m = 0; m < n; ++m)
+        cap += (size_t)((members[m]->K + epw[m] - 1) / epw[m]);
+    if ((e = hipMalloc(&b->items, cap * sizeof(BatchItem))))
         return bail("hipMalloc", e);
     if ((e = hipMemcpy(b->items, items.data(), items.size() * sizeof(BatchItem),
                        hipMemcpyHostToDevice)))
@@ -2728,7 +2743,7 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
     for (auto &ev : b->ring_ev)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)))
             return bail("hipEventCreate", e);
-    b->device_bytes = (int64_t)(items.size() * sizeof(BatchItem) + sizeof(StageParams) * ring_entries);
+    b->device_bytes = (int64_t)(cap * sizeof(BatchItem) + sizeof(StageParams) * ring_entries);
     if (batch_waves(b) != RV_OK) {
         free_batch(b);
         return bfail(nullptr, RV_E_HIP, "occupancy query of the batch kernel failed");

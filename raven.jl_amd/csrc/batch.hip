@@ -34,6 +34,15 @@ struct BatchRange {
     static constexpr int NQ_MAX = 8;
 };
 
+// Waves of one item of a kind launching on its own: 2-D Nq = 5, 6, 7 pack
+// their cells into a workgroup of G waves, as the single-operator kernel does;
+// the work-list kernel (and every other kind) runs one wave per item.
+template <int DIM, int NQ, int LAW, typename R>
+constexpr int kind_group_waves()
+{
+    return DIM == 2 ? quad2_group_waves<NQ>() : 1;
+}
+
 // LDS of one wave for a kind: the single-operator kernel's slab, in elements
 template <int DIM, int NQ, int LAW, typename R>
 constexpr int kind_slab_elems()
@@ -52,6 +61,11 @@ constexpr int kind_slab_bytes()
 {
     return (int)((kind_slab_elems<DIM, NQ, LAW, R>() * sizeof(R) + 15) / 16 * 16);
 }
+
+// A packed 2-D kind's group: cells and LDS bytes of one workgroup
+template <int NQ, int LAW, typename R>
+using PackedQuad2 = Quad2<NQ, LAW, R, FLUX_RUSANOV, quad2_group_waves<NQ>(),
+                          quad2_nodes_per_lane<LAW, R>()>;
 
 template <int DIM, int NQ, int LAW, typename R>
 __device__ __attribute__((always_inline)) inline void run_item(const StageParams &P, int64_t cell0, unsigned char *wl)
@@ -128,6 +142,20 @@ dg_batch_kind_kernel(const StageParams *__restrict__ members, const BatchItem *_
                      int64_t first, int64_t count, int slab_bytes)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dg_lds[];
+    if constexpr (kind_group_waves<DIM, NQ, LAW, R>() > 1) {
+        // one packed item per workgroup (the whole workgroup returns or runs it)
+        constexpr int G = kind_group_waves<DIM, NQ, LAW, R>();
+        const int64_t k = xcd_remap(blockIdx.x, gridDim.x);
+        (void)slab_bytes;
+        if (k >= count)
+            return;
+        const BatchItem w = items[first + k];
+        R *gl = reinterpret_cast<R *>(dg_lds);
+        quad2_clear_spare(gl, (int)threadIdx.x);
+        quad2_group<NQ, LAW, R, FLUX_RUSANOV, 0, G, quad2_nodes_per_lane<LAW, R>()>(
+            members[w.member], w.cell0, gl, (int)threadIdx.x);
+        return;
+    }
     const int wave = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
     const int64_t k = wave_uniform(xcd_remap(blockIdx.x, gridDim.x) * wpb + wave);
@@ -144,29 +172,40 @@ dg_batch_kind_kernel(const StageParams *__restrict__ members, const BatchItem *_
     run_item<DIM, NQ, LAW, R>(members[w.member], w.cell0, reinterpret_cast<unsigned char *>(wl));
 }
 
+// slab_bytes: a wave's slab, or a packed kind's whole workgroup's
+template <int DIM, int NQ, int LAW, typename R>
+void launch_kind_one(const StageParams *members, const BatchItem *items, int64_t first,
+                     int64_t count, int slab_bytes, int wpb, hipStream_t st)
+{
+    constexpr int G = kind_group_waves<DIM, NQ, LAW, R>();
+    if (G > 1)
+        wpb = G;
+    const int64_t blocks = G > 1 ? count : (count + wpb - 1) / wpb;
+    const size_t lds = G > 1 ? (size_t)slab_bytes : (size_t)wpb * slab_bytes;
+    hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW, R>), dim3((unsigned)blocks),
+                       dim3(64 * wpb), lds, st, members, items, first, count, slab_bytes);
+}
+
 template <int DIM, int NQ>
 int launch_kind_nq(int sub, const StageParams *members, const BatchItem *items, int64_t first,
                    int64_t count, int slab_bytes, int wpb, hipStream_t st)
 {
-    const int64_t blocks = (count + wpb - 1) / wpb;
-    const size_t lds = (size_t)wpb * slab_bytes;
-    const dim3 g((unsigned)blocks), b(64 * wpb);
     switch (sub) {
     case 0:
-        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_ADVECTION, double>), g, b, lds, st,
-                           members, items, first, count, slab_bytes);
+        launch_kind_one<DIM, NQ, LAW_ADVECTION, double>(members, items, first, count, slab_bytes,
+                                                        wpb, st);
         break;
     case 1:
-        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_ADVECTION, float>), g, b, lds, st,
-                           members, items, first, count, slab_bytes);
+        launch_kind_one<DIM, NQ, LAW_ADVECTION, float>(members, items, first, count, slab_bytes,
+                                                       wpb, st);
         break;
     case 2:
-        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_EULER, double>), g, b, lds, st,
-                           members, items, first, count, slab_bytes);
+        launch_kind_one<DIM, NQ, LAW_EULER, double>(members, items, first, count, slab_bytes, wpb,
+                                                    st);
         break;
     default:
-        hipLaunchKernelGGL((dg_batch_kind_kernel<DIM, NQ, LAW_EULER, float>), g, b, lds, st,
-                           members, items, first, count, slab_bytes);
+        launch_kind_one<DIM, NQ, LAW_EULER, float>(members, items, first, count, slab_bytes, wpb,
+                                                   st);
         break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -186,29 +225,38 @@ int launch_kind(int kind, const StageParams *members, const BatchItem *items, in
     }
 }
 
+// cells per item and LDS bytes per item: ep[0], sb[0] in the work-list
+// kernel, ep[1], sb[1] in a launch of the kind alone
 template <int DIM, int NQ, int LAW, typename R>
-void layout_of(int *epw, int *slab_bytes)
+void layout_of(int *ep, int *sb)
 {
     if constexpr (DIM == 3)
-        *epw = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
+        ep[0] = Plane3<NQ, LAW, R, GEO_AFFINE, FLUX_RUSANOV>::EPW;
     else
-        *epw = Quad2<NQ, LAW, R, FLUX_RUSANOV, 1, quad2_nodes_per_lane<LAW, R>()>::CELLS;
-    *slab_bytes = kind_slab_bytes<DIM, NQ, LAW, R>();
+        ep[0] = Quad2<NQ, LAW, R, FLUX_RUSANOV, 1, quad2_nodes_per_lane<LAW, R>()>::CELLS;
+    sb[0] = kind_slab_bytes<DIM, NQ, LAW, R>();
+    ep[1] = ep[0];
+    sb[1] = sb[0];
+    if constexpr (DIM == 2 && kind_group_waves<DIM, NQ, LAW, R>() > 1) {
+        using Q = PackedQuad2<NQ, LAW, R>;
+        ep[1] = Q::CELLS;
+        sb[1] = (int)((Q::SLAB * sizeof(R) + 15) / 16 * 16);
+    }
 }
 
 template <int DIM, int NQ>
-bool layout_nq(int nq, int sub, int *epw, int *slab_bytes)
+bool layout_nq(int nq, int sub, int *ep, int *sb)
 {
     if constexpr (NQ > BatchRange<DIM>::NQ_MAX) {
         return false;
     } else {
         if (nq != NQ)
-            return layout_nq<DIM, NQ + 1>(nq, sub, epw, slab_bytes);
+            return layout_nq<DIM, NQ + 1>(nq, sub, ep, sb);
         switch (sub) {
-        case 0: layout_of<DIM, NQ, LAW_ADVECTION, double>(epw, slab_bytes); break;
-        case 1: layout_of<DIM, NQ, LAW_ADVECTION, float>(epw, slab_bytes); break;
-        case 2: layout_of<DIM, NQ, LAW_EULER, double>(epw, slab_bytes); break;
-        default: layout_of<DIM, NQ, LAW_EULER, float>(epw, slab_bytes); break;
+        case 0: layout_of<DIM, NQ, LAW_ADVECTION, double>(ep, sb); break;
+        case 1: layout_of<DIM, NQ, LAW_ADVECTION, float>(ep, sb); break;
+        case 2: layout_of<DIM, NQ, LAW_EULER, double>(ep, sb); break;
+        default: layout_of<DIM, NQ, LAW_EULER, float>(ep, sb); break;
         }
         return true;
     }
@@ -222,7 +270,7 @@ const void *kernel_of()
 
 }  // namespace
 
-bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int *cells_per_wave,
+bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int *cells_per_item,
                 int *slab_bytes)
 {
     const int sub = (law == LAW_EULER ? 2 : 0) + (real_bytes == 4 ? 1 : 0);
@@ -232,10 +280,10 @@ bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int *cells_
         return false;
     bool ok = false;
     if (dim == 2) {
-        ok = layout_nq<2, BatchRange<2>::NQ_MIN>(nq, sub, cells_per_wave, slab_bytes);
+        ok = layout_nq<2, BatchRange<2>::NQ_MIN>(nq, sub, cells_per_item, slab_bytes);
         *kind = (nq - BatchRange<2>::NQ_MIN) * KINDS_PER_NQ + sub;
     } else if (dim == 3) {
-        ok = layout_nq<3, BatchRange<3>::NQ_MIN>(nq, sub, cells_per_wave, slab_bytes);
+        ok = layout_nq<3, BatchRange<3>::NQ_MIN>(nq, sub, cells_per_item, slab_bytes);
         *kind = (nq - BatchRange<3>::NQ_MIN) * KINDS_PER_NQ + sub;
     }
     return ok;

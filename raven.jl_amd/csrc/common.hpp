@@ -174,10 +174,12 @@ struct BatchItem {
     int32_t kind;
     int64_t cell0;
 };
-// kind id, cells per wave and LDS bytes per wave of a batch member; false when
-// the batch kernels do not cover (dim, nq, law, precision)
-bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int *cells_per_wave,
-                int *slab_bytes);
+// kind id, and cells and LDS bytes per item of a batch member: [0] in the
+// work-list kernel (a wave per item), [1] when its kind launches on its own
+// (2-D Nq = 5, 6, 7: a packed workgroup per item); false when the batch
+// kernels do not cover (dim, nq, law, precision)
+bool batch_kind(int dim, int nq, int law, int real_bytes, int *kind, int cells_per_item[2],
+                int slab_bytes[2]);
 // waves of the batch kernel resident on the whole device at once
 int batch_resident_waves(int dim, int wpb, int slab_bytes, int64_t *waves);
 // one persistent launch over the work list (members: device table)

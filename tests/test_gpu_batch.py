@@ -60,11 +60,15 @@ MIX2 = [(3, 6, "advection", "f64"), (4, 5, "euler", "f64"), (5, 7, "advection", 
 MIX3 = [(4, 3, "euler", "f64"), (5, 2, "advection", "f32"), (6, 2, "euler", "f32"),
         (8, 2, "euler", "f64"), (7, 3, "advection", "f64")]
 # kinds with enough items to launch on their own (two members of one of them)
-# beside small kinds that share the mixed launch
-# (a 2-D wave holds 64 / Nq cells: 96^2 and 92^2 cells at Nq = 8 give 1152 and
-# 1058 items, 130^2 at Nq = 4 gives 1057; 1024 items launch on their own)
+# beside small kinds that share the mixed launch (1024 work-list items launch
+# on their own by default).  A work-list item of the 2-D node kernel is one
+# wave: one cell at Nq = 8 in FP64 Euler (96^2 and 92^2 cells: 9216 and 8464
+# items), 16 cells at Nq = 4 in FP32 advection (130^2: 1057 items), one cell
+# at Nq = 6 in FP64 Euler (34^2: 1156 items), whose own launch packs 7 cells
+# into a workgroup of 4 waves instead: configuring every kind into the
+# work-list launch then lengthens the list.
 MIX2H = [(8, 96, "euler", "f64"), (3, 6, "advection", "f64"), (8, 92, "euler", "f64"),
-         (4, 130, "advection", "f32"), (5, 7, "advection", "f32")]
+         (4, 130, "advection", "f32"), (5, 7, "advection", "f32"), (6, 34, "euler", "f64")]
 # several members of one 3-D kind
 MIX3H = [(4, 3, "euler", "f64"), (5, 2, "advection", "f32"), (4, 4, "euler", "f64"),
          (5, 3, "advection", "f32")]
