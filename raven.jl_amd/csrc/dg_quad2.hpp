@@ -19,7 +19,8 @@
 //    spare, which leaves their sums unchanged);
 //  * Nq^2 = 25, 36, 49 leave lanes of a wave idle, so G = 2, 4, 4 waves
 //    share one workgroup's cells (5, 7, 5 cells over 128, 256, 256 lanes) and
-//    synchronise with workgroup barriers; the work-list kernels use G = 1;
+//    synchronise with workgroup barriers (so do batches launching a kind on
+//    its own); the batches' work-list kernel uses G = 1;
 //  * light cells (advection, FP32 Euler) give a lane V = 4 or 2 nodes, one in
 //    each of V sets of cells, all requested before the first wait: a wave
 //    then has V times the bytes in flight.
