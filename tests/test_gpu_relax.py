@@ -137,7 +137,8 @@ def test_two_point_surface_flux_rhs_matches_oracle(torch, R, dim, nq, law, flux,
 
 
 # fused (the stage kernel forms the entropy production): flux differencing
-# with its own flux on the faces; separate update passes: every other operator
+# with its own flux on the faces, and the 3-D weak form (affine and general
+# cells); separate update passes: every other operator
 RLSRK_CASES = [
     # law, dim, volume flux, surface flux, scheme, geometry
     ("euler", 2, "ec", "ec", 54, "auto"),
