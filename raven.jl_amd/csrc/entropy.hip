@@ -105,8 +105,10 @@ __global__ void __launch_bounds__(RELAX_THREADS) relax_kernel(const RelaxParams 
                 kk[s] = (double)ks;
             }
             const double mj = P.MJ[i];
-            eta = cadd(eta, mj * law_entropy<LAW, DIM>(x, g));
-            prod = cadd(prod, mj * law_entropy_dot<LAW, DIM>(x, kk, g));
+            double et, dt;
+            law_entropy_and_dot<LAW, DIM>(x, kk, g, et, dt);
+            eta = cadd(eta, mj * et);
+            prod = cadd(prod, mj * dt);
         } else {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {

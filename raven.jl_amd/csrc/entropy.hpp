@@ -69,6 +69,39 @@ __device__ inline double law_entropy_dot(const double *q, const double *k, doubl
     }
 }
 
+// law_entropy(q) and law_entropy_dot(q, k) together, the logarithms once
+template <int LAW, int DIM>
+__device__ inline void law_entropy_and_dot(const double *q, const double *k, double g,
+                                           double &eta, double &dot)
+{
+    if constexpr (LAW == LAW_ADVECTION) {
+        eta = q[0] * q[0] / 2;
+        dot = q[0] * k[0];
+    } else {
+        double uu2 = q[1] * q[1];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            uu2 = uu2 + q[1 + d] * q[1 + d];
+        const double p = (g - 1) * (q[DIM + 1] - uu2 / (2 * q[0]));
+        const double s = log(p) - g * log(q[0]);
+        eta = -q[0] * s / (g - 1);
+        const double b = q[0] / (2 * p);
+        double u[DIM], uu;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            u[d] = q[1 + d] / q[0];
+        uu = u[0] * u[0];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d)
+            uu = uu + u[d] * u[d];
+        double r = ((g - s) / (g - 1) - uu * b) * k[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            r = r + (2 * b * u[d]) * k[1 + d];
+        dot = r + (-2 * b) * k[DIM + 1];
+    }
+}
+
 // law_entropy_dot for 3-D states, written for fewer registers (the column
 // kernel's and the 3-D flux-differencing kernel's fused relaxation stages)
 template <int LAW>
