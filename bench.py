@@ -277,7 +277,8 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
     qs = [q.clone() for q in q0]
     dqs = [torch.empty_like(q) for q in q0]
     # launch layout chosen by timing (each kind's waves per block, the mixed
-    # launch's waves per block and the own-launch threshold)
+    # launch's waves per block, the own-launch threshold and the number of
+    # concurrent streams)
     tuned = batch.autotune(qs, dts)  # {layout: (median, min, max) ms per step}
     layout = min(tuned, key=lambda k: tuned[k][0])
     batch.lsrk_run(qs, dqs, dts, steps)  # builds and caches the step graphs
@@ -338,14 +339,16 @@ def run_batched(R, torch, n_inst=256, steps=10, seed=2024):
         "batched_achieved_GBs": gbs, "batched_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
         "work_items": info["items"], "persistent_waves": info["waves"],
         "launch": "per stage: one launch per kind large enough to fill the device, one "
-                  "persistent work-list launch for the small kinds; one hipGraph per step",
+                  "persistent work-list launch for the small kinds, these chains spread "
+                  "over concurrent streams; one hipGraph per step",
         "autotuned_layout": {"mixed_waves_per_block": layout[0],
                              "own_launch_items": layout[1],
-                             "step_ms_by_layout": {f"{w},{t}": round(v[0], 4)
-                                                    for (w, t), v in tuned.items()},
+                             "streams": layout[2],
+                             "step_ms_by_layout": {f"{w},{t},{s}": round(v[0], 4)
+                                                    for (w, t, s), v in tuned.items()},
                              "step_ms_spread_by_layout": {
-                                 f"{w},{t}": [round(v[1], 4), round(v[2], 4)]
-                                 for (w, t), v in tuned.items()},
+                                 f"{w},{t},{s}": [round(v[1], 4), round(v[2], 4)]
+                                 for (w, t, s), v in tuned.items()},
                              "timed_runs_per_layout": 3},
     }
 

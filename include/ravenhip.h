@@ -372,6 +372,16 @@ int rv_batch_info(const rv_batch *b, int64_t *nitems, int64_t *nwaves, int32_t *
  * Waits for the batch's stream; cached step graphs are rebuilt on next use.
  * Results do not depend on the layout. */
 int rv_batch_configure(rv_batch *b, int mixed_waves_per_block, int64_t own_launch_items);
+/* Concurrent streams (1 to 4, default 2) the launches of one call are spread
+ * over.  The launches of one kind, and those of the work-list launch, form a
+ * chain that touches no other member; chains go, heaviest first, to the
+ * batch's stream and up to three private streams forked from it and joined
+ * back with events, so a call stays ordered on the batch's stream and
+ * captured step graphs get parallel branches.  Waits for the batch's stream;
+ * cached step graphs are rebuilt on next use.  Results do not depend on it.
+ * No counterpart in the reference, which advances one solver per call
+ * (odesolvers.jl:9-30). */
+int rv_batch_set_streams(rv_batch *b, int32_t nstreams);
 /* One fused stage for every member m (arrays of nmembers entries):
  *   dq[m] = a[m]*dq[m] + L_m(q_in[m]);  q_out[m] = q_in[m] + b_dt[m]*dq[m]. */
 int rv_batch_stage(rv_batch *b, void *const *q_out, const void *const *q_in, void *const *dq,

@@ -2467,6 +2467,15 @@ struct rv_batch {
     int64_t nitems = 0, nwaves = 0;
     hipStream_t stream = nullptr;          // the first member's stream
     hipStream_t capture = nullptr;         // private stream graphs are captured on
+    // Launch chains (a kind's stages, or the work-list launch's) touch
+    // disjoint members and are spread over `lanes` streams: lane 0 is the
+    // calling stream, lanes 1.. private streams forked from it and joined back
+    // with events (captured step graphs get parallel branches the same way).
+    static constexpr int MAXLANES = 4;
+    int lanes = 2;
+    hipStream_t lane[MAXLANES] = {};       // [0] unused
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[MAXLANES] = {};
     std::vector<rv_handle *> members;
     BatchItem *items = nullptr;            // device work list
     StageParams *host_ring = nullptr;      // pinned, RING * n
@@ -2542,6 +2551,16 @@ void free_batch(rv_batch *b)
             (void)hipEventDestroy(ev);
     if (b->capture)
         (void)hipStreamDestroy(b->capture);
+    for (int l = 1; l < rv_batch::MAXLANES; ++l) {
+        if (b->lane[l]) {
+            (void)hipStreamSynchronize(b->lane[l]);
+            (void)hipStreamDestroy(b->lane[l]);
+        }
+        if (b->join_ev[l])
+            (void)hipEventDestroy(b->join_ev[l]);
+    }
+    if (b->fork_ev)
+        (void)hipEventDestroy(b->fork_ev);
     delete b;
 }
 
@@ -2622,49 +2641,101 @@ int batch_waves(rv_batch *b)
 // Stages 0..nst-1 of the batch, stage s's table at table + s * n.  active
 // (optional) marks the members that step.  Kind by kind, all stages of the
 // kind's stepping members run back to back (a member's state stays in the
-// caches from one stage to the next, as when it runs alone); then the mixed
-// tail runs each stage once, or not at all when none of its members steps
-// (the others' items return at once, their K being 0).
+// caches from one stage to the next, as when it runs alone); the work-list
+// launch runs each stage once, or not at all when none of its members steps
+// (the others' items return at once, their K being 0).  These chains touch
+// disjoint members: they go to the batch's lanes, heaviest first, each to
+// the lane with the fewest bytes so far, so that one kind's launches overlap
+// another's ramp and tail.
 int batch_launch(rv_batch *b, const StageParams *table, int nst, hipStream_t st,
                  const std::vector<char> *active = nullptr)
 {
-    int rc = 0;
     const size_t n = b->members.size();
-    for (const auto &r : b->kind_ranges) {
-        // runs of consecutive stepping members: [first item, count)
-        std::vector<std::pair<int64_t, int64_t>> runs;
+    struct Chain {
+        int range;                                      // kind_ranges index, -1: work list
+        std::vector<std::pair<int64_t, int64_t>> runs;  // [first item, count)
+        double bytes;
+    };
+    auto member_bytes = [&](int m) {
+        const rv_handle *h = b->members[m];
+        return (double)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
+    };
+    std::vector<Chain> chains;
+    for (size_t ri = 0; ri < b->kind_ranges.size(); ++ri) {
+        const auto &r = b->kind_ranges[ri];
+        Chain c{(int)ri, {}, 0.0};
+        // runs of consecutive stepping members
         int64_t first = -1, count = 0;
         for (int p = r.mbegin; p <= r.mend; ++p) {
             const int m = p < r.mend ? b->order[p] : -1;
             if (m < 0 || (active && !(*active)[m])) {
                 if (count > 0)
-                    runs.push_back({first, count});
+                    c.runs.push_back({first, count});
                 first = -1, count = 0;
                 continue;
             }
             if (first < 0)
                 first = b->mfirst[m];
             count += b->mcount[m];
+            c.bytes += member_bytes(m);
         }
-        for (int s = 0; s < nst && rc == 0; ++s)
-            for (const auto &run : runs)
-                if ((rc = launch_batch_kind(b->dim, r.kind, table + s * n, b->items, run.first,
-                                            run.second, r.slab_bytes, r.wpb, st)))
-                    break;
-        if (rc)
-            break;
+        if (!c.runs.empty())
+            chains.push_back(std::move(c));
     }
     const int64_t ntail = b->nitems - b->tail_first;
-    bool any = ntail > 0 && !active;
-    for (int p = b->tail_mbegin; ntail > 0 && active && p < (int)n; ++p)
-        any = any || (*active)[b->order[p]];
-    for (int s = 0; s < nst && any && rc == 0; ++s)
-        rc = launch_batch(b->dim, table + s * n, b->items + b->tail_first, ntail, b->nwaves,
-                          b->slab_bytes, b->wpb, st);
-    if (rc) {
-        hipError_t e = hipGetLastError();
-        return bfail(b, RV_E_HIP, std::string("batch launch failed: ") + hipGetErrorString(e));
+    if (ntail > 0) {
+        Chain c{-1, {}, 0.0};
+        bool any = false;
+        for (int p = b->tail_mbegin; p < (int)n; ++p) {
+            const int m = b->order[p];
+            if (!active || (*active)[m]) {
+                any = true;
+                c.bytes += member_bytes(m);
+            }
+        }
+        if (any)
+            chains.push_back(std::move(c));
     }
+    std::stable_sort(chains.begin(), chains.end(),
+                     [](const Chain &x, const Chain &y) { return x.bytes > y.bytes; });
+    const int nl = std::max(1, std::min<int>(b->lanes, (int)chains.size()));
+    std::vector<double> load(nl, 0.0);
+    std::vector<int> lane_of(chains.size());
+    for (size_t c = 0; c < chains.size(); ++c) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lane_of[c] = l;
+        load[l] += chains[c].bytes;
+    }
+    if (nl > 1) {
+        BHIP_TRY(b, hipEventRecord(b->fork_ev, st));
+        for (int l = 1; l < nl; ++l)
+            BHIP_TRY(b, hipStreamWaitEvent(b->lane[l], b->fork_ev, 0));
+    }
+    int rc = 0;
+    for (size_t c = 0; c < chains.size() && rc == 0; ++c) {
+        hipStream_t ls = lane_of[c] == 0 ? st : b->lane[lane_of[c]];
+        if (chains[c].range >= 0) {
+            const auto &r = b->kind_ranges[chains[c].range];
+            for (int s = 0; s < nst && rc == 0; ++s)
+                for (const auto &run : chains[c].runs)
+                    if ((rc = launch_batch_kind(b->dim, r.kind, table + s * n, b->items,
+                                                run.first, run.second, r.slab_bytes, r.wpb, ls)))
+                        break;
+        } else {
+            for (int s = 0; s < nst && rc == 0; ++s)
+                rc = launch_batch(b->dim, table + s * n, b->items + b->tail_first, ntail,
+                                  b->nwaves, b->slab_bytes, b->wpb, ls);
+        }
+    }
+    const hipError_t launch_err = rc ? hipGetLastError() : hipSuccess;
+    // the lanes join back also after a failed launch (a capture must end joined)
+    for (int l = 1; l < nl; ++l) {
+        BHIP_TRY(b, hipEventRecord(b->join_ev[l], b->lane[l]));
+        BHIP_TRY(b, hipStreamWaitEvent(st, b->join_ev[l], 0));
+    }
+    if (rc)
+        return bfail(b, RV_E_HIP, std::string("batch launch failed: ") +
+                                      hipGetErrorString(launch_err));
     return RV_OK;
 }
 
@@ -2743,6 +2814,14 @@ int rv_batch_create(rv_handle *const *members, int32_t n, rv_batch **out)
     for (auto &ev : b->ring_ev)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)))
             return bail("hipEventCreate", e);
+    if ((e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming)))
+        return bail("hipEventCreate", e);
+    for (int l = 1; l < rv_batch::MAXLANES; ++l) {
+        if ((e = hipStreamCreateWithFlags(&b->lane[l], hipStreamNonBlocking)))
+            return bail("hipStreamCreate", e);
+        if ((e = hipEventCreateWithFlags(&b->join_ev[l], hipEventDisableTiming)))
+            return bail("hipEventCreate", e);
+    }
     b->device_bytes = (int64_t)(cap * sizeof(BatchItem) + sizeof(StageParams) * ring_entries);
     if (batch_waves(b) != RV_OK) {
         free_batch(b);
@@ -2792,6 +2871,18 @@ int rv_batch_configure(rv_batch *b, int mixed_waves_per_block, int64_t own_launc
                           hipMemcpyHostToDevice));
     if (batch_waves(b) != RV_OK)
         return bfail(b, RV_E_HIP, "occupancy query of the batch kernel failed");
+    return RV_OK;
+}
+
+int rv_batch_set_streams(rv_batch *b, int32_t nstreams)
+{
+    if (!b)
+        return bfail(nullptr, RV_E_ARG, "null batch");
+    if (nstreams < 1 || nstreams > rv_batch::MAXLANES)
+        return bfail(b, RV_E_ARG, "streams must be 1 to 4");
+    BHIP_TRY(b, hipStreamSynchronize(b->stream));
+    drop_graphs(b);
+    b->lanes = nstreams;
     return RV_OK;
 }
 

@@ -178,6 +178,7 @@ def lib():
         "rv_batch_last_error": (ctypes.c_char_p, [H]),
         "rv_batch_info": (ctypes.c_int, [H, c_i64p, c_i64p, c_i32p]),
         "rv_batch_configure": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int64]),
+        "rv_batch_set_streams": (ctypes.c_int, [H, ctypes.c_int32]),
         "rv_batch_stage": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(vp), c_f64p, c_f64p]),
         "rv_batch_step": (ctypes.c_int, [H, ctypes.POINTER(vp), ctypes.POINTER(vp),

@@ -97,14 +97,19 @@ class Batch:
         L.check_batch(L.lib().rv_batch_step(self._b, *ptrs, L.ptr(cv, ctypes.c_int32),
                                             L.ptr(dv, ctypes.c_double), int(scheme)), self._b)
 
-    def configure(self, mixed_waves_per_block=2, own_launch_items=1024):
-        """Launch layout: waves per block of the mixed launch and the item
-        count from which a 2-D kind launches on its own (rv_batch_configure)."""
+    def configure(self, mixed_waves_per_block=2, own_launch_items=1024, streams=None):
+        """Launch layout: waves per block of the mixed launch, the item count
+        from which a 2-D kind launches on its own (rv_batch_configure) and,
+        when given, the number of concurrent streams the launch chains are
+        spread over (rv_batch_set_streams, 1 to 4; the library starts at 2)."""
         L.check_batch(L.lib().rv_batch_configure(self._b, int(mixed_waves_per_block),
                                                  int(own_launch_items)), self._b)
+        if streams is not None:
+            L.check_batch(L.lib().rv_batch_set_streams(self._b, int(streams)), self._b)
 
     def autotune(self, q, dt, mixed_waves=(1, 2, 4), own_items=(512, 1024, 4096, 1 << 40),
-                 member_waves=(1, 2), steps=2, scheme=L.RV_LSRK54, repeats=3):
+                 streams=(1, 2, 4), member_waves=(1, 2), steps=2, scheme=L.RV_LSRK54,
+                 repeats=3):
         """Pick the launch layout by timing what a run does: whole LSRK steps
         replayed as hipGraphs (lsrk_run), on scratch copies of the states q
         with step sizes dt (q is not modified).
@@ -112,11 +117,13 @@ class Batch:
         First each kind's waves per block (the single operator's launch shape,
         timed on the kind's largest member with DGSEM.autotune and applied to
         every member of the kind), then the mixed launch's waves per block and
-        the own-launch threshold: `repeats` rounds over the layouts in turn
-        (so a drift of the clock touches them alike), each setting the layout,
-        building its graphs with one run of `steps` steps and timing a second;
-        the layout with the lowest median is kept.  Returns {(mixed_waves, own_items): (median,
-        min, max) ms per step}."""
+        the own-launch threshold at streams[1] concurrent streams (streams[0]
+        if there is one choice), then the stream count at the best of those:
+        `repeats` rounds over the layouts of a pass in turn (so a drift of the
+        clock touches them alike), each setting the layout, building its
+        graphs with one run of `steps` steps and timing a second; the layout
+        with the lowest median is kept.  Returns {(mixed_waves, own_items,
+        streams): (median, min, max) ms per step}."""
         import torch
         by_kind = {}
         for k, dg in enumerate(self.dgs):
@@ -131,21 +138,27 @@ class Batch:
                     dg.waves_per_block = w
         qs = [t.clone() for t in q]
         dqs = [torch.empty_like(t) for t in q]
-        layouts = [(w, t) for w in mixed_waves for t in own_items]
-        runs = {k: [] for k in layouts}
-        for _ in range(repeats):
-            for k in layouts:
-                # a new layout drops the captured graphs: one run captures
-                # them, the next is timed
-                self.configure(*k)
-                self.lsrk_run(qs, dqs, dt, steps, scheme)
-                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-                ev[0].record()
-                self.lsrk_run(qs, dqs, dt, steps, scheme)
-                ev[1].record()
-                ev[1].synchronize()
-                runs[k].append(ev[0].elapsed_time(ev[1]) / steps)
-        times = {k: (float(np.median(v)), min(v), max(v)) for k, v in runs.items()}
+
+        def timed(layouts):
+            runs = {k: [] for k in layouts}
+            for _ in range(repeats):
+                for k in layouts:
+                    # a new layout drops the captured graphs: one run captures
+                    # them, the next is timed
+                    self.configure(*k)
+                    self.lsrk_run(qs, dqs, dt, steps, scheme)
+                    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                    ev[0].record()
+                    self.lsrk_run(qs, dqs, dt, steps, scheme)
+                    ev[1].record()
+                    ev[1].synchronize()
+                    runs[k].append(ev[0].elapsed_time(ev[1]) / steps)
+            return {k: (float(np.median(v)), min(v), max(v)) for k, v in runs.items()}
+
+        s0 = streams[1] if len(streams) > 1 else streams[0]
+        times = timed([(w, t, s0) for w in mixed_waves for t in own_items])
+        w, t, _ = min(times, key=lambda k: times[k][0])
+        times.update(timed([(w, t, s) for s in streams if s != s0]))
         best = min(times, key=lambda k: times[k][0])
         self.configure(*best)
         return times

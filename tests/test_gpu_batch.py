@@ -192,8 +192,9 @@ def test_batch_step_skips_resting_members(torch, R, dim, mix):
 
 def test_batch_layout_does_not_change_results(torch, R):
     """Every launch layout rv_batch_configure offers (all kinds on their own,
-    all in the mixed launch, each mixed waves-per-block) and the autotuner's
-    choice give bit-identical stages."""
+    all in the mixed launch, each mixed waves-per-block), each number of
+    concurrent streams (rv_batch_set_streams) and the autotuner's choice give
+    bit-identical stages."""
     ms = [member(R, torch, 2, nq, n, law, dt, 30 + s) for s, (nq, n, law, dt) in enumerate(MIX2H)]
     dgs = [m[0] for m in ms]
     batch = R.Batch(dgs)
@@ -212,12 +213,15 @@ def test_batch_layout_does_not_change_results(torch, R):
         return q_out, dq
 
     want = run()
-    for w, t in [(1, 1), (4, 1), (1, 1 << 40), (2, 1 << 40), (4, 1 << 40), (2, 1024)]:
-        batch.configure(w, t)
+    for w, t, s in [(1, 1, 1), (4, 1, 4), (1, 1 << 40, 2), (2, 1 << 40, 1), (4, 1 << 40, 3),
+                    (2, 1024, 1), (2, 1024, 4), (2, 64, 3)]:
+        batch.configure(w, t, s)
         got = run()
         for k in range(len(ms)):
             assert torch.equal(got[0][k], want[0][k]) and torch.equal(got[1][k], want[1][k]), \
-                f"layout ({w}, {t}): member {k} differs"
+                f"layout ({w}, {t}, {s} streams): member {k} differs"
+    with pytest.raises(R.RavenHIPError):
+        batch.configure(2, 1024, 5)
     times = batch.autotune(q_in, [1e-3] * len(ms))
     say(f"   autotune: {times}")
     got = run()
