@@ -204,7 +204,9 @@ int rv_lsrk_stage(rv_handle *h, void *q_out, const void *q_in, void *dq, double 
                   double b_dt, double t);
 
 /* solve!(q, timeend, LSRK(dg, q, dt; t0); adjust_final) (odesolvers.jl:9-61):
- * dq is zeroed first (odesolvers.jl:43-44); steps are taken until
+ * dq starts at zero (odesolvers.jl:43-44): its entry values are not read by
+ * schemes whose first stage has a = 0 (LSRK54, LSRK144; that stage writes
+ * dq = L(q)) and are zeroed first otherwise; steps are taken until
  * time + dt >= timeend, and with adjust_final the last one is shortened to
  * timeend - time (odesolvers.jl:21-25), else it keeps dt and ends past
  * timeend.  FP32 handles keep t0, dt and the clock in FP32 as LSRK{Float32}
@@ -402,7 +404,8 @@ int rv_batch_step(rv_batch *b, void *const *q0, void *const *q1, void *const *dq
  * any member in one call. */
 int rv_batch_steps(rv_batch *b, void *const *q0, void *const *q1, void *const *dq,
                    const int32_t *cur, const double *dt, int scheme, int64_t nsteps);
-/* nsteps whole LSRK steps of every member with its own dt[m]: dq[m] is zeroed,
+/* nsteps whole LSRK steps of every member with its own dt[m]: dq[m] starts
+ * at zero as in rv_lsrk_run (its entry values unread when the first a is 0),
  * each step replays a captured hipGraph of the scheme's stages, and q[m] holds
  * the result (the library ping-pongs through buffers of its own).  The graphs
  * are kept while q, dq, scheme and dt stay the same. */

@@ -1479,7 +1479,10 @@ int lsrk_run_impl(rv_handle *h, void *q, void *dq, int scheme, double t0, double
     (void)C;  // the laws here do not depend on the stage time
     const bool f32 = h->dtype == RV_F32;
     const size_t bytes = (size_t)h->Np * h->ns * h->K * (f32 ? 4 : 8);
-    HIP_TRY(h, hipMemsetAsync(dq, 0, bytes, h->stream));
+    // dq starts at zero (odesolvers.jl:43-44); a first stage with a = 0 does
+    // not read it (dq = L(q), as dq .*= 0 plus the increment gives from zero)
+    if (A[0] != 0.0)
+        HIP_TRY(h, hipMemsetAsync(dq, 0, bytes, h->stream));
     // LSRK(rhs!, ...) converts dt, t0 and the coefficients to the state type
     // (odesolvers.jl:41-48); in FP32 the clock is FP32 too.  solve! compares
     // time + dt with timeend as given and assigns timeend - time to the FP32
@@ -3069,8 +3072,12 @@ int rv_batch_lsrk_run(rv_batch *b, void *const *q, void *const *dq, int scheme, 
         }
         b->graph_key = key;
     }
-    // solve! starts from dq = 0 (odesolvers.jl:43-44); the clear is outside the graphs
-    for (int m = 0; m < n; ++m) {
+    // LSRK's dq starts at zero (odesolvers.jl:43-44).  A first stage with
+    // a = 0 (LSRK54, LSRK144) writes dq = L(q) without reading it, which is
+    // what dq .*= 0 followed by the increment gives from that zero, so the
+    // clear is needed only when a scheme's first a is not 0.  It stays outside
+    // the graphs.
+    for (int m = 0; m < n && A[0] != 0.0; ++m) {
         rv_handle *h = b->members[m];
         const size_t bytes = (size_t)h->Np * h->ns * h->K * (h->dtype == RV_F32 ? 4 : 8);
         BHIP_TRY(b, hipMemsetAsync(dq[m], 0, bytes, b->stream));

@@ -69,8 +69,8 @@ class Batch:
                                              L.ptr(bv, ctypes.c_double)), self._b)
 
     def lsrk_run(self, q, dq, dt, nsteps, scheme=L.RV_LSRK54):
-        """nsteps whole LSRK steps of every member with its own dt (dq zeroed
-        first); q is updated in place."""
+        """nsteps whole LSRK steps of every member with its own dt (dq starts
+        at zero as in rv_lsrk_run); q is updated in place."""
         self._check(q), self._check(dq)
         dv = np.ascontiguousarray(dt, dtype=np.float64)
         if dv.size != len(self.dgs):
@@ -108,7 +108,7 @@ class Batch:
             L.check_batch(L.lib().rv_batch_set_streams(self._b, int(streams)), self._b)
 
     def autotune(self, q, dt, mixed_waves=(1, 2, 4), own_items=(512, 1024, 4096, 1 << 40),
-                 streams=(1, 2, 4), member_waves=(1, 2), steps=2, scheme=L.RV_LSRK54,
+                 streams=(1, 2, 4), member_waves=(1, 2), steps=4, scheme=L.RV_LSRK54,
                  repeats=3):
         """Pick the launch layout by timing what a run does: whole LSRK steps
         replayed as hipGraphs (lsrk_run), on scratch copies of the states q

@@ -113,7 +113,8 @@ def solve_(q, timeend, solver, after_step=None, after_stage=None, adjust_final=T
         solver.time = t.value
         return n.value
     else:
-        # LSRK's dq starts at zero (odesolvers.jl:43-44), as rv_lsrk_solve makes it
+        # LSRK's dq starts at zero (odesolvers.jl:43-44); rv_lsrk_solve's
+        # schemes (first a = 0) give the same without reading it
         solver.dq.zero_()
         qtmp = q.new_empty(q.shape)
 

@@ -129,7 +129,8 @@ def test_batch_rhs_stage_matches_oracle(torch, R):
 def test_batch_graph_run_equals_stage_loop(torch, R, dim, mix):
     """rv_batch_lsrk_run (a captured hipGraph per step, replayed) against each
     member stepping alone through its own stages; odd step counts exercise the
-    final copy back from the ping-pong buffer."""
+    final copy back from the ping-pong buffer.  dq enters as NaN: LSRK54's
+    first stage (a = 0) must not read it."""
     ms = [member(R, torch, dim, nq, n, law, dt, 10 + s) for s, (nq, n, law, dt) in enumerate(mix)]
     dgs = [m[0] for m in ms]
     dts = [2e-3 * (1 + 0.25 * k) for k in range(len(ms))]
@@ -144,12 +145,13 @@ def test_batch_graph_run_equals_stage_loop(torch, R, dim, mix):
             want.append(qa)
         batch = R.Batch(dgs)
         qs = [m[1].clone() for m in ms]
-        dqs = [torch.empty_like(q) for q in qs]
+        dqs = [torch.full_like(q, float("nan")) for q in qs]
         batch.lsrk_run(qs, dqs, dts, nsteps)
         # a replay of the cached graphs from the same buffers
         qs2 = qs
         for k in range(len(ms)):
             qs2[k].copy_(ms[k][1])
+            dqs[k].fill_(float("nan"))
         batch.lsrk_run(qs2, dqs, dts, nsteps)
         torch.cuda.synchronize()
         for k in range(len(ms)):

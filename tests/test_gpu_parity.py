@@ -896,6 +896,13 @@ def test_solve_with_callbacks_matches_library_loop(torch, R):
     assert [t for _, t in steps] == pytest.approx([0.0, 0.01, 0.02, 0.03])
     assert solver.time == pytest.approx(T)
     assert torch.equal(qa, qb)
+    # the library loop does not read dq's entry values (LSRK54's first a is 0)
+    qc = to_dev(torch, dg, q0)
+    sc = R.LSRK54(dg, qc, dt)
+    sc.dq.fill_(float("nan"))
+    assert R.solve_(qc, T, sc) == na
+    dg.synchronize()
+    assert torch.equal(qa, qc)
     with pytest.raises(ValueError):
         R.solve_(qb, T, R.RLSRK54(dg, qb, dt), after_stage=lambda t, q: None)
 
