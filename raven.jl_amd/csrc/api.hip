@@ -100,6 +100,7 @@ struct rv_handle {
     MortarLauncher mortar = nullptr;
     int64_t mortar_cells = 0;
     int64_t *m_cells = nullptr, *m_partner = nullptr;
+    int64_t mortar_pairs = 0, *m_pairs = nullptr;  // non-conforming (cell, face) pairs
     int32_t *m_kind = nullptr, *m_slot = nullptr;
     double *m_sgeo = nullptr, *m_halves = nullptr;
     unsigned char *m_delta = nullptr;  // per (mortar cell, face) updates of one stage
@@ -203,7 +204,7 @@ void free_handle(rv_handle *h)
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
                     h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
                     h->m_delta, h->m_slot, h->cellJ,  h->relax_partials, h->eprod,
-                    h->relax_sums};
+                    h->relax_sums, h->m_pairs};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -858,6 +859,8 @@ int launch(rv_handle *h, const StageParams &P0, StageLauncher fn = nullptr)
         M.stage = P;
         M.ncells = h->mortar_cells;
         M.cells = as_global(h->m_cells);
+        M.npairs = h->mortar_pairs;
+        M.pairs = as_global(h->m_pairs);
         M.kind = as_global(h->m_kind);
         M.partner = as_global(h->m_partner);
         M.sgeo = as_global(h->m_sgeo);
@@ -1333,15 +1336,29 @@ int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *par
     std::vector<double> halves(2 * h->nq * h->nq);
     lobatto_halves(h->nq, halves.data(), halves.data() + h->nq * h->nq);
     std::vector<int32_t> slot(h->K, -1);
-    for (size_t li = 0; li < cells.size(); ++li)
+    std::vector<int64_t> pairs;  // the face kernel's workgroups, in (cell, face) order
+    for (size_t li = 0; li < cells.size(); ++li) {
         slot[cells[li]] = (int32_t)li;
+        for (int f = 0; f < nf; ++f)
+            if (face_kind[cells[li] * nf + f] != 0)
+                pairs.push_back((int64_t)li * nf + f);
+    }
+    if (pairs.size() > (size_t)UINT32_MAX)
+        return fail(h, RV_E_UNSUPPORTED, "too many non-conforming faces for one launch");
     const size_t nsg = (size_t)h->K * nf * (h->dim + 1) * h->nfp;
+    const int64_t ncells = (int64_t)cells.size(), npairs = (int64_t)pairs.size();
+    // (a grid without non-conforming faces keeps one-entry lists)
+    if (cells.empty())
+        cells.push_back(0);
+    if (pairs.empty())
+        pairs.push_back(0);
     int rc;
-    if ((rc = upload(h, &h->m_cells, cells.data(), std::max<size_t>(cells.size(), 1))) ||
+    if ((rc = upload(h, &h->m_cells, cells.data(), cells.size())) ||
         (rc = upload(h, &h->m_kind, face_kind, (size_t)h->K * nf)) ||
         (rc = upload(h, &h->m_partner, partner, (size_t)h->K * nf * 4)) ||
         (rc = upload(h, &h->m_halves, halves.data(), halves.size())) ||
         (rc = upload(h, &h->m_slot, slot.data(), slot.size())) ||
+        (rc = upload(h, &h->m_pairs, pairs.data(), pairs.size())) ||
         (rc = dalloc(h, &h->m_sgeo, nsg)) ||
         (rc = dalloc(h, &h->m_delta, cells.size() * nf * h->ns * h->nfp *
                                          (h->dtype == RV_F32 ? 4 : 8))))
@@ -1353,7 +1370,8 @@ int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *par
                        h->stream, h->sgeo, h->m_kind, h->K, nf, h->dim, h->nfp);
     HIP_TRY(h, hipGetLastError());
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    h->mortar_cells = (int64_t)cells.size();
+    h->mortar_cells = ncells;
+    h->mortar_pairs = npairs;
     h->mortar = ml;
     return RV_OK;
 }

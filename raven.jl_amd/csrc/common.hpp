@@ -115,6 +115,8 @@ struct MortarParams {
     StageParams stage;       // the stage just launched (states, law, mji)
     int64_t ncells;          // cells with non-conforming faces
     const int64_t RV_GLOBAL *cells;
+    int64_t npairs;          // their non-conforming (cell, face) pairs
+    const int64_t RV_GLOBAL *pairs;    // li * 2dim + f, li indexing cells
     const int32_t RV_GLOBAL *kind;     // (K, 2dim): 0 conforming, 1 hanging, 2 covered
     const int64_t RV_GLOBAL *partner;  // (K, 2dim, 4), see forest_mortars
     const double RV_GLOBAL *sgeo;      // surface geometry with the lift weights kept

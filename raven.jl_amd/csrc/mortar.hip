@@ -52,12 +52,14 @@ __global__ void __launch_bounds__(64) mortar_face_kernel(const MortarParams M)
     __shared__ R u[NSUB][NS][NFP];
     const StageParams &P = M.stage;
     const Law<DIM, LAW, R> law(P);
-    const int64_t li = blockIdx.x / NF;
-    const int f = (int)(blockIdx.x - li * NF);
+    // one workgroup per non-conforming (cell, face) pair
+    const int64_t pf = M.pairs[blockIdx.x];
+    const int64_t li = pf / NF;
+    const int f = (int)(pf - li * NF);
     const int64_t e = M.cells[li];
     const int kind = M.kind[e * NF + f];
     if (kind == 0)
-        return;  // the whole workgroup: a conforming face
+        return;  // the whole workgroup (not reached: the pairs list none)
     const int m = threadIdx.x;
     const bool on = m < NFP;
     const int a = m % NQ, b = m / NQ;  // face node m = a + NQ*b (b = 0 in 2-D)
@@ -327,8 +329,10 @@ int launch_mortar(const MortarParams &M, void *stream, int phase)
         return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (phase == 0) {
+        if (M.npairs <= 0)
+            return 0;
         hipLaunchKernelGGL((mortar_face_kernel<DIM, NQ, LAW, R, FLUX>),
-                           dim3((unsigned)(M.ncells * 2 * DIM)), dim3(64), 0, st, M);
+                           dim3((unsigned)M.npairs), dim3(64), 0, st, M);
     } else {
         const int64_t nodes = M.ncells * NP;
         hipLaunchKernelGGL((mortar_apply_kernel<DIM, NQ, LAW, R>),

@@ -639,6 +639,33 @@ def test_mortar_stage_on_adapted_grid_matches_oracle(torch, R, nq, dtype):
         assert np.all(np.abs(tot) <= 1e-11 * scale)
 
 
+def test_mortars_on_a_conforming_forest_change_nothing(torch, R):
+    """rv_enable_mortars on a warped forest without hanging faces (no mortar
+    cell, no face kernel workgroup): the stage is the one without mortars,
+    bit for bit."""
+    gm = R.GridManager(R.LobattoCell(4, 4, 4),
+                       R.brick([PB.julia_range(0, 10, 5)] * 3, (True,) * 3))
+    gm.adapt(np.full(len(gm), R.AdaptNone, dtype=np.uint8))
+    grid = R.generate(gm, warp=_warp10)
+    assert grid.face_kind.shape == (grid.K, 6) and not np.any(grid.face_kind)
+    dgs = [R.DGSEM(law=R.EulerLaw(3), grid=grid, surface_numericalflux=R.RusanovFlux())
+           for _ in range(2)]
+    dgs[1].enable_mortars(grid)
+    rng = np.random.default_rng(4)
+    P = grid.points
+    q = PB.isentropic_vortex_3d(P, 0.0, centre=(5.0, 5.0),
+                                noise=1e-3 * rng.uniform(-1, 1, P[:, 0, :].shape))
+    dq0 = 1e-3 * rng.standard_normal(q.size)
+    got = []
+    for dg in dgs:
+        qd, dqd = to_dev(torch, dg, q), to_dev(torch, dg, dq0)
+        qo = torch.empty_like(qd)
+        dg.lsrk_stage(qo, qd, dqd, -0.6, 2e-3)
+        dg.synchronize()
+        got.append((qo, dqd))
+    assert torch.equal(got[0][0], got[1][0]) and torch.equal(got[0][1], got[1][1])
+
+
 def test_mortar_stage_with_flux_differencing_matches_oracle(torch, R):
     """FluxDifferencingForm(EntropyConservativeFlux()) on the warped 2:1 forest
     with mortar-coupled hanging faces: the flux-differencing stage kernel
