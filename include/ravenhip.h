@@ -60,7 +60,9 @@ enum { RV_FLUX_RUSANOV = 1, RV_FLUX_CENTRAL = 2, RV_FLUX_EC = 3 };
 /* volume form (dgsem.jl:1-17): WeakForm, or FluxDifferencingForm(volume flux)
  * evaluated direction by direction as esvolumeterm_dir! (dgsem.jl:514-608) */
 enum { RV_FORM_WEAK = 0, RV_FORM_FLUX_DIFFERENCING = 1 };
-/* state precision: FP64, or FP32 state/flux arithmetic with FP64 geometry */
+/* state precision: FP64, or FP32 state/flux arithmetic with FP64 geometry
+ * (the 3-D weak-form stage reads FP32 copies of the general-cell geometry,
+ * rounded once, which give the values it would round each use to) */
 enum { RV_F64 = 1, RV_F32 = 2 };
 /* geometry handling: AUTO detects affine elements (per-element scale factors)
  * and otherwise keeps per-node FP64 metrics; GENERAL forces the latter. */

@@ -103,6 +103,8 @@ struct rv_handle {
     int64_t mortar_pairs = 0, *m_pairs = nullptr;  // non-conforming (cell, face) pairs
     int32_t *m_kind = nullptr, *m_slot = nullptr;
     double *m_sgeo = nullptr, *m_halves = nullptr;
+    // FP32 handles with 3-D general cells: vgeo, mji, sgeo rounded to FP32
+    float *vgeo32 = nullptr, *mji32 = nullptr, *sgeo32 = nullptr, *mj32 = nullptr;
     unsigned char *m_delta = nullptr;  // per (mortar cell, face) updates of one stage
 };
 
@@ -204,7 +206,7 @@ void free_handle(rv_handle *h)
                     h->partials, h->qtmp,       h->dtoc,     h->ctod_ptr, h->ctod_rows,
                     h->m_cells, h->m_partner,   h->m_kind,   h->m_sgeo,   h->m_halves,
                     h->m_delta, h->m_slot, h->cellJ,  h->relax_partials, h->eprod,
-                    h->relax_sums, h->m_pairs};
+                    h->relax_sums, h->m_pairs, h->vgeo32, h->mji32, h->sgeo32, h->mj32};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -539,6 +541,40 @@ __global__ void affine_mj_kernel(double *MJ, const double *J, WeightsArg wa, int
     MJ[idx] = wp * J[e];
 }
 
+__global__ void round_to_f32_kernel(const double *__restrict__ src, float *__restrict__ dst,
+                                    int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+// The FP32 copies of the general geometry (and MJ) that the 3-D column kernel
+// of an FP32 handle reads (StageParams::vgeo32 ...), made again after every
+// change of the FP64 arrays.
+int refresh_geo32(rv_handle *h)
+{
+    if (!(h->dtype == RV_F32 && h->geo == GEO_GENERAL && h->dim == 3) || h->K <= 0)
+        return RV_OK;
+    const int64_t nv = h->K * 9 * h->Np, nm = h->K * h->Np, ns = h->K * 6 * 4 * h->nfp;
+    int rc;
+    if (!h->vgeo32 && ((rc = dalloc(h, &h->vgeo32, nv)) || (rc = dalloc(h, &h->mji32, nm)) ||
+                       (rc = dalloc(h, &h->sgeo32, ns)) || (rc = dalloc(h, &h->mj32, nm))))
+        return rc;
+    const struct {
+        const double *src;
+        float *dst;
+        int64_t n;
+    } parts[] = {{h->vgeo, h->vgeo32, nv}, {h->mji, h->mji32, nm}, {h->sgeo, h->sgeo32, ns},
+                 {h->MJ, h->mj32, nm}};
+    for (const auto &c : parts)
+        hipLaunchKernelGGL(round_to_f32_kernel,
+                           dim3((unsigned)std::min<int64_t>((c.n + 255) / 256, 4096)), dim3(256),
+                           0, h->stream, c.src, c.dst, c.n);
+    HIP_TRY(h, hipGetLastError());
+    return RV_OK;
+}
+
 // Lift weight 0 on the non-conforming faces: the stage kernels then leave
 // them out and the mortar kernel supplies their flux.
 __global__ void zero_lift_kernel(double *sgeo, const int32_t *kind, int64_t K, int nf, int dim,
@@ -579,7 +615,8 @@ int finish_setup(rv_handle *h, const double *D, const double *w)
     if ((rc = upload(h, &h->Dt, Dt.data(), Dt.size())) ||
         (rc = upload(h, &h->Dw, Dw.data(), Dw.size())) ||
         (rc = upload(h, &h->winv, winv.data(), winv.size())) ||
-        (rc = dalloc(h, &h->partials, (size_t)2 * RED_BLOCKS * h->ns)))
+        (rc = dalloc(h, &h->partials, (size_t)2 * RED_BLOCKS * h->ns)) ||
+        (rc = refresh_geo32(h)))
         return rc;
     KernelKey key{h->dim, nq, h->law, h->dtype == RV_F64 ? 8 : 4, h->geo, h->flux};
     key.vflux = h->vflux;
@@ -834,6 +871,10 @@ StageParams stage_params(rv_handle *h)
     P.mji = as_global(h->mji);
     P.mj = as_global(h->MJ);
     P.sgeo = as_global(h->sgeo);
+    P.vgeo32 = as_global(h->vgeo32);
+    P.mji32 = as_global(h->mji32);
+    P.sgeo32 = as_global(h->sgeo32);
+    P.mj32 = as_global(h->mj32);
     P.Dt = as_global(h->Dt);
     P.Dw = as_global(h->Dw);
     P.winv = as_global(h->winv);
@@ -1219,11 +1260,15 @@ int rv_get_info(const rv_handle *h, rv_info *info)
     info->state_bytes = nodes * h->ns * b;
     info->device_bytes = h->device_bytes;
     // compulsory traffic per node: read q, [read dq], write dq, write q; plus
-    // per-node FP64 geometry (wJ*dRdX, 1/wJ) and per face-node (n, wsJ) when
-    // the cells are not affine; affine geometry is per cell (~0 per node)
+    // per-node geometry (wJ*dRdX, 1/wJ) and per face-node (n, wsJ) when the
+    // cells are not affine, FP64 except for the 3-D weak form of FP32
+    // handles, whose stage reads the FP32 copies; affine geometry is per cell
+    // (~0 per node)
     int64_t geo = 0;
-    if (h->geo == GEO_GENERAL)
-        geo = nodes * (h->dim * h->dim + 1) * 8 + h->K * 2 * h->dim * h->nfp * (h->dim + 1) * 8;
+    if (h->geo == GEO_GENERAL) {
+        const int gb = h->dtype == RV_F32 && h->dim == 3 && h->vflux == 0 ? 4 : 8;
+        geo = nodes * (h->dim * h->dim + 1) * gb + h->K * 2 * h->dim * h->nfp * (h->dim + 1) * gb;
+    }
     info->stage_bytes = 4 * info->state_bytes + geo;
     info->stage_bytes_first = 3 * info->state_bytes + geo;
     info->rhs_bytes = 2 * info->state_bytes + geo;
@@ -1369,6 +1414,8 @@ int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *par
     hipLaunchKernelGGL(zero_lift_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        h->stream, h->sgeo, h->m_kind, h->K, nf, h->dim, h->nfp);
     HIP_TRY(h, hipGetLastError());
+    if ((rc = refresh_geo32(h)))
+        return rc;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     h->mortar_cells = ncells;
     h->mortar_pairs = npairs;

@@ -63,6 +63,13 @@ struct StageParams {
     const double RV_GLOBAL *mji;   // (K, Np)
     const double RV_GLOBAL *mj;    // (K, Np) = wJ
     const double RV_GLOBAL *sgeo;  // (K, 2dim, dim+1, Nfp): n_c ..., wsJ
+    // FP32 handles, 3-D general cells: vgeo, mji and sgeo rounded to FP32 once
+    // (the 3-D column kernel rounds every use of them to the state type, so
+    // reading these gives the same results with half the geometry bytes)
+    const float RV_GLOBAL *vgeo32;
+    const float RV_GLOBAL *mji32;
+    const float RV_GLOBAL *sgeo32;
+    const float RV_GLOBAL *mj32;
     // operators (FP64, device)
     const double RV_GLOBAL *Dt;    // [i*Nq + n] = D[n,i]            (general)
     const double RV_GLOBAL *Dw;    // [i*Nq + n] = D[n,i] w_n / w_i  (affine)

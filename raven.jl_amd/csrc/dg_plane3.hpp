@@ -150,6 +150,17 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     };
     const Law<3, LAW, R> law(P);
     const R *qbase = static_cast<const R *>(P.q_in);
+    // general cells' geometry: FP32 state reads the FP32 copies (every use
+    // rounds to R, so the values are the ones the FP64 arrays give)
+    constexpr bool G32 = GEO == GEO_GENERAL && sizeof(R) == 4;
+    using GT = typename std::conditional<G32, float, double>::type;
+    const GT RV_GLOBAL *vgeo_p, *sgeo_p, *mji_p;
+    const GT RV_GLOBAL *mj_p;  // MJ for the a*MJ factor of dq
+    if constexpr (G32) {
+        vgeo_p = P.vgeo32, sgeo_p = P.sgeo32, mji_p = P.mji32, mj_p = P.mj32;
+    } else {
+        vgeo_p = P.vgeo, sgeo_p = P.sgeo, mji_p = P.mji, mj_p = P.mj;
+    }
     const bool dx_on = P.dir_mask & 1, dy_on = (P.dir_mask >> 1) & 1,
                dz_on = (P.dir_mask >> 2) & 1;
 
@@ -222,7 +233,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                     acc[s][k] = load_dq<ST>(dqe + s * NP + k * NQ2);
 #pragma unroll
             for (int k = 0; k < NQ; ++k)
-                zscale[k] = GEO == GEO_AFFINE ? a : a * R(P.mj[e * NP + l + k * NQ2]);
+                zscale[k] = GEO == GEO_AFFINE ? a : a * R(mj_p[e * NP + l + k * NQ2]);
             if (!late_scale) {
 #pragma unroll
                 for (int k = 0; k < NQ; ++k)
@@ -268,7 +279,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     };
     // general cells: wJ*dRdX of plane k in gbuf[k & 1], requested two planes
     // ahead (one plane's work is shorter than a load's latency)
-    double gbuf[2][3][3];
+    GT gbuf[2][3][3];
     auto load_geo = [&](const int k) {
         if constexpr (GEO == GEO_GENERAL) {
             const int node = l + k * NQ2;
@@ -276,7 +287,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    gbuf[k & 1][d][c] = P.vgeo[((e * 3 + d) * 3 + c) * NP + node];
+                    gbuf[k & 1][d][c] = vgeo_p[((e * 3 + d) * 3 + c) * NP + node];
         }
     };
     auto plane_flux = [&](const int k) {
@@ -430,7 +441,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     // no longer need, together with the z faces' normals and weights; affine
     // cells after the plane loop
     R qp[6][NS];
-    double zsg[2][4];  // general cells: n and wsJ of this lane's z-face nodes
+    GT zsg[2][4];  // general cells: n and wsJ of this lane's z-face nodes
     auto face_loads = [&] {
 #pragma unroll
         for (int f = 2; f < 6; ++f)
@@ -441,7 +452,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
             for (int side = 0; side < 2; ++side)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    zsg[side][c] = P.sgeo[((e * 6 + 4 + side) * 4 + c) * NQ2 + l];
+                    zsg[side][c] = sgeo_p[((e * 6 + 4 + side) * 4 + c) * NQ2 + l];
         }
     };
     lds_sync();
@@ -475,14 +486,14 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
     // requested before the z faces are evaluated (FP64 has no registers to
     // spare for them and reads them where the faces are evaluated)
     constexpr bool XYPRE = GEO == GEO_GENERAL && sizeof(R) == 4;
-    double xysg[XYPRE ? 4 : 1][4];
+    GT xysg[XYPRE ? 4 : 1][4];
     if (active) {
         if constexpr (XYPRE) {
 #pragma unroll
             for (int f = 0; f < 4; ++f)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    xysg[f][c] = P.sgeo[((e * 6 + f) * 4 + c) * NQ2 + l];
+                    xysg[f][c] = sgeo_p[((e * 6 + f) * 4 + c) * NQ2 + l];
         }
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
@@ -574,7 +585,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
                 lift = R(xysg[f][3]);
                 S::nflux_general(law, nv, qm, qp[f], nf);
             } else {
-                const double *sg = P.sgeo + ((e * 6 + f) * 4) * NQ2 + l;
+                const GT *sg = sgeo_p + ((e * 6 + f) * 4) * NQ2 + l;
                 R nv[3] = {R(sg[0]), R(sg[NQ2]), R(sg[2 * NQ2])};
                 lift = R(sg[3 * NQ2]);
                 S::nflux_general(law, nv, qm, qp[f], nf);
@@ -610,7 +621,7 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
         R mjiv[NQ];
 #pragma unroll
         for (int k = 0; k < NQ; ++k)
-            mjiv[k] = GEO == GEO_GENERAL ? R(P.mji[e * NP + l + k * NQ2]) : R(1);
+            mjiv[k] = GEO == GEO_GENERAL ? R(mji_p[e * NP + l + k * NQ2]) : R(1);
         if constexpr (RX) {
             const R *dqe = static_cast<const R *>(P.dq) + e * (NS * NP) + l;
             const double *mje = P.mj + e * NP + l;
