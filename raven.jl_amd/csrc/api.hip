@@ -100,7 +100,8 @@ struct rv_handle {
     MortarLauncher mortar = nullptr;
     int64_t mortar_cells = 0;
     int64_t *m_cells = nullptr, *m_partner = nullptr;
-    int64_t mortar_pairs = 0, *m_pairs = nullptr;  // non-conforming (cell, face) pairs
+    int64_t mortar_pairs = 0;        // non-conforming (cell, face) pairs,
+    MortarPair *m_pairs = nullptr;  // covered sides first
     int32_t *m_kind = nullptr, *m_slot = nullptr;
     double *m_sgeo = nullptr, *m_halves = nullptr;
     // FP32 handles with 3-D general cells: vgeo, mji, sgeo rounded to FP32
@@ -1381,22 +1382,37 @@ int rv_enable_mortars(rv_handle *h, const int32_t *face_kind, const int64_t *par
     std::vector<double> halves(2 * h->nq * h->nq);
     lobatto_halves(h->nq, halves.data(), halves.data() + h->nq * h->nq);
     std::vector<int32_t> slot(h->K, -1);
-    std::vector<int64_t> pairs;  // the face kernel's workgroups, in (cell, face) order
+    // the face kernel's workgroups: covered sides (the longer ones) first,
+    // then hanging sides, each in (cell, face) order
+    std::vector<MortarPair> pairs, hanging;
     for (size_t li = 0; li < cells.size(); ++li) {
-        slot[cells[li]] = (int32_t)li;
-        for (int f = 0; f < nf; ++f)
-            if (face_kind[cells[li] * nf + f] != 0)
-                pairs.push_back((int64_t)li * nf + f);
+        const int64_t e = cells[li];
+        slot[e] = (int32_t)li;
+        for (int f = 0; f < nf; ++f) {
+            const int k = face_kind[e * nf + f];
+            if (k == 0)
+                continue;
+            MortarPair pr{};
+            pr.e = e;
+            pr.li = (int64_t)li;
+            for (int t = 0; t < 4; ++t)
+                pr.part[t] = partner[4 * (e * nf + f) + t];
+            pr.f = f;
+            pr.kind = k;
+            (k == 2 ? pairs : hanging).push_back(pr);
+        }
     }
+    pairs.insert(pairs.end(), hanging.begin(), hanging.end());
     if (pairs.size() > (size_t)UINT32_MAX)
         return fail(h, RV_E_UNSUPPORTED, "too many non-conforming faces for one launch");
+    const int64_t npairs = (int64_t)pairs.size();
     const size_t nsg = (size_t)h->K * nf * (h->dim + 1) * h->nfp;
-    const int64_t ncells = (int64_t)cells.size(), npairs = (int64_t)pairs.size();
+    const int64_t ncells = (int64_t)cells.size();
     // (a grid without non-conforming faces keeps one-entry lists)
     if (cells.empty())
         cells.push_back(0);
     if (pairs.empty())
-        pairs.push_back(0);
+        pairs.push_back(MortarPair{});
     int rc;
     if ((rc = upload(h, &h->m_cells, cells.data(), cells.size())) ||
         (rc = upload(h, &h->m_kind, face_kind, (size_t)h->K * nf)) ||

@@ -118,12 +118,23 @@ struct KernelKey {
 };
 
 // Mortar coupling of hanging faces (mortar.hip), launched after a stage.
+// One non-conforming face of a mortar cell: its cell e (and slot li in the
+// list of mortar cells), face f, kind (1 hanging, 2 covered) and partners
+// (hanging: coarse cell, sub-face; covered: the 2^(dim-1) fine cells)
+struct MortarPair {
+    int64_t e, li;
+    int64_t part[4];
+    int32_t f, kind;
+};
+
 struct MortarParams {
     StageParams stage;       // the stage just launched (states, law, mji)
     int64_t ncells;          // cells with non-conforming faces
     const int64_t RV_GLOBAL *cells;
-    int64_t npairs;          // their non-conforming (cell, face) pairs
-    const int64_t RV_GLOBAL *pairs;    // li * 2dim + f, li indexing cells
+    // their non-conforming (cell, face) pairs: covered (coarse) sides first,
+    // which take longer, then hanging (fine) sides
+    int64_t npairs;
+    const MortarPair RV_GLOBAL *pairs;
     const int32_t RV_GLOBAL *kind;     // (K, 2dim): 0 conforming, 1 hanging, 2 covered
     const int64_t RV_GLOBAL *partner;  // (K, 2dim, 4), see forest_mortars
     const double RV_GLOBAL *sgeo;      // surface geometry with the lift weights kept

@@ -52,14 +52,11 @@ __global__ void __launch_bounds__(64) mortar_face_kernel(const MortarParams M)
     __shared__ R u[NSUB][NS][NFP];
     const StageParams &P = M.stage;
     const Law<DIM, LAW, R> law(P);
-    // one workgroup per non-conforming (cell, face) pair
-    const int64_t pf = M.pairs[blockIdx.x];
-    const int64_t li = pf / NF;
-    const int f = (int)(pf - li * NF);
-    const int64_t e = M.cells[li];
-    const int kind = M.kind[e * NF + f];
-    if (kind == 0)
-        return;  // the whole workgroup (not reached: the pairs list none)
+    // one workgroup per non-conforming (cell, face) pair, its cell, face,
+    // kind and partners in one record
+    const MortarPair &pr = M.pairs[blockIdx.x];
+    const int64_t e = pr.e, li = pr.li;
+    const int f = pr.f, kind = pr.kind;
     const int m = threadIdx.x;
     const bool on = m < NFP;
     const int a = m % NQ, b = m / NQ;  // face node m = a + NQ*b (b = 0 in 2-D)
@@ -152,7 +149,7 @@ __global__ void __launch_bounds__(64) mortar_face_kernel(const MortarParams M)
     int64_t part[NSUB];
 #pragma unroll
     for (int t = 0; t < NSUB; ++t)
-        part[t] = M.partner[4 * (e * NF + f) + t];
+        part[t] = pr.part[t];
     const int node = face_node<DIM, NQ>(f, m);
     if (kind == 1) {
         // this cell is the fine side: the coarse trace on its face f^1
