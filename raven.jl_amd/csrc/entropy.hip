@@ -188,7 +188,11 @@ int launch_real(const RelaxParams &P, hipStream_t st)
 
 // One workgroup per (slot, which): each thread sums its strided share in
 // index order, then a fixed tree over the threads; (sum, compensation) pairs
-// combined by TwoSum throughout.
+// combined by TwoSum throughout.  Element i sits at
+// (i / group) * group_stride + i % group; G is group when it is 1 or 2 (no
+// division), else 0.  Sixteen elements are requested before they are added,
+// in order.
+template <int G>
 __global__ void __launch_bounds__(256) slot_sums_kernel(const double *__restrict__ src, int64_t n,
                                                         int64_t slot_stride, int64_t which_stride,
                                                         int group, int64_t group_stride,
@@ -197,9 +201,28 @@ __global__ void __launch_bounds__(256) slot_sums_kernel(const double *__restrict
     __shared__ double red[2 * 256];
     const int t = threadIdx.x;
     const double *base = src + blockIdx.x * slot_stride + blockIdx.y * which_stride;
+    auto at = [&](int64_t i) {
+        if constexpr (G == 1)
+            return base[i * group_stride];
+        else if constexpr (G == 2)
+            return base[(i >> 1) * group_stride + (i & 1)];
+        else
+            return base[(i / group) * group_stride + i % group];
+    };
+    constexpr int U = 16;
     double s = 0.0, c = 0.0;
-    for (int64_t i = t; i < n; i += 256)
-        comp_add(s, c, base[(i / group) * group_stride + i % group], 0.0);
+    int64_t i = t;
+    for (; i + (U - 1) * 256 < n; i += U * 256) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            v[u] = at(i + u * 256);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            comp_add(s, c, v[u], 0.0);
+    }
+    for (; i < n; i += 256)
+        comp_add(s, c, at(i), 0.0);
     red[2 * t] = s;
     red[2 * t + 1] = c;
     __syncthreads();
@@ -222,9 +245,18 @@ int launch_slot_sums(const double *src, int64_t n, int nslots, int nwhich, int64
 {
     if (nslots <= 0 || nwhich <= 0)
         return 0;
-    hipLaunchKernelGGL(slot_sums_kernel, dim3((unsigned)nslots, (unsigned)nwhich), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), src, n, slot_stride, which_stride,
-                       group < 1 ? 1 : group, group_stride, out);
+    const dim3 grid((unsigned)nslots, (unsigned)nwhich);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    group = group < 1 ? 1 : group;
+    if (group == 1)
+        hipLaunchKernelGGL(slot_sums_kernel<1>, grid, dim3(256), 0, st, src, n, slot_stride,
+                           which_stride, group, group_stride, out);
+    else if (group == 2)
+        hipLaunchKernelGGL(slot_sums_kernel<2>, grid, dim3(256), 0, st, src, n, slot_stride,
+                           which_stride, group, group_stride, out);
+    else
+        hipLaunchKernelGGL(slot_sums_kernel<0>, grid, dim3(256), 0, st, src, n, slot_stride,
+                           which_stride, group, group_stride, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
