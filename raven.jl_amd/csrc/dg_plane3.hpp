@@ -625,17 +625,46 @@ __device__ __attribute__((always_inline)) inline void plane3_wave(const StagePar
         if constexpr (RX) {
             const R *dqe = static_cast<const R *>(P.dq) + e * (NS * NP) + l;
             const double *mje = P.mj + e * NP + l;
-#pragma unroll
-            for (int k = 0; k < NQ; ++k) {
-                const R mji = mjiv[k];
-                // dq_old and MJ of the plane, requested before its sums
-                R dcur[NS];
+            // dq_old and MJ of a plane are requested before its sums; affine
+            // cells request them one plane ahead (general cells have no
+            // registers to spare for that)
+            constexpr bool AHEAD = GEO == GEO_AFFINE;
+            R dnext[NS];
+            double mjnext = 0.0;
+            if constexpr (AHEAD) {
+                mjnext = mje[0];
                 if (P.read_dq) {
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
-                        dcur[s] = load_dq<ST>(dqe + s * NP + k * NQ2);
+                        dnext[s] = load_dq<ST>(dqe + s * NP);
                 }
-                const double mjk = mje[k * NQ2];
+            }
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) {
+                const R mji = mjiv[k];
+                R dcur[NS];
+                double mjk;
+                if constexpr (AHEAD) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        dcur[s] = dnext[s];
+                    mjk = mjnext;
+                    if (k + 1 < NQ) {
+                        if (P.read_dq) {
+#pragma unroll
+                            for (int s = 0; s < NS; ++s)
+                                dnext[s] = load_dq<ST>(dqe + s * NP + (k + 1) * NQ2);
+                        }
+                        mjnext = mje[(k + 1) * NQ2];
+                    }
+                } else {
+                    if (P.read_dq) {
+#pragma unroll
+                        for (int s = 0; s < NS; ++s)
+                            dcur[s] = load_dq<ST>(dqe + s * NP + k * NQ2);
+                    }
+                    mjk = mje[k * NQ2];
+                }
                 R kv[NS];
                 double x[NS], kk[NS];
 #pragma unroll
